@@ -1,0 +1,228 @@
+/*
+ * ngp_hip.h — C ABI of libngp_hip.so, the gfx950 kernels for the NeRF hot path
+ * (multiresolution hash encoder, occupancy-grid ray marcher, volume integration,
+ * inference renderer and the fused NeRF MLP).
+ *
+ * Every compute entry point has the legacy XLA custom-call signature
+ *
+ *     void target(stream, void **buffers, const char *opaque, size_t opaque_len)
+ *
+ * `buffers` lists the operands in call order followed by the results in result
+ * order; every buffer is device memory owned by the caller.  `opaque` is the raw
+ * bytes of the target's descriptor struct below and must be exactly
+ * sizeof(descriptor) long.  Targets never throw: on a bad descriptor, a null
+ * buffer or a failed launch they record an error that ngp_get_last_error()
+ * returns (and clears) on the calling thread, and launch nothing.
+ *
+ * Work is enqueued asynchronously on `stream` (a hipStream_t; NULL is the legacy
+ * default stream).  No target synchronises the stream.
+ */
+#ifndef NGP_HIP_H
+#define NGP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A hipStream_t, passed as an opaque pointer so this header needs no HIP headers. */
+typedef void *ngp_stream_t;
+
+/* ---------------------------------------------------------------------------
+ * Descriptors.  Field order, types and sizes follow the reference's
+ * deps/volume-rendering-jax/lib/impl/volrend.h:23-118 so that the opaque bytes a
+ * reference-side binding builds are accepted unchanged.
+ * ------------------------------------------------------------------------- */
+
+/* volrend.h:23-29 */
+typedef struct IntegratingDescriptor {
+    uint32_t n_rays;
+    uint32_t total_samples;
+} IntegratingDescriptor;
+
+/* volrend.h:32-42 */
+typedef struct IntegratingBackwardDescriptor {
+    uint32_t n_rays;
+    uint32_t total_samples;
+    float near_distance;
+} IntegratingBackwardDescriptor;
+
+/* volrend.h:45-54 */
+typedef struct IntegratingInferenceDescriptor {
+    uint32_t n_total_rays;
+    uint32_t n_rays;
+    uint32_t march_steps_cap;
+} IntegratingInferenceDescriptor;
+
+/* volrend.h:57-86 */
+typedef struct MarchingDescriptor {
+    uint32_t n_rays;
+    uint32_t total_samples;
+    uint32_t diagonal_n_steps;
+    uint32_t K;
+    uint32_t G;
+    float bound;
+    float stepsize_portion;
+} MarchingDescriptor;
+
+/* volrend.h:89-108 */
+typedef struct MarchingInferenceDescriptor {
+    uint32_t n_total_rays;
+    uint32_t n_rays;
+    uint32_t diagonal_n_steps;
+    uint32_t K;
+    uint32_t G;
+    uint32_t march_steps_cap;
+    float bound;
+    float stepsize_portion;
+} MarchingInferenceDescriptor;
+
+/* volrend.h:110-113 */
+typedef struct Morton3DDescriptor {
+    uint32_t length;
+} Morton3DDescriptor;
+
+/* volrend.h:116-118 */
+typedef struct PackbitsDescriptor {
+    uint32_t n_bytes;
+} PackbitsDescriptor;
+
+/* Hash grid (no native counterpart in the reference: models/encoders.py:57-256 is
+ * pure JAX).  Level scales are computed on the host in float64 and rounded to f32
+ * (encoders.py:91, 216); resolutions and row offsets come from the same host loop
+ * (encoders.py:89-103).  They are never recomputed on the device. */
+#define NGP_HASHGRID_MAX_LEVELS 32
+#define NGP_DTYPE_F32 0u
+#define NGP_DTYPE_F16 1u
+typedef struct HashGridDescriptor {
+    uint32_t n_points;          /* N */
+    uint32_t L;                 /* levels, 1..NGP_HASHGRID_MAX_LEVELS */
+    uint32_t F;                 /* features per entry: 1, 2, 4 or 8 */
+    uint32_t T;                 /* max entries per level */
+    uint32_t first_hash_level;  /* levels [0, first_hash_level) are dense */
+    uint32_t table_dtype;       /* NGP_DTYPE_F32 or NGP_DTYPE_F16 */
+    uint32_t n_rows;            /* offsets[L], rows of the table */
+    float bound;                /* positions are normalised as (pos + bound) / (2 bound) */
+    float scales[NGP_HASHGRID_MAX_LEVELS];
+    uint32_t resolutions[NGP_HASHGRID_MAX_LEVELS];
+    uint32_t offsets[NGP_HASHGRID_MAX_LEVELS + 1];
+} HashGridDescriptor;
+
+/* Fused NeRF MLP of models/nerfs.py:39-86 with make_nerf_ngp's shapes
+ * (nerfs.py:422-454): density MLP 32->64->16, SH degree 4 of the direction,
+ * colour MLP (16+16)->64->64->3, no biases, ReLU hidden layers, trunc_exp density
+ * and sigmoid colour.  The parameters are one flat f32 array laid out as the flax
+ * Dense kernels ([in, out], row-major) in this order:
+ *   density_mlp/Dense_0 [32,64], density_mlp/Dense_1 [64,16],
+ *   rgb_mlp/Dense_0 [32,64], rgb_mlp/Dense_1 [64,64], rgb_mlp/Dense_2 [64,3]. */
+#define NGP_MLP_N_PARAMS 9408u
+typedef struct NerfMlpDescriptor {
+    uint32_t n_samples;
+    uint32_t n_workspace_blocks;  /* rows of the backward workspace (0 for forward) */
+} NerfMlpDescriptor;
+
+/* ---------------------------------------------------------------------------
+ * Entry points replacing the reference's custom-call targets
+ * (deps/volume-rendering-jax/lib/ffi.cc:25-51, volrend.h:121-175).
+ * ------------------------------------------------------------------------- */
+
+/* marching.cu:435-522.  opaque: MarchingDescriptor.
+ * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n],
+ *      noises f32[n], occupancy_bitfield u8[K*G^3/8]
+ * out: next_sample_write_location u32[1], number_of_exceeded_samples u32[1],
+ *      ray_is_valid bool[n], rays_n_samples u32[n], rays_sample_startidx u32[n],
+ *      idcs u32[S], xyzs f32[S,3], dirs f32[S,3], dss f32[S], z_vals f32[S]
+ * Sample ranges are assigned by an exclusive prefix sum in ray order (the
+ * reference assigns them with atomicAdd in scheduling order). */
+void march_rays(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* marching.cu:524-604.  opaque: MarchingInferenceDescriptor.
+ * in : rays_o f32[N,3], rays_d f32[N,3], t_starts f32[N], t_ends f32[N],
+ *      occupancy_bitfield u8[K*G^3/8], next_ray_index_in u32[1],
+ *      terminated bool[n], indices_in u32[n]
+ * out: next_ray_index u32[1], indices_out u32[n], n_samples u32[n],
+ *      t_starts_out f32[n], xyzs f32[n,cap,3], dss f32[n,cap], z_vals f32[n,cap]
+ * Terminated slots receive new rays in slot order. */
+void march_rays_inference(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* marching.cu:606-636.  opaque: Morton3DDescriptor.  in: xyzs u32[len,3]; out: idcs u32[len]. */
+void morton3d(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* marching.cu:638-665.  opaque: Morton3DDescriptor.  in: idcs u32[len]; out: xyzs u32[len,3]. */
+void morton3d_invert(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* packbits.cu:36-72.  opaque: PackbitsDescriptor.
+ * in : density_threshold f32[8*n_bytes], density_grid f32[8*n_bytes]
+ * out: occupied_mask bool[8*n_bytes], occupancy_bitfield u8[n_bytes] */
+void pack_density_into_bits(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* integrating.cu:325-377.  opaque: IntegratingDescriptor.
+ * in : rays_sample_startidx u32[n], rays_n_samples u32[n], bgs f32[n,3],
+ *      dss f32[S], z_vals f32[S], drgbs f32[S,4]
+ * out: measured_batch_size u32[1], final_rgbds f32[n,4], final_opacities f32[n] */
+void integrate_rays(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* integrating.cu:379-446.  opaque: IntegratingBackwardDescriptor.
+ * in : rays_sample_startidx u32[n], rays_n_samples u32[n], bgs f32[n,3],
+ *      dss f32[S], z_vals f32[S], drgbs f32[S,4], final_rgbds f32[n,4],
+ *      final_opacities f32[n], dL_dfinal_rgbds f32[n,4]
+ * out: dL_dbgs f32[n,3], dL_dz_vals f32[S], dL_ddrgbs f32[S,4] */
+void integrate_rays_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* integrating.cu:448-507.  opaque: IntegratingInferenceDescriptor.
+ * in : rays_bg f32[N,3], rays_rgbd f32[N,4], rays_T f32[N], n_samples u32[n],
+ *      indices u32[n], dss f32[n,cap], z_vals f32[n,cap], drgbs f32[n,cap,4]
+ * out: terminate_cnt u32[1], terminated bool[n], rays_rgbd_out f32[n,4],
+ *      rays_T_out f32[n] */
+void integrate_rays_inference(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* ---------------------------------------------------------------------------
+ * Entry points with no native counterpart in the reference (its hash encoder and
+ * MLPs are XLA-generated code).  Same calling convention.
+ * ------------------------------------------------------------------------- */
+
+/* Forward of HashGridEncoder.__call__ (models/encoders.py:82-233).
+ * opaque: HashGridDescriptor.
+ * in : pos f32[N,3], table f32|f16[n_rows,F]
+ * out: encodings f32[N, L*F]   (level-major per point: enc[n, l*F + f]) */
+void hashgrid_encode(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* Table gradient of HashGridEncoder (the VJP of the gather at encoders.py:226 and
+ * of the weighted sum at encoders.py:231).  opaque: HashGridDescriptor.
+ * in : pos f32[N,3], dL_dencodings f32[N, L*F]
+ * out: dL_dtable f32[n_rows, F]   (overwritten, not accumulated into) */
+void hashgrid_encode_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* Forward of NeRF.__call__ after the position encoder (models/nerfs.py:66-86).
+ * opaque: NerfMlpDescriptor.
+ * in : encodings f32[S,32], dirs f32[S,3], params f32[NGP_MLP_N_PARAMS]
+ * out: drgbs f32[S,4]   (density, r, g, b) */
+void nerf_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* Backward of nerf_mlp_forward (JAX autodiff of nerfs.py:66-86 with trunc_exp's
+ * custom VJP, nerfs.py:221-238).  opaque: NerfMlpDescriptor.
+ * in : encodings f32[S,32], dirs f32[S,3], params f32[NGP_MLP_N_PARAMS],
+ *      dL_ddrgbs f32[S,4]
+ * out: dL_dencodings f32[S,32], dL_dparams f32[NGP_MLP_N_PARAMS],
+ *      workspace f32[n_workspace_blocks, NGP_MLP_N_PARAMS] */
+void nerf_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* ---------------------------------------------------------------------------
+ * Status.
+ * ------------------------------------------------------------------------- */
+
+/* Returns 0 when no error was recorded on this thread since the last call,
+ * otherwise a nonzero code; copies a NUL-terminated message into buf (if buf is
+ * non-NULL) and clears the error. */
+int ngp_get_last_error(char *buf, size_t buf_len);
+
+/* Library version string. */
+const char *ngp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGP_HIP_H */

@@ -1,0 +1,170 @@
+"""ctypes binding of libngp_hip.so (the C ABI declared in include/ngp_hip.h).
+
+Every kernel target has the legacy XLA custom-call signature
+``void target(stream, void **buffers, const char *opaque, size_t opaque_len)``;
+this module packs descriptors (byte-identical to the reference's
+deps/volume-rendering-jax/lib/impl/volrend.h:23-118 structs), builds the buffer
+array from torch tensors, calls the target on the current HIP stream and turns a
+recorded error into a Python exception.
+
+There is no fallback: if the library is missing, importing a kernel wrapper
+raises.  Build it with ``python -m jaxngp_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes
+import struct
+from pathlib import Path
+from typing import Sequence
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libngp_hip.so"
+
+TARGETS = (
+    "march_rays",
+    "march_rays_inference",
+    "morton3d",
+    "morton3d_invert",
+    "pack_density_into_bits",
+    "integrate_rays",
+    "integrate_rays_backward",
+    "integrate_rays_inference",
+    "hashgrid_encode",
+    "hashgrid_encode_backward",
+    "nerf_mlp_forward",
+    "nerf_mlp_backward",
+)
+
+HASHGRID_MAX_LEVELS = 32
+DTYPE_F32 = 0
+DTYPE_F16 = 1
+MLP_N_PARAMS = 9408
+
+
+class NGPError(RuntimeError):
+    """An error recorded by libngp_hip.so (bad descriptor, bad buffer, launch failure)."""
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libngp_hip.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP kernels with `python -m jaxngp_amd.build` "
+            "(there is no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.ngp_get_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.ngp_get_last_error.restype = ctypes.c_int
+    lib.ngp_version.argtypes = []
+    lib.ngp_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+_SIG = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]
+_targets: dict = {}
+
+
+def target(name: str):
+    """The C function of kernel target `name`, with its signature set."""
+    fn = _targets.get(name)
+    if fn is None:
+        fn = getattr(load(), name)
+        fn.argtypes = _SIG
+        fn.restype = None
+        _targets[name] = fn
+    return fn
+
+
+def last_error() -> tuple[int, str]:
+    buf = ctypes.create_string_buffer(512)
+    code = load().ngp_get_last_error(buf, len(buf))
+    return code, buf.value.decode()
+
+
+def _check() -> None:
+    code, msg = last_error()
+    if code != 0:
+        raise NGPError(msg)
+
+
+def current_stream(device: torch.device | None = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def call(name: str, tensors: Sequence[torch.Tensor], opaque: bytes, stream: int | None = None) -> None:
+    """Calls target `name` with device tensors (operands then results)."""
+    for i, t in enumerate(tensors):
+        if not t.is_cuda:
+            raise ValueError(f"{name}: buffer {i} is not a device tensor")
+        if not t.is_contiguous():
+            raise ValueError(f"{name}: buffer {i} is not contiguous")
+    ptrs = (ctypes.c_void_p * max(1, len(tensors)))(*[t.data_ptr() for t in tensors])
+    if stream is None:
+        stream = current_stream(tensors[0].device if tensors else None)
+    target(name)(ctypes.c_void_p(stream), ptrs, opaque, len(opaque))
+    _check()
+
+
+def call_raw(name: str, ptrs: Sequence[int], opaque: bytes, stream: int = 0) -> None:
+    """Calls target `name` with raw pointers (host-side error tests use NULLs)."""
+    arr = (ctypes.c_void_p * max(1, len(ptrs)))(*ptrs)
+    target(name)(ctypes.c_void_p(stream), arr, opaque, len(opaque))
+    _check()
+
+
+# ---------------------------------------------------------------------------
+# descriptors (little-endian, natural alignment, no padding: all 4-byte fields)
+# ---------------------------------------------------------------------------
+
+def marching_descriptor(n_rays, total_samples, diagonal_n_steps, K, G, bound, stepsize_portion) -> bytes:
+    return struct.pack("<5I2f", n_rays, total_samples, diagonal_n_steps, K, G, bound, stepsize_portion)
+
+
+def marching_inference_descriptor(n_total_rays, n_rays, diagonal_n_steps, K, G, march_steps_cap, bound,
+                                  stepsize_portion) -> bytes:
+    return struct.pack("<6I2f", n_total_rays, n_rays, diagonal_n_steps, K, G, march_steps_cap, bound,
+                       stepsize_portion)
+
+
+def integrating_descriptor(n_rays, total_samples) -> bytes:
+    return struct.pack("<2I", n_rays, total_samples)
+
+
+def integrating_backward_descriptor(n_rays, total_samples, near_distance) -> bytes:
+    return struct.pack("<2If", n_rays, total_samples, near_distance)
+
+
+def integrating_inference_descriptor(n_total_rays, n_rays, march_steps_cap) -> bytes:
+    return struct.pack("<3I", n_total_rays, n_rays, march_steps_cap)
+
+
+def morton3d_descriptor(length) -> bytes:
+    return struct.pack("<I", length)
+
+
+def packbits_descriptor(n_bytes) -> bytes:
+    return struct.pack("<I", n_bytes)
+
+
+def hashgrid_descriptor(n_points, L, F, T, first_hash_level, table_dtype, n_rows, bound, scales, resolutions,
+                        offsets) -> bytes:
+    M = HASHGRID_MAX_LEVELS
+    if not (1 <= L <= M):
+        raise ValueError(f"L={L} must be in [1, {M}]")
+    sc = list(scales) + [0.0] * (M - L)
+    rs = list(resolutions) + [0] * (M - L)
+    of = list(offsets) + [offsets[-1]] * (M + 1 - len(offsets))
+    return struct.pack(f"<7If{M}f{M}I{M + 1}I", n_points, L, F, T, first_hash_level, table_dtype, n_rows, bound,
+                       *sc, *rs, *of)
+
+
+def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0) -> bytes:
+    return struct.pack("<2I", n_samples, n_workspace_blocks)

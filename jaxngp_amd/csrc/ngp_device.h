@@ -1,0 +1,126 @@
+// ngp_device.h — device helpers of the occupancy-grid marcher.
+//
+// Arithmetic follows one convention shared with the CPU oracle: f32, every
+// operation rounded on its own (sources built with -ffp-contract=off), IEEE
+// division.  Cited reference lines are in
+// deps/volume-rendering-jax/lib/impl/marching.cu.
+#pragma once
+
+#include "ngp_common.h"
+
+namespace ngp {
+
+constexpr float kSqrt3 = 1.7320508075688772f;  // volrend.h:18
+constexpr float kTThreshold = 1e-4f;           // integrating.cu:12
+
+// 10-bit bit interleave by multiply-and-mask (marching.cu:52-68).
+__device__ __forceinline__ uint32_t morton_spread(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    return morton_spread(x) | (morton_spread(y) << 1) | (morton_spread(z) << 2);
+}
+// Keep every third bit and compact (marching.cu:70-77).
+__device__ __forceinline__ uint32_t morton_compact(uint32_t x) {
+    x = x & 0x49249249u;
+    x = (x | (x >> 2)) & 0xc30c30c3u;
+    x = (x | (x >> 4)) & 0x0f00f00fu;
+    x = (x | (x >> 8)) & 0xff0000ffu;
+    x = (x | (x >> 16)) & 0x0000ffffu;
+    return x;
+}
+
+struct MarchParams {
+    uint32_t diag, K, G, G3;
+    float bound, portion, inv_G, ds_lo, ds_hi, sample_limit;
+};
+
+inline MarchParams make_march_params(uint32_t diag, uint32_t K, uint32_t G, float bound, float portion) {
+    MarchParams p;
+    p.diag = diag;
+    p.K = K;
+    p.G = G;
+    p.G3 = G * G * G;
+    p.bound = bound;
+    p.portion = portion;
+    p.inv_G = 1.0f / (float)G;
+    // marching.cu:18-22
+    p.ds_lo = ((2.0f * kSqrt3) * (bound < 1.0f ? bound : 1.0f)) / (float)diag;
+    p.ds_hi = ((2.0f * kSqrt3) * bound) * p.inv_G;
+    p.sample_limit = (float)diag * bound;  // marching.cu:165
+    return p;
+}
+
+// marching.cu:15-23
+__device__ __forceinline__ float step_size(const MarchParams &p, float t) {
+    return fminf(fmaxf(t * p.portion, p.ds_lo), p.ds_hi);
+}
+
+__device__ __forceinline__ int frexp_exp(float x) {
+    int e;
+    frexpf(x, &e);
+    return e;
+}
+
+__device__ __forceinline__ uint32_t clamp_cascade(int e, uint32_t K) {
+    int hi = (int)K - 1;
+    return (uint32_t)(e < 0 ? 0 : (e > hi ? hi : e));
+}
+
+// Occupancy lookup of one position (marching.cu:79-98 and 170-176).  Fills the
+// float grid coordinate and the mip bound used by the empty-cell skip.
+__device__ __forceinline__ bool lookup_cell(const MarchParams &p, const uint8_t *__restrict__ bits, float px, float py,
+                                            float pz, float ds, float &gx, float &gy, float &gz, float &mip_bound) {
+    uint32_t cascade = 0;
+    if (p.K > 1) {
+        float m = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
+        uint32_t c1 = clamp_cascade(frexp_exp(m), p.K);
+        uint32_t c2 = clamp_cascade(frexp_exp(ds * (float)p.G), p.K);
+        cascade = c1 > c2 ? c1 : c2;
+    }
+    mip_bound = fminf(__uint_as_float((cascade + 127u) << 23), p.bound);
+    const float G = (float)p.G;
+    gx = ((px / mip_bound + 1.0f) * 0.5f) * G;
+    gy = ((py / mip_bound + 1.0f) * 0.5f) * G;
+    gz = ((pz / mip_bound + 1.0f) * 0.5f) * G;
+    const float gmax = (float)(p.G - 1);
+    // clamp in float: identical to clampi((int)floorf(.), 0, G-1) for every input
+    uint32_t ix = (uint32_t)fminf(fmaxf(floorf(gx), 0.0f), gmax);
+    uint32_t iy = (uint32_t)fminf(fmaxf(floorf(gy), 0.0f), gmax);
+    uint32_t iz = (uint32_t)fminf(fmaxf(floorf(gz), 0.0f), gmax);
+    uint32_t idx = cascade * p.G3 + morton3(ix, iy, iz);
+    return (bits[idx >> 3] >> (idx & 7u)) & 1u;
+}
+
+// Exit distance of the current cell along the ray (marching.cu:182-185).
+__device__ __forceinline__ float cell_exit_t(const MarchParams &p, float t, float px, float py, float pz, float gx,
+                                             float gy, float gz, float mip_bound, float dx, float dy, float dz,
+                                             float idx_, float idy, float idz) {
+    float nx = floorf((gx + 0.5f) + 0.5f * copysignf(1.0f, dx));
+    float ny = floorf((gy + 0.5f) + 0.5f * copysignf(1.0f, dy));
+    float nz = floorf((gz + 0.5f) + 0.5f * copysignf(1.0f, dz));
+    float ex = ((((nx * p.inv_G) - 0.5f) * 2.0f) * mip_bound - px) * idx_;
+    float ey = ((((ny * p.inv_G) - 0.5f) * 2.0f) * mip_bound - py) * idy;
+    float ez = ((((nz * p.inv_G) - 0.5f) * 2.0f) * mip_bound - pz) * idz;
+    return t + fmaxf(0.0f, fminf(ex, fminf(ey, ez)));
+}
+
+// Steps by ds from t until past next_t (marching.cu:186-189).  A step that no
+// longer changes t (where the reference would loop forever) ends the march:
+// returns +inf, so every loop here terminates.
+__device__ __forceinline__ float skip_to(const MarchParams &p, float t, float ds, float next_t) {
+    float nt = t + ds;
+    if (nt == t) return __builtin_huge_valf();
+    while (nt < next_t) {
+        float s = nt + step_size(p, nt);
+        if (s == nt) return __builtin_huge_valf();
+        nt = s;
+    }
+    return nt;
+}
+
+}  // namespace ngp

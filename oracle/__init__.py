@@ -1,0 +1,419 @@
+"""CPU oracle of the NeRF hot path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline; the product
+package (jaxngp_amd) never imports it.
+
+Two layers:
+  * ``liboracle.so`` (oracle.c): an f32 restatement of the reference kernels under
+    a fixed rounding convention (no contraction, IEEE division, a deterministic
+    exp), processing rays in index order.  The HIP kernels follow the same
+    convention, so integer outputs (sample counts, offsets, cells, early-stop
+    indices, hash indices) and most f32 outputs match bit for bit.
+  * numpy float64 transcriptions (this file) of the floating-point parts, used to
+    bound the error of the f32 results: integration forward/backward, the hash
+    encoder (vertices from the f32 path, weights and sums in f64), the NeRF MLP.
+
+Each function cites the reference lines it restates.  How each is pinned
+independently of the reference (which cannot run here: CUDA-only kernels and a
+JAX-only encoder) is listed in tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB))
+        _lib.ngp_expf.argtypes = [ctypes.c_float]
+        _lib.ngp_expf.restype = ctypes.c_float
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+u32 = ctypes.c_uint32
+f32 = ctypes.c_float
+
+
+# ---------------------------------------------------------------------------
+# f32 restatement (liboracle.so)
+# ---------------------------------------------------------------------------
+
+def expf(x: float) -> float:
+    return lib().ngp_expf(f32(x))
+
+
+def morton3d(xyzs) -> np.ndarray:
+    xyzs = _c(xyzs, np.uint32)
+    out = np.empty(xyzs.shape[0], np.uint32)
+    lib().oracle_morton3d(u32(xyzs.shape[0]), _p(xyzs), _p(out))
+    return out
+
+
+def morton3d_invert(idcs) -> np.ndarray:
+    idcs = _c(idcs, np.uint32)
+    out = np.empty((idcs.shape[0], 3), np.uint32)
+    lib().oracle_morton3d_invert(u32(idcs.shape[0]), _p(idcs), _p(out))
+    return out
+
+
+def packbits(thr, density):
+    density = _c(density, np.float32)
+    thr = _c(np.broadcast_to(np.asarray(thr, np.float32), density.shape), np.float32)
+    n_bytes = density.shape[0] // 8
+    mask = np.empty(density.shape[0], np.uint8)
+    bits = np.empty(n_bytes, np.uint8)
+    lib().oracle_packbits(u32(n_bytes), _p(thr), _p(density), _p(mask), _p(bits))
+    return mask.astype(bool), bits
+
+
+def march_rays(total_samples, diagonal_n_steps, K, G, bound, stepsize_portion, rays_o, rays_d, t_starts, t_ends,
+               noises, occupancy_bitfield):
+    """Sequential (ray-order) restatement of marching.cu:101-268 + marching/__init__.py:70-93."""
+    rays_o = _c(rays_o, np.float32)
+    rays_d = _c(rays_d, np.float32)
+    n = rays_o.shape[0]
+    t_starts = _c(t_starts, np.float32)
+    t_ends = _c(t_ends, np.float32)
+    noises = _c(np.broadcast_to(np.asarray(noises, np.float32), (n,)), np.float32)
+    bits = _c(occupancy_bitfield, np.uint8)
+    S = int(total_samples)
+    nxt = np.zeros(1, np.uint32)
+    exc = np.zeros(1, np.uint32)
+    valid = np.zeros(n, np.uint8)
+    nsamp = np.zeros(n, np.uint32)
+    start = np.zeros(n, np.uint32)
+    idcs = np.zeros(S, np.uint32)
+    xyzs = np.zeros((S, 3), np.float32)
+    dirs = np.zeros((S, 3), np.float32)
+    dss = np.zeros(S, np.float32)
+    z = np.zeros(S, np.float32)
+    lib().oracle_march_rays(u32(n), u32(S), u32(diagonal_n_steps), u32(K), u32(G), f32(bound), f32(stepsize_portion),
+                            _p(rays_o), _p(rays_d), _p(t_starts), _p(t_ends), _p(noises), _p(bits), _p(nxt), _p(exc),
+                            _p(valid), _p(nsamp), _p(start), _p(idcs), _p(xyzs), _p(dirs), _p(dss), _p(z))
+    measured = np.uint32(int(nxt[0]) - int(exc[0]))
+    return dict(measured=measured, next_sample_write_location=nxt[0], number_of_exceeded_samples=exc[0],
+                ray_is_valid=valid.astype(bool), rays_n_samples=nsamp, rays_sample_startidx=start, idcs=idcs,
+                xyzs=xyzs, dirs=dirs, dss=dss, z_vals=z)
+
+
+def march_rays_inference(diagonal_n_steps, K, G, march_steps_cap, bound, stepsize_portion, rays_o, rays_d, t_starts,
+                         t_ends, occupancy_bitfield, next_ray_index_in, terminated, indices):
+    """Slot-order restatement of marching.cu:271-397; t_starts returned raw (per slot)."""
+    rays_o = _c(rays_o, np.float32)
+    rays_d = _c(rays_d, np.float32)
+    nt = rays_o.shape[0]
+    terminated = _c(terminated, np.uint8)
+    n = terminated.shape[0]
+    cap = int(march_steps_cap)
+    nri = _c(np.asarray(next_ray_index_in).reshape(1), np.uint32)
+    idx_in = _c(indices, np.uint32)
+    nxt = np.zeros(1, np.uint32)
+    idx = np.zeros(n, np.uint32)
+    nsamp = np.zeros(n, np.uint32)
+    tout = np.zeros(n, np.float32)
+    xyzs = np.zeros((n, cap, 3), np.float32)
+    dss = np.zeros((n, cap), np.float32)
+    z = np.zeros((n, cap), np.float32)
+    lib().oracle_march_rays_inference(u32(nt), u32(n), u32(diagonal_n_steps), u32(K), u32(G), u32(cap), f32(bound),
+                                      f32(stepsize_portion), _p(rays_o), _p(rays_d), _p(_c(t_starts, np.float32)),
+                                      _p(_c(t_ends, np.float32)), _p(_c(occupancy_bitfield, np.uint8)), _p(nri),
+                                      _p(terminated), _p(idx_in), _p(nxt), _p(idx), _p(nsamp), _p(tout), _p(xyzs),
+                                      _p(dss), _p(z))
+    return dict(next_ray_index=nxt[0], indices=idx, n_samples=nsamp, t_starts_out=tout, xyzs=xyzs, dss=dss, z_vals=z)
+
+
+def integrate_rays(startidx, nsamp, bgs, dss, z_vals, drgbs):
+    """f32 restatement of integrating.cu:24-104."""
+    startidx = _c(startidx, np.uint32)
+    nsamp = _c(nsamp, np.uint32)
+    n = startidx.shape[0]
+    bgs = _c(np.broadcast_to(np.asarray(bgs, np.float32), (n, 3)), np.float32)
+    dss = _c(dss, np.float32)
+    S = dss.shape[0]
+    measured = np.zeros(1, np.uint32)
+    rgbd = np.zeros((n, 4), np.float32)
+    op = np.zeros(n, np.float32)
+    lib().oracle_integrate_rays(u32(n), u32(S), _p(startidx), _p(nsamp), _p(bgs), _p(dss),
+                                _p(_c(z_vals, np.float32)), _p(_c(drgbs, np.float32)), _p(measured), _p(rgbd), _p(op))
+    return measured[0], rgbd, op
+
+
+def integrate_rays_backward(near_distance, startidx, nsamp, bgs, dss, z_vals, drgbs, final_rgbds, final_opacities,
+                            dL_dfinal_rgbds):
+    """f32 restatement of integrating.cu:106-240."""
+    startidx = _c(startidx, np.uint32)
+    nsamp = _c(nsamp, np.uint32)
+    n = startidx.shape[0]
+    bgs = _c(np.broadcast_to(np.asarray(bgs, np.float32), (n, 3)), np.float32)
+    dss = _c(dss, np.float32)
+    S = dss.shape[0]
+    dbg = np.zeros((n, 3), np.float32)
+    dz = np.zeros(S, np.float32)
+    dd = np.zeros((S, 4), np.float32)
+    lib().oracle_integrate_rays_backward(u32(n), u32(S), f32(near_distance), _p(startidx), _p(nsamp), _p(bgs),
+                                         _p(dss), _p(_c(z_vals, np.float32)), _p(_c(drgbs, np.float32)),
+                                         _p(_c(final_rgbds, np.float32)), _p(_c(final_opacities, np.float32)),
+                                         _p(_c(dL_dfinal_rgbds, np.float32)), _p(dbg), _p(dz), _p(dd))
+    return dbg, dz, dd
+
+
+def integrate_rays_inference(rays_bg, rays_rgbd, rays_T, n_samples, indices, dss, z_vals, drgbs):
+    """f32 restatement of integrating.cu:242-322 (per-slot outputs, no scatter)."""
+    rays_rgbd = _c(rays_rgbd, np.float32)
+    ntot = rays_rgbd.shape[0]
+    dss = _c(dss, np.float32)
+    n, cap = dss.shape
+    cnt = np.zeros(1, np.uint32)
+    term = np.zeros(n, np.uint8)
+    rgbd = np.zeros((n, 4), np.float32)
+    T = np.zeros(n, np.float32)
+    lib().oracle_integrate_rays_inference(u32(ntot), u32(n), u32(cap), _p(_c(rays_bg, np.float32)), _p(rays_rgbd),
+                                          _p(_c(rays_T, np.float32)), _p(_c(n_samples, np.uint32)),
+                                          _p(_c(indices, np.uint32)), _p(dss), _p(_c(z_vals, np.float32)),
+                                          _p(_c(drgbs, np.float32)), _p(cnt), _p(term), _p(rgbd), _p(T))
+    return cnt[0], term.astype(bool), rgbd, T
+
+
+# ---------------------------------------------------------------------------
+# hash grid
+# ---------------------------------------------------------------------------
+
+def hashgrid_levels(L: int, T: int, N_min: int, N_max: int, dim: int = 3):
+    """Level table of HashGridEncoder (models/encoders.py:76-103, utils/common.py:336-337):
+    float64 b and scales, integer resolutions, 8-aligned dense sizes, row offsets."""
+    b = math.exp((math.log(N_max) - math.log(N_min)) / (L - 1))
+    scales, res, offsets, first_hash = [], [], [0], 0
+    for i in range(L):
+        s = N_min * (b ** i) - 1
+        r = math.ceil(s) + 1
+        n = (r ** dim + 7) // 8 * 8
+        if n <= T:
+            first_hash += 1
+        else:
+            n = T
+        scales.append(s)
+        res.append(r)
+        offsets.append(offsets[-1] + n)
+    return dict(b=b, scales=np.asarray(scales, np.float64), scales_f32=np.asarray(scales, np.float32),
+                resolutions=np.asarray(res, np.uint32), offsets=np.asarray(offsets, np.int64),
+                first_hash_level=first_hash, n_rows=offsets[-1])
+
+
+def hashgrid_vertices(pos, bound, lv, T):
+    """f32 vertex indices [N, L, 8] (int64) and f32 fractional parts [N, L, 3] of the
+    encoder (encoders.py:87, 116-123, 134-189, 216-218).  Corner c = (cx, cy, cz) with
+    cz the fastest bit."""
+    pos = np.asarray(pos, np.float32)
+    b32 = np.float32(bound)
+    p = (pos + b32) / (np.float32(2) * b32)  # [N, 3]
+    L = len(lv["scales_f32"])
+    ps = p[:, None, :] * lv["scales_f32"][None, :, None] + np.float32(0.5)  # [N, L, 3] f32
+    fl = np.floor(ps)
+    frac = ps - fl
+    corners = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)], np.float32)
+    vert = (fl[:, :, None, :] + corners[None, None]).astype(np.uint64)  # [N, L, 8, 3]
+    res = lv["resolutions"].astype(np.uint64)[None, :, None]
+    M = np.uint64(0xFFFFFFFF)
+    dense = (vert[..., 0] + vert[..., 1] * res + vert[..., 2] * ((res * res) & M)) & M
+    hashed = (vert[..., 0] ^ ((vert[..., 1] * np.uint64(2654435761)) & M) ^ ((vert[..., 2] * np.uint64(805459861)) & M)) & M
+    lvl = np.arange(L)[None, :, None]
+    idx = np.where(lvl < lv["first_hash_level"], dense, hashed) % np.uint64(T)
+    idx = idx.astype(np.int64) + lv["offsets"][:-1][None, :, None]
+    idx = np.minimum(idx, lv["n_rows"] - 1)
+    return idx, frac
+
+
+def hashgrid_weights64(frac):
+    """float64 trilinear weights [N, L, 8] from f32 fractional parts (encoders.py:191-213)."""
+    f = frac.astype(np.float64)
+    corners = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)], np.float64)
+    w = np.where(corners[None, None] == 1, f[:, :, None, :], 1.0 - f[:, :, None, :])
+    return w.prod(axis=-1)
+
+
+def hashgrid_forward64(pos, table, bound, lv, T):
+    """Encodings [N, L*F] in float64 (encoders.py:226-233)."""
+    idx, frac = hashgrid_vertices(pos, bound, lv, T)
+    w = hashgrid_weights64(frac)
+    tab = np.asarray(table, np.float64)
+    enc = (tab[idx] * w[..., None]).sum(axis=2)  # [N, L, F]
+    return enc.reshape(enc.shape[0], -1)
+
+
+def hashgrid_backward64(pos, g, bound, lv, T, F):
+    """Table gradient [n_rows, F] in float64: scatter-add of w * g."""
+    idx, frac = hashgrid_vertices(pos, bound, lv, T)
+    w = hashgrid_weights64(frac)
+    N, L = idx.shape[:2]
+    g = np.asarray(g, np.float64).reshape(N, L, F)
+    contrib = w[..., None] * g[:, :, None, :]  # [N, L, 8, F]
+    out = np.zeros((lv["n_rows"], F), np.float64)
+    for f in range(F):
+        np.add.at(out[:, f], idx.reshape(-1), contrib[..., f].reshape(-1))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# integration in float64
+# ---------------------------------------------------------------------------
+
+T_THRESHOLD = 1e-4
+
+
+def integrate_rays64(startidx, nsamp, bgs, dss, z_vals, drgbs, stop_at=None):
+    """float64 transcription of integrating.cu:24-104.  `stop_at` (per ray) forces the
+    early-stop index (e.g. the f32 path's), so values can be compared at the same
+    composited prefix; None stops where the float64 transmittance crosses 1e-4."""
+    n = len(startidx)
+    bgs = np.broadcast_to(np.asarray(bgs, np.float64), (n, 3))
+    rgbd = np.zeros((n, 4))
+    op = np.zeros(n)
+    stops = np.zeros(n, np.int64)
+    for i in range(n):
+        s0, m = int(startidx[i]), int(nsamp[i])
+        T, acc, k = 1.0, np.zeros(4), 0
+        while k < m and (T > T_THRESHOLD if stop_at is None else k < stop_at[i]):
+            s = s0 + k
+            a = 1.0 - math.exp(-float(drgbs[s, 0]) * float(dss[s]))
+            w = T * a
+            acc += w * np.array([drgbs[s, 1], drgbs[s, 2], drgbs[s, 3], z_vals[s]], np.float64)
+            T *= 1.0 - a
+            k += 1
+        stops[i] = k
+        op[i] = 1.0 - T
+        if T <= T_THRESHOLD:
+            rgbd[i] = acc / (1.0 - T)
+        else:
+            rgbd[i, :3] = acc[:3] + T * bgs[i]
+            rgbd[i, 3] = acc[3]
+    return rgbd, op, stops
+
+
+def integrate_rays_backward64(near_distance, startidx, nsamp, bgs, dss, z_vals, drgbs, final_rgbds, final_opacities,
+                              g, stop_at=None):
+    """float64 transcription of the reference VJP (integrating.cu:158-239)."""
+    n = len(startidx)
+    bgs = np.broadcast_to(np.asarray(bgs, np.float64), (n, 3))
+    S = len(dss)
+    dbg = np.zeros((n, 3))
+    dz = np.zeros(S)
+    dd = np.zeros((S, 4))
+    for i in range(n):
+        s0, m = int(startidx[i]), int(nsamp[i])
+        R = np.asarray(final_rgbds[i], np.float64)
+        gr = np.asarray(g[i], np.float64)
+        op = float(final_opacities[i])
+        term = op >= 1.0 - T_THRESHOLD
+        bgk = 0.0 if term else 1.0 - op
+        T, cur, k = 1.0, np.zeros(4), 0
+        while k < m and (T > T_THRESHOLD if stop_at is None else k < stop_at[i]):
+            s = s0 + k
+            sigma, ds, z = float(drgbs[s, 0]), float(dss[s]), float(z_vals[s])
+            a = 1.0 - math.exp(-sigma * ds)
+            w = T * a
+            c = np.array([drgbs[s, 1], drgbs[s, 2], drgbs[s, 3]], np.float64)
+            cur[:3] += w * c
+            cur[3] += w * z
+            T *= 1.0 - a
+            dz[s] = w * gr[3]
+            dsig = ds * (np.sum(gr[:3] * (T * c - (R[:3] - cur[:3]) - bgs[i] * bgk)) + gr[3] * (T * z - (R[3] - cur[3])))
+            reg = 1e-4 if (sigma > 4e-5 and z < near_distance) else 0.0
+            dd[s, 0] = min(z * z, 1.0) * dsig + reg
+            dd[s, 1:] = w * gr[:3]
+            k += 1
+        if T > T_THRESHOLD:
+            dbg[i] = T * gr[:3]
+    return dbg, dz, dd
+
+
+# ---------------------------------------------------------------------------
+# NeRF MLP in float64 (models/nerfs.py:39-128, 216-238; encoders.py:365-406)
+# ---------------------------------------------------------------------------
+
+MLP_SHAPES = [("density_mlp/Dense_0", (32, 64)), ("density_mlp/Dense_1", (64, 16)),
+              ("rgb_mlp/Dense_0", (32, 64)), ("rgb_mlp/Dense_1", (64, 64)), ("rgb_mlp/Dense_2", (64, 3))]
+
+
+def unpack_mlp(params):
+    params = np.asarray(params, np.float64)
+    out, o = [], 0
+    for _, (a, b) in MLP_SHAPES:
+        out.append(params[o:o + a * b].reshape(a, b))
+        o += a * b
+    return out
+
+
+def sh4_64(d):
+    """Real spherical harmonics up to degree 4 (16 values), encoders.py:384-405."""
+    d = np.asarray(d, np.float64)
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    xy, xz, yz, x2, y2, z2 = x * y, x * z, y * z, x * x, y * y, z * z
+    return np.stack([
+        np.full_like(x, 0.28209479177387814), -0.48860251190291987 * y, 0.48860251190291987 * z,
+        -0.48860251190291987 * x, 1.0925484305920792 * xy, -1.0925484305920792 * yz,
+        0.94617469575755997 * z2 - 0.31539156525251999, -1.0925484305920792 * xz,
+        0.54627421529603959 * x2 - 0.54627421529603959 * y2, 0.59004358992664352 * y * (-3.0 * x2 + y2),
+        2.8906114426405538 * xy * z, 0.45704579946446572 * y * (1.0 - 5.0 * z2),
+        0.3731763325901154 * z * (5.0 * z2 - 3.0), 0.45704579946446572 * x * (1.0 - 5.0 * z2),
+        1.4453057213202769 * z * (x2 - y2), 0.59004358992664352 * x * (-x2 + 3.0 * y2)], axis=-1)
+
+
+def mlp_forward64(enc, dirs, params):
+    """drgbs [S, 4] (density, rgb) and the intermediates, float64."""
+    W1, W2, W3, W4, W5 = unpack_mlp(params)
+    X = np.asarray(enc, np.float64)
+    h1 = np.maximum(X @ W1, 0.0)
+    o = h1 @ W2
+    sigma = np.exp(o[:, 0])
+    sh = sh4_64(dirs)
+    cin = np.concatenate([o, sh], axis=1)
+    h2 = np.maximum(cin @ W3, 0.0)
+    h3 = np.maximum(h2 @ W4, 0.0)
+    r = h3 @ W5
+    rgb = 1.0 / (1.0 + np.exp(-r))
+    return np.concatenate([sigma[:, None], rgb], axis=1), dict(X=X, h1=h1, o=o, cin=cin, h2=h2, h3=h3, rgb=rgb)
+
+
+def mlp_backward64(enc, dirs, params, g):
+    """(dL/denc [S,32], dL/dparams [9408]) in float64, with trunc_exp's VJP
+    exp(clip(x, -15, 15)) (nerfs.py:230-233)."""
+    W1, W2, W3, W4, W5 = unpack_mlp(params)
+    _, a = mlp_forward64(enc, dirs, params)
+    g = np.asarray(g, np.float64)
+    dr = g[:, 1:] * a["rgb"] * (1.0 - a["rgb"])
+    dh3 = (dr @ W5.T) * (a["h3"] > 0)
+    dh2 = (dh3 @ W4.T) * (a["h2"] > 0)
+    dcin = dh2 @ W3.T
+    do = dcin[:, :16].copy()
+    do[:, 0] += g[:, 0] * np.exp(np.clip(a["o"][:, 0], -15.0, 15.0))
+    dh1 = (do @ W2.T) * (a["h1"] > 0)
+    dX = dh1 @ W1.T
+    grads = [a["X"].T @ dh1, a["h1"].T @ do, a["cin"].T @ dh2, a["h2"].T @ dh3, a["h3"].T @ dr]
+    return dX, np.concatenate([gw.reshape(-1) for gw in grads])

@@ -1,0 +1,103 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point the
+header declares, descriptor packing matches the C structs, and host-side failures
+are reported through ngp_get_last_error instead of crashing (the reference throws
+C++ exceptions across the XLA boundary, volrend.h:9-16, serde.h:35-41)."""
+import re
+import struct
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from jaxngp_amd import _lib
+
+REPO = Path(__file__).resolve().parents[1]
+HEADER = REPO / "include" / "ngp_hip.h"
+
+
+def _declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:void|int|const char \*)\s*\*?\s*([a-z_0-9]+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _declared_functions()
+    assert "march_rays" in declared and "ngp_get_last_error" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in nm.splitlines() if line.strip()}
+    assert set(declared) <= exported
+    assert set(_lib.TARGETS) <= set(declared)
+
+
+def test_version():
+    assert _lib.load().ngp_version().decode().startswith("ngp-hip")
+
+
+def test_descriptor_sizes_match_c_structs():
+    # sizes of the structs in include/ngp_hip.h (reference volrend.h:23-118 for the first seven)
+    assert len(_lib.integrating_descriptor(1, 2)) == 8
+    assert len(_lib.integrating_backward_descriptor(1, 2, 0.3)) == 12
+    assert len(_lib.integrating_inference_descriptor(1, 2, 3)) == 12
+    assert len(_lib.marching_descriptor(1, 2, 3, 4, 5, 1.0, 0.0)) == 28
+    assert len(_lib.marching_inference_descriptor(1, 2, 3, 4, 5, 6, 1.0, 0.0)) == 32
+    assert len(_lib.morton3d_descriptor(5)) == 4
+    assert len(_lib.packbits_descriptor(5)) == 4
+    assert len(_lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])) == 8 * 4 + 32 * 4 * 2 + 33 * 4
+    assert len(_lib.nerf_mlp_descriptor(5, 0)) == 8
+    # field order: n_rays first, bound at byte 20
+    d = _lib.marching_descriptor(7, 9, 1024, 1, 128, 1.5, 0.25)
+    assert struct.unpack_from("<I", d, 0)[0] == 7 and struct.unpack_from("<f", d, 20)[0] == 1.5
+
+
+def test_compile_header_as_c():
+    """The header is plain C (no HIP types) and its struct sizes are as documented."""
+    src = r'''
+#include "ngp_hip.h"
+_Static_assert(sizeof(MarchingDescriptor) == 28, "");
+_Static_assert(sizeof(MarchingInferenceDescriptor) == 32, "");
+_Static_assert(sizeof(IntegratingDescriptor) == 8, "");
+_Static_assert(sizeof(IntegratingBackwardDescriptor) == 12, "");
+_Static_assert(sizeof(IntegratingInferenceDescriptor) == 12, "");
+_Static_assert(sizeof(PackbitsDescriptor) == 4, "");
+_Static_assert(sizeof(Morton3DDescriptor) == 4, "");
+_Static_assert(sizeof(HashGridDescriptor) == 8 * 4 + 32 * 8 + 33 * 4, "");
+_Static_assert(sizeof(NerfMlpDescriptor) == 8, "");
+int main(void) { return 0; }
+'''
+    out = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{REPO / 'include'}", "-x", "c", "-", "-o",
+                          "/dev/null"], input=src, capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+
+
+@pytest.mark.parametrize("target", _lib.TARGETS)
+def test_wrong_descriptor_size_is_an_error(target):
+    with pytest.raises(_lib.NGPError, match="opaque"):
+        _lib.call_raw(target, [0] * 16, b"\x00\x00\x00")
+
+
+def test_null_buffer_is_an_error():
+    with pytest.raises(_lib.NGPError, match="NULL"):
+        _lib.call_raw("march_rays", [0] * 16, _lib.marching_descriptor(4, 16, 1024, 1, 128, 1.0, 0.0))
+    with pytest.raises(_lib.NGPError, match="NULL"):
+        _lib.call_raw("integrate_rays_backward", [0] * 12, _lib.integrating_backward_descriptor(4, 16, 0.3))
+
+
+def test_bad_statics_are_errors():
+    with pytest.raises(_lib.NGPError, match="statics"):
+        _lib.call_raw("march_rays", [0] * 16, _lib.marching_descriptor(4, 16, 1024, 0, 128, 1.0, 0.0))
+    with pytest.raises(_lib.NGPError, match="statics"):
+        _lib.call_raw("march_rays", [0] * 16, _lib.marching_descriptor(4, 16, 1024, 1, 128, -1.0, 0.0))
+    with pytest.raises(_lib.NGPError, match="total_samples"):
+        _lib.call_raw("march_rays", [0] * 16, _lib.marching_descriptor(4, 0, 1024, 1, 128, 1.0, 0.0))
+    with pytest.raises(_lib.NGPError, match="march_steps_cap"):
+        _lib.call_raw("march_rays_inference", [0] * 15,
+                      _lib.marching_inference_descriptor(10, 4, 1024, 1, 128, 0, 1.0, 0.0))
+    with pytest.raises(_lib.NGPError, match="F="):
+        _lib.call_raw("hashgrid_encode", [0] * 3, _lib.hashgrid_descriptor(4, 1, 3, 8, 1, 0, 8, 1.0, [1.0], [2], [0, 8]))
+    with pytest.raises(_lib.NGPError, match="level table"):
+        _lib.call_raw("hashgrid_encode", [0] * 3, _lib.hashgrid_descriptor(4, 1, 2, 8, 1, 0, 9, 1.0, [1.0], [2], [0, 8]))
+    # the error is cleared once read
+    assert _lib.last_error() == (0, "")

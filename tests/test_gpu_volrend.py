@@ -1,0 +1,241 @@
+"""GPU parity of the volume-rendering targets against the CPU oracle.
+
+Integer outputs (sample counts, offsets, validity, ray indices, cells, bitfields,
+Morton codes, early-stop counts, termination flags) must match bit for bit.  The
+f32 outputs follow the oracle's rounding convention and are compared bit for bit
+as well; the oracle's own f32-vs-float64 error bounds live in test_oracle_pins.py.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from jaxngp_amd import volrend
+from tests.helpers import blob_bitfield, integrate_inputs, march_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(dev)
+
+
+def U(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint32)).to(dev)
+
+
+def N(t):
+    return t.cpu().numpy()
+
+
+def test_morton_round_trip_all_cells(dev):
+    G = 128
+    idx = np.arange(G ** 3, dtype=np.uint32)
+    xyz = volrend.morton3d_invert(U(idx, dev))
+    np.testing.assert_array_equal(N(xyz), oracle.morton3d_invert(idx))
+    back = volrend.morton3d(xyz)
+    np.testing.assert_array_equal(N(back), idx)
+    rnd = np.random.default_rng(0).integers(0, 2 ** 32, 100000, dtype=np.uint64).astype(np.uint32)
+    np.testing.assert_array_equal(N(volrend.morton3d_invert(U(rnd, dev))), oracle.morton3d_invert(rnd))
+    xyz_r = np.random.default_rng(1).integers(0, 2 ** 32, (50000, 3), dtype=np.uint64).astype(np.uint32)
+    np.testing.assert_array_equal(N(volrend.morton3d(U(xyz_r, dev))), oracle.morton3d(xyz_r))
+
+
+@pytest.mark.parametrize("n_bits", [8, 64 * 8, 128 ** 3, 2 * 128 ** 3 + 8])
+def test_packbits(dev, rng, n_bits):
+    dens = rng.normal(size=n_bits).astype(np.float32)
+    thr = rng.normal(size=n_bits).astype(np.float32)
+    mask, bits = volrend.packbits(T(thr, dev), T(dens, dev))
+    m_o, b_o = oracle.packbits(thr, dens)
+    np.testing.assert_array_equal(N(mask), m_o)
+    np.testing.assert_array_equal(N(bits), b_o)
+    mask, bits = volrend.packbits(0.1, T(dens, dev))
+    np.testing.assert_array_equal(N(bits), oracle.packbits(np.float32(0.1), dens)[1])
+
+
+def _march_both(dev, S, diag, K, G, bound, portion, o, d, t0, t1, noise, bits):
+    got = volrend.march_rays(S, diag, K, G, bound, portion, T(o, dev), T(d, dev), T(t0, dev), T(t1, dev),
+                             T(noise, dev), T(bits, dev))
+    want = oracle.march_rays(S, diag, K, G, bound, portion, o, d, t0, t1, noise, bits)
+    return got, want
+
+
+def _assert_march_equal(got, want):
+    measured, valid, ns, start, idcs, xyzs, dirs, dss, z = got
+    assert int(N(measured.view(torch.int32))) == int(want["measured"])
+    np.testing.assert_array_equal(N(valid), want["ray_is_valid"])
+    np.testing.assert_array_equal(N(ns), want["rays_n_samples"])
+    np.testing.assert_array_equal(N(start), want["rays_sample_startidx"])
+    np.testing.assert_array_equal(N(idcs), want["idcs"])
+    np.testing.assert_array_equal(N(xyzs), want["xyzs"])
+    np.testing.assert_array_equal(N(dirs), want["dirs"])
+    np.testing.assert_array_equal(N(dss), want["dss"])
+    np.testing.assert_array_equal(N(z), want["z_vals"])
+
+
+@pytest.mark.parametrize("case", [
+    dict(n=2048, S=2 ** 20, K=1, G=128, bound=1.0, portion=0.0),
+    dict(n=4096, S=20000, K=1, G=128, bound=1.0, portion=0.0),      # buffer fills mid-batch
+    dict(n=777, S=1, K=1, G=128, bound=1.0, portion=0.0),           # tiny buffer
+    dict(n=1500, S=2 ** 19, K=2, G=64, bound=2.0, portion=1 / 256),  # cascades + growing steps
+    dict(n=1000, S=2 ** 19, K=1, G=32, bound=1.0, portion=0.0),
+])
+def test_march_rays_bit_exact(dev, case):
+    rng = np.random.default_rng(case["n"])
+    o, d, t0, t1, noise, bits, _ = march_inputs(rng, case["n"], bound=case["bound"], G=case["G"], K=case["K"])
+    t1[::17] = t0[::17]  # some invalid rays
+    got, want = _march_both(dev, case["S"], 1024, case["K"], case["G"], case["bound"], case["portion"], o, d, t0, t1,
+                            noise, bits)
+    _assert_march_equal(got, want)
+
+
+def test_march_rays_full_and_empty_fields(dev, rng):
+    o, d, t0, t1, noise, _, _ = march_inputs(rng, 3000)
+    for fill in (0, 255):
+        bits = np.full(128 ** 3 // 8, fill, np.uint8)
+        got, want = _march_both(dev, 2 ** 20, 1024, 1, 128, 1.0, 0.0, o, d, t0, t1, noise, bits)
+        _assert_march_equal(got, want)
+
+
+def test_march_rays_zero_rays(dev):
+    z = np.zeros((0, 3), np.float32)
+    e = np.zeros(0, np.float32)
+    bits = np.full(128 ** 3 // 8, 255, np.uint8)
+    measured, valid, ns, start, idcs, xyzs, dirs, dss, zz = volrend.march_rays(
+        64, 1024, 1, 128, 1.0, 0.0, T(z, dev), T(z, dev), T(e, dev), T(e, dev), T(e, dev), T(bits, dev))
+    assert int(N(measured.view(torch.int32))) == 0 and not N(xyzs).any() and not N(idcs.view(torch.int32)).any()
+
+
+def test_march_rays_reference_batch_shape(dev):
+    """Training batch at the reference convention n_rays = total_samples = 2^18."""
+    rng = np.random.default_rng(2024)
+    n = 2 ** 18
+    o, d, t0, t1, noise, bits, _ = march_inputs(rng, n)
+    got, want = _march_both(dev, n, 1024, 1, 128, 1.0, 0.0, o, d, t0, t1, noise, bits)
+    _assert_march_equal(got, want)
+    # size-independent properties: ranges tile [0, measured) in ray order
+    ns = N(got[2]).astype(np.int64)
+    st = N(got[3]).astype(np.int64)
+    v = ns > 0
+    assert (st[v][1:] == (st[v] + ns[v])[:-1]).all()
+    assert int(ns.sum()) == int(N(got[0].view(torch.int32)))
+
+
+def test_march_rays_inference_bit_exact(dev):
+    rng = np.random.default_rng(99)
+    n_total, n_slots, cap = 5000, 1024, 8
+    o, d, t0, t1, _, bits, _ = march_inputs(rng, n_total)
+    t1[::13] = t0[::13] - 0.1  # t_end < t_start rays
+    ts = t0.copy()
+    term = np.ones(n_slots, bool)
+    idx = np.zeros(n_slots, np.uint32)
+    nri = np.zeros(1, np.uint32)
+    ts_dev = T(ts, dev)
+    term_dev, idx_dev, nri_dev = T(term, dev), U(idx, dev), U(nri, dev)
+    for it in range(12):
+        got = volrend.march_rays_inference(1024, 1, 128, cap, 1.0, 0.0, T(o, dev), T(d, dev), ts_dev, T(t1, dev),
+                                           T(bits, dev), nri_dev, term_dev, idx_dev)
+        want = oracle.march_rays_inference(1024, 1, 128, cap, 1.0, 0.0, o, d, ts, t1, bits, nri, term, idx)
+        nxt, gidx, gns, gts, gx, gds, gz = got
+        assert int(N(nxt.view(torch.int32))[0]) == int(want["next_ray_index"])
+        np.testing.assert_array_equal(N(gidx), want["indices"])
+        np.testing.assert_array_equal(N(gns), want["n_samples"])
+        np.testing.assert_array_equal(N(gx), want["xyzs"])
+        np.testing.assert_array_equal(N(gds), want["dss"])
+        np.testing.assert_array_equal(N(gz), want["z_vals"])
+        # host-side update as the reference wrapper does (t_starts.at[indices].set)
+        keep = want["indices"] < n_total
+        ts[want["indices"][keep]] = want["t_starts_out"][keep]
+        np.testing.assert_array_equal(N(gts), ts)
+        nri = np.array([want["next_ray_index"]], np.uint32)
+        idx = want["indices"]
+        term = (want["n_samples"] < cap) | (rng.uniform(size=n_slots) < 0.3)
+        ts_dev, nri_dev, idx_dev, term_dev = gts, nxt, gidx, T(term, dev)
+
+
+def test_integrate_rays_bit_exact_and_backward(dev):
+    rng = np.random.default_rng(5)
+    start, ns, bgs, dss, z, drgbs = integrate_inputs(rng, 4096, 200)
+    m_o, rgbd_o, op_o = oracle.integrate_rays(start, ns, bgs, dss, z, drgbs)
+    drgbs_t = T(drgbs, dev).requires_grad_(True)
+    bgs_t = T(bgs, dev).requires_grad_(True)
+    z_t = T(z, dev).requires_grad_(True)
+    m, rgbd, op = volrend.integrate_rays(0.3, U(start, dev), U(ns, dev), bgs_t, T(dss, dev), z_t, drgbs_t)
+    assert int(N(m.view(torch.int32))) == int(m_o)
+    np.testing.assert_array_equal(N(rgbd.detach()), rgbd_o)
+    np.testing.assert_array_equal(N(op.detach()), op_o)
+    g = rng.normal(size=(4096, 4)).astype(np.float32)
+    rgbd.backward(T(g, dev))
+    dbg_o, dz_o, dd_o = oracle.integrate_rays_backward(0.3, start, ns, bgs, dss, z, drgbs, rgbd_o, op_o, g)
+    np.testing.assert_array_equal(N(bgs_t.grad), dbg_o)
+    np.testing.assert_array_equal(N(z_t.grad), dz_o)
+    np.testing.assert_array_equal(N(drgbs_t.grad), dd_o)
+
+
+def test_integrate_rays_near_plane_and_broadcast_bg(dev):
+    rng = np.random.default_rng(6)
+    start, ns, bgs, dss, z, drgbs = integrate_inputs(rng, 500, 40)
+    z = (z - 2.6).astype(np.float32)  # samples before the near plane and z < 1
+    drgbs_t = T(drgbs, dev).requires_grad_(True)
+    m, rgbd, op = volrend.integrate_rays(0.3, U(start, dev), U(ns, dev), [1.0, 1.0, 1.0], T(dss, dev), T(z, dev),
+                                         drgbs_t)
+    m_o, rgbd_o, op_o = oracle.integrate_rays(start, ns, np.ones(3, np.float32), dss, z, drgbs)
+    np.testing.assert_array_equal(N(rgbd.detach()), rgbd_o)
+    g = rng.normal(size=(500, 4)).astype(np.float32)
+    rgbd.backward(T(g, dev))
+    _, _, dd_o = oracle.integrate_rays_backward(0.3, start, ns, np.ones((500, 3), np.float32), dss, z, drgbs, rgbd_o,
+                                                op_o, g)
+    np.testing.assert_array_equal(N(drgbs_t.grad), dd_o)
+
+
+def test_integrate_rays_2pow20_samples_properties(dev):
+    """Config-2 size: 1024 rays x 1024 samples; opacity in [0,1], measured <= S, and
+    the f32 result within float64 error on a subset."""
+    rng = np.random.default_rng(8)
+    n_rays, m = 1024, 1024
+    ns = np.full(n_rays, m, np.uint32)
+    start = (np.arange(n_rays) * m).astype(np.uint32)
+    S = n_rays * m
+    dss = np.full(S, 2 * 1.7320508 / 1024, np.float32)
+    z = np.tile(3.0 + np.arange(m, dtype=np.float32) * dss[0], n_rays).astype(np.float32)
+    sig = np.log1p(np.exp(rng.normal(0, 2, S))).astype(np.float32)
+    drgbs = np.concatenate([sig[:, None], rng.uniform(0, 1, (S, 3)).astype(np.float32)], 1)
+    bgs = rng.uniform(0, 1, (n_rays, 3)).astype(np.float32)
+    mm, rgbd, op = volrend.integrate_rays(0.3, U(start, dev), U(ns, dev), T(bgs, dev), T(dss, dev), T(z, dev),
+                                          T(drgbs, dev))
+    m_o, rgbd_o, op_o = oracle.integrate_rays(start, ns, bgs, dss, z, drgbs)
+    assert int(N(mm.view(torch.int32))) == int(m_o) <= S
+    np.testing.assert_array_equal(N(rgbd), rgbd_o)
+    sub = np.arange(0, n_rays, 37)
+    rgbd64, op64, _ = oracle.integrate_rays64(start[sub], ns[sub], bgs[sub], dss, z, drgbs)
+    np.testing.assert_allclose(N(rgbd)[sub], rgbd64, rtol=1e-4, atol=2e-5)
+
+
+def test_integrate_rays_inference_bit_exact(dev):
+    rng = np.random.default_rng(11)
+    n_total, n, cap = 3000, 512, 8
+    rays_bg = rng.uniform(0, 1, (n_total, 3)).astype(np.float32)
+    rays_rgbd = rng.uniform(0, 0.5, (n_total, 4)).astype(np.float32)
+    rays_T = rng.uniform(0, 1, n_total).astype(np.float32)
+    rays_T[::7] = 5e-5
+    idx = rng.choice(n_total + 100, n, replace=False).astype(np.uint32)
+    ns = rng.integers(0, cap + 1, n).astype(np.uint32)
+    dss = rng.uniform(0.001, 0.05, (n, cap)).astype(np.float32)
+    z = rng.uniform(2, 5, (n, cap)).astype(np.float32)
+    drgbs = np.concatenate([rng.uniform(0, 80, (n, cap, 1)), rng.uniform(0, 1, (n, cap, 3))], 2).astype(np.float32)
+    cnt, term, rgbd, Tn = volrend.integrate_rays_inference(T(rays_bg, dev), T(rays_rgbd, dev), T(rays_T, dev),
+                                                           U(ns, dev), U(idx, dev), T(dss, dev), T(z, dev),
+                                                           T(drgbs, dev))
+    c_o, t_o, rgbd_o, T_o = oracle.integrate_rays_inference(rays_bg, rays_rgbd, rays_T, ns, idx, dss, z, drgbs)
+    assert int(N(cnt.view(torch.int32))) == int(c_o)
+    np.testing.assert_array_equal(N(term), t_o)
+    keep = idx < n_total
+    want_rgbd = rays_rgbd.copy()
+    want_T = rays_T.copy()
+    want_rgbd[idx[keep]] = rgbd_o[keep]
+    want_T[idx[keep]] = T_o[keep]
+    np.testing.assert_array_equal(N(rgbd), want_rgbd)
+    np.testing.assert_array_equal(N(Tn), want_T)
