@@ -210,6 +210,61 @@ void nerf_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, s
 void nerf_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* ---------------------------------------------------------------------------
+ * Training-step harness (the device-side counterpart of app/nerf/_utils.py:80-170
+ * and app/nerf/train.py:47-48).  Same calling convention.
+ * ------------------------------------------------------------------------- */
+
+#define NGP_TRAIN_PERTURB 1u
+#define NGP_TRAIN_RANDOM_BG 2u
+typedef struct NgpTrainRaysDescriptor {
+    uint32_t n_rays;
+    uint32_t n_views;
+    uint32_t width;
+    uint32_t height;
+    float fx, fy, cx, cy; /* pinhole intrinsics, pixel centres at +0.5 */
+    float bound;          /* scene AABB [-bound, bound]^3 */
+    uint32_t seed_lo, seed_hi, step;
+    uint32_t flags;       /* NGP_TRAIN_PERTURB | NGP_TRAIN_RANDOM_BG */
+    float bg[3];          /* background when NGP_TRAIN_RANDOM_BG is clear */
+} NgpTrainRaysDescriptor;
+
+/* Samples n_rays pixels uniformly over all views (with replacement) and builds
+ * their rays (utils/types.py:400-439, app/nerf/_utils.py:97-115), AABB near/far
+ * (models/renderers/cuda.py:56-97), marching noise U[0,1), background colours and
+ * alpha-blended ground truth (utils/data.py:443-464).
+ * in : transforms f32[V,12] (camera-to-world rotation row-major, then translation),
+ *      rgba u8[V,H,W,4]
+ * out: rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n],
+ *      noises f32[n], bgs f32[n,3], gt_rgb f32[n,3], pixel_ids u32[n] */
+void train_make_rays(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct NgpLossDescriptor {
+    uint32_t n_rays;
+    float delta; /* Huber delta (0.1 in app/nerf/_utils.py:154) */
+} NgpLossDescriptor;
+
+/* Huber loss averaged over rgb and over valid rays, and its gradient with respect
+ * to final_rgbds (app/nerf/_utils.py:149-158; the depth channel gets 0).
+ * in : final_rgbds f32[n,4], gt_rgb f32[n,3], ray_is_valid bool[n]
+ * out: n_valid_rays u32[1], dL_dfinal_rgbds f32[n,4], loss f32[1] */
+void train_loss_grad(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct NgpAdamDescriptor {
+    uint64_t n;
+    float lr, b1, b2, eps, eps_root;
+    float b1_pow_t, b2_pow_t; /* b1^t and b2^t for the bias corrections, t = update count */
+    float weight_decay;       /* added to the (already lr-scaled) update: u += wd * p */
+    float grad_scale;         /* gradients are multiplied by this first (loss-scale undo) */
+    uint32_t mirror_dtype;    /* NGP_DTYPE_F16: also write the updated params as f16 */
+} NgpAdamDescriptor;
+
+/* One optax.adam(lr, b1, b2, eps, eps_root) step followed by
+ * add_decayed_weights(weight_decay), in that order (app/nerf/_utils.py:19-77).
+ * in/out: params f32[n]; in: grads f32[n]; in/out: m f32[n], v f32[n];
+ * out: mirror f16[n] (only read when mirror_dtype is NGP_DTYPE_F16; may be NULL) */
+void adam_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* ---------------------------------------------------------------------------
  * Status.
  * ------------------------------------------------------------------------- */
 

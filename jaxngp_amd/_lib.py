@@ -34,6 +34,9 @@ TARGETS = (
     "hashgrid_encode_backward",
     "nerf_mlp_forward",
     "nerf_mlp_backward",
+    "train_make_rays",
+    "train_loss_grad",
+    "adam_update",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -168,3 +171,22 @@ def hashgrid_descriptor(n_points, L, F, T, first_hash_level, table_dtype, n_rows
 
 def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0) -> bytes:
     return struct.pack("<2I", n_samples, n_workspace_blocks)
+
+
+TRAIN_PERTURB = 1
+TRAIN_RANDOM_BG = 2
+
+
+def train_rays_descriptor(n_rays, n_views, width, height, fx, fy, cx, cy, bound, seed, step, flags, bg) -> bytes:
+    return struct.pack("<4I5f4I3f", n_rays, n_views, width, height, fx, fy, cx, cy, bound, seed & 0xFFFFFFFF,
+                       (seed >> 32) & 0xFFFFFFFF, step & 0xFFFFFFFF, flags, *bg)
+
+
+def loss_descriptor(n_rays, delta) -> bytes:
+    return struct.pack("<If", n_rays, delta)
+
+
+def adam_descriptor(n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_decay, grad_scale=1.0,
+                    mirror_dtype=DTYPE_F32) -> bytes:
+    return struct.pack("<Q9fI", n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_decay, grad_scale,
+                       mirror_dtype)

@@ -47,6 +47,7 @@ SOURCES = {
     "integrate.hip": ["-ffp-contract=off"],
     "hashgrid.hip": ["-ffp-contract=off"],
     "nerf_mlp.hip": ["-ffp-contract=off"],
+    "harness.hip": ["-ffp-contract=off"],
 }
 
 

@@ -1,0 +1,230 @@
+// harness.hip — device-side pieces of the training step around the hot path.
+//
+// train_make_rays:  pixel sampling + pinhole ray generation + AABB near/far +
+//                   perturbation noise + background colour + ground-truth blend
+//                   in one kernel (app/nerf/train.py:47-48, app/nerf/_utils.py:93-150,
+//                   utils/types.py:400-439, models/renderers/cuda.py:56-97,
+//                   utils/data.py:443-464).
+// train_loss_grad:  Huber(delta) loss over valid rays and its gradient with respect
+//                   to final_rgbds (app/nerf/_utils.py:149-158).
+// adam_update:      optax.adam(b1, b2, eps, eps_root) followed by
+//                   add_decayed_weights, as chained in app/nerf/_utils.py:19-77.
+//
+// Random numbers come from a stateless 64-bit mix of (seed, step, ray, stream),
+// so a step is reproducible from its inputs and needs no generator state.
+#include <hip/hip_fp16.h>
+
+#include "ngp_common.h"
+
+namespace ngp {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    // splitmix64 finaliser (Steele, Lea, Flood 2014)
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t draw(uint64_t seed, uint64_t step, uint64_t i, uint64_t stream) {
+    return mix64(seed ^ mix64(step * 0x9e3779b97f4a7c15ull ^ mix64(i * 4 + stream)));
+}
+
+// uniform in [0, 1) with 24 random bits
+__device__ __forceinline__ float unit24(uint64_t r) { return (float)(uint32_t)(r >> 40) * (1.0f / 16777216.0f); }
+
+__global__ void __launch_bounds__(kBlock)
+    make_rays_kernel(NgpTrainRaysDescriptor d, const float *__restrict__ transforms, const uint8_t *__restrict__ rgba,
+                     float *__restrict__ rays_o, float *__restrict__ rays_d, float *__restrict__ t_starts,
+                     float *__restrict__ t_ends, float *__restrict__ noises, float *__restrict__ bgs,
+                     float *__restrict__ gt, uint32_t *__restrict__ pixel_ids) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.n_rays) return;
+    const uint64_t seed = ((uint64_t)d.seed_hi << 32) | d.seed_lo;
+    const uint64_t n_px = (uint64_t)d.width * d.height;
+    const uint64_t pick = draw(seed, d.step, i, 0) % (n_px * d.n_views);
+    const uint32_t view = (uint32_t)(pick / n_px);
+    const uint32_t pix = (uint32_t)(pick - (uint64_t)view * n_px);
+    const uint32_t px = pix % d.width, py = pix / d.width;
+    // camera-space direction, CV -> CG axis flip, normalised (types.py:417-439)
+    const float x = (((float)px + 0.5f) - d.cx) / d.fx;
+    const float y = (((float)py + 0.5f) - d.cy) / d.fy;
+    const float cx = x, cy = -y, cz = -1.0f;
+    const float inv = 1.0f / sqrtf(cx * cx + cy * cy + cz * cz);
+    const float dcx = cx * inv, dcy = cy * inv, dcz = cz * inv;
+    const float *T = transforms + 12 * (size_t)view;  // R row-major (9) then translation (3)
+    const float dx = T[0] * dcx + T[1] * dcy + T[2] * dcz;
+    const float dy = T[3] * dcx + T[4] * dcy + T[5] * dcz;
+    const float dz = T[6] * dcx + T[7] * dcy + T[8] * dcz;
+    const float ox = T[9], oy = T[10], oz = T[11];
+    // AABB [-bound, bound]^3 entry / exit (cuda.py:56-97)
+    const float b = d.bound;
+    float t0 = 0.0f, t1 = 0.0f;
+    {
+        const float o3[3] = {ox, oy, oz};
+        const float d3[3] = {dx, dy, dz};
+        float lo = -__builtin_huge_valf(), hi = __builtin_huge_valf();
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float da = d3[a];
+            da = signbit(da) ? fminf(da, -1e-15f) : fmaxf(da, 1e-15f);
+            const float ta = (-b - o3[a]) / da, tb = (b - o3[a]) / da;
+            lo = fmaxf(lo, fminf(ta, tb));
+            hi = fminf(hi, fmaxf(ta, tb));
+        }
+        t0 = fmaxf(0.0f, lo);
+        t1 = hi;
+    }
+    rays_o[3 * (size_t)i + 0] = ox;
+    rays_o[3 * (size_t)i + 1] = oy;
+    rays_o[3 * (size_t)i + 2] = oz;
+    rays_d[3 * (size_t)i + 0] = dx;
+    rays_d[3 * (size_t)i + 1] = dy;
+    rays_d[3 * (size_t)i + 2] = dz;
+    t_starts[i] = t0;
+    t_ends[i] = t1;
+    noises[i] = (d.flags & NGP_TRAIN_PERTURB) ? unit24(draw(seed, d.step, i, 1)) : 0.0f;
+    float bg[3] = {d.bg[0], d.bg[1], d.bg[2]};
+    if (d.flags & NGP_TRAIN_RANDOM_BG) {
+        const uint64_t r = draw(seed, d.step, i, 2);
+        bg[0] = (float)((uint32_t)(r >> 43) & 0x1fffffu) * (1.0f / 2097152.0f);
+        bg[1] = (float)((uint32_t)(r >> 22) & 0x1fffffu) * (1.0f / 2097152.0f);
+        bg[2] = (float)((uint32_t)(r >> 1) & 0x1fffffu) * (1.0f / 2097152.0f);
+    }
+    bgs[3 * (size_t)i + 0] = bg[0];
+    bgs[3 * (size_t)i + 1] = bg[1];
+    bgs[3 * (size_t)i + 2] = bg[2];
+    // ground truth: rgb * alpha + bg * (1 - alpha) on [0,1] floats (data.py:443-464)
+    const uint8_t *p = rgba + 4 * ((uint64_t)view * n_px + pix);
+    const float alpha = (float)p[3] / 255.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gt[3 * (size_t)i + c] = ((float)p[c] / 255.0f) * alpha + bg[c] * (1.0f - alpha);
+    pixel_ids[i] = (uint32_t)(pick & 0xffffffffu);
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) count_true_kernel(uint32_t n, const bool *__restrict__ flags,
+                                                            uint32_t *__restrict__ count) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t v = (i < n && flags[i]) ? 1u : 0u;
+    const uint32_t s = wave_sum(v);
+    if (lane_id() == 0 && s) atomicAdd(count, s);
+}
+
+// optax.huber_loss(pred, target, delta): 0.5 e^2 if |e| <= delta else delta (|e| - delta/2)
+__global__ void __launch_bounds__(kBlock)
+    loss_grad_kernel(uint32_t n, float delta, const float *__restrict__ rgbd, const float *__restrict__ gt,
+                     const bool *__restrict__ valid, const uint32_t *__restrict__ n_valid,
+                     float *__restrict__ dL_drgbd, float *__restrict__ loss) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    float l = 0.0f;
+    if (i < n) {
+        const float nv = (float)max(*n_valid, 1u);
+        float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (valid[i]) {
+            float e[3], h = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                e[c] = rgbd[4 * (size_t)i + c] - gt[3 * (size_t)i + c];
+                const float ae = fabsf(e[c]);
+                h += ae <= delta ? 0.5f * e[c] * e[c] : delta * (ae - 0.5f * delta);
+            }
+            l = (h / 3.0f) / nv;
+            const float s = 1.0f / (3.0f * nv);
+            g = make_float4(fminf(fmaxf(e[0], -delta), delta) * s, fminf(fmaxf(e[1], -delta), delta) * s,
+                            fminf(fmaxf(e[2], -delta), delta) * s, 0.0f);
+        }
+        *reinterpret_cast<float4 *>(dL_drgbd + 4 * (size_t)i) = g;
+    }
+    // wave-reduced loss
+#pragma unroll
+    for (int k = kWave / 2; k > 0; k >>= 1) l += __shfl_xor(l, k, kWave);
+    if (lane_id() == 0 && l != 0.0f) atomicAdd(loss, l);
+}
+
+template <typename TT>
+__global__ void __launch_bounds__(kBlock)
+    adam_kernel(NgpAdamDescriptor d, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                float *__restrict__ v, TT *__restrict__ mirror) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const float c1 = 1.0f - d.b1_pow_t, c2 = 1.0f - d.b2_pow_t;
+    for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < d.n; k += stride) {
+        const float gk = g[k] * d.grad_scale;
+        const float mk = d.b1 * m[k] + (1.0f - d.b1) * gk;
+        const float vk = d.b2 * v[k] + (1.0f - d.b2) * (gk * gk);
+        m[k] = mk;
+        v[k] = vk;
+        const float mh = mk / c1, vh = vk / c2;
+        const float u = -d.lr * (mh / (sqrtf(vh + d.eps_root) + d.eps)) + d.weight_decay * p[k];
+        const float pk = p[k] + u;
+        p[k] = pk;
+        if (mirror != nullptr) mirror[k] = (TT)pk;
+    }
+}
+
+}  // namespace
+}  // namespace ngp
+
+using namespace ngp;
+
+extern "C" void train_make_rays(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "train_make_rays";
+    NgpTrainRaysDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.n_rays == 0) return;
+    if (d.n_views == 0 || d.width == 0 || d.height == 0 || !(d.bound > 0.0f)) {
+        set_error(kBadDescriptor, "%s: need views, a positive image size and bound", target);
+        return;
+    }
+    if (!check_buffers(buffers, 10, target)) return;
+    hipLaunchKernelGGL(make_rays_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, d,
+                       (const float *)buffers[0], (const uint8_t *)buffers[1], (float *)buffers[2],
+                       (float *)buffers[3], (float *)buffers[4], (float *)buffers[5], (float *)buffers[6],
+                       (float *)buffers[7], (float *)buffers[8], (uint32_t *)buffers[9]);
+    check_launch(target);
+}
+
+extern "C" void train_loss_grad(ngp_stream_t stream_, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "train_loss_grad";
+    NgpLossDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (!check_buffer_list(buffers, {3, 5}, target)) return;
+    if (d.n_rays > 0 && !check_buffers(buffers, 6, target)) return;
+    hipStream_t stream = (hipStream_t)stream_;
+    uint32_t *n_valid = (uint32_t *)buffers[3];
+    float *loss = (float *)buffers[5];
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, n_valid, 1);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, (uint32_t *)loss, 1);
+    if (d.n_rays > 0) {
+        hipLaunchKernelGGL(count_true_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
+                           (const bool *)buffers[2], n_valid);
+        hipLaunchKernelGGL(loss_grad_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
+                           d.delta, (const float *)buffers[0], (const float *)buffers[1], (const bool *)buffers[2],
+                           n_valid, (float *)buffers[4], loss);
+    }
+    check_launch(target);
+}
+
+extern "C" void adam_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "adam_update";
+    NgpAdamDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.n == 0) return;
+    if (!check_buffers(buffers, 4, target)) return;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(div_up(d.n, kBlock), 8192);
+    if (d.mirror_dtype == NGP_DTYPE_F16 && buffers[4] != nullptr) {
+        hipLaunchKernelGGL(adam_kernel<__half>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, d,
+                           (float *)buffers[0], (const float *)buffers[1], (float *)buffers[2], (float *)buffers[3],
+                           (__half *)buffers[4]);
+    } else {
+        hipLaunchKernelGGL(adam_kernel<float>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, d,
+                           (float *)buffers[0], (const float *)buffers[1], (float *)buffers[2], (float *)buffers[3],
+                           (float *)nullptr);
+    }
+    check_launch(target);
+}

@@ -1,0 +1,122 @@
+"""Ray generation and renderers (models/renderers/cuda.py) on the gfx950 kernels.
+
+``render_rays_train`` is the differentiable training renderer (cuda.py:100-162):
+march_rays -> NeRF -> integrate_rays, with gradients through the hand-written
+integrate_rays VJP, the fused MLP backward and the hash-grid table gradient.
+
+``render_image_inference`` renders a full frame with the reference's ray-pool
+scheme (cuda.py:176-373): a pool of up to 8192 ray slots, each round marches at
+most ``march_steps_cap`` occupied steps per ray (march_rays_inference), evaluates
+the field on those samples and composites them (integrate_rays_inference);
+terminated slots take the next unrendered rays.  Unlike the reference it keeps the
+pool on the device between rounds and only synchronises once per outer round to
+read the termination count.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import volrend
+from .scene import Camera, f32_to_u8
+
+
+def make_rays_worldspace(camera: Camera, pose: torch.Tensor):
+    """Pixel-centre rays of every pixel (cuda.py:21-53, types.py:400-439).
+    pose: camera-to-world [3, 4] (rotation | translation)."""
+    dev = pose.device
+    idx = torch.arange(camera.n_pixels, device=dev)
+    x = (idx % camera.width).float()
+    y = torch.div(idx, camera.width, rounding_mode="floor").float()
+    dx = ((x + 0.5) - camera.cx) / camera.fx
+    dy = ((y + 0.5) - camera.cy) / camera.fy
+    d = torch.stack([dx, -dy, -torch.ones_like(dx)], -1)
+    d = d / torch.linalg.norm(d, dim=-1, keepdim=True)
+    R = pose[:, :3].float()
+    o = pose[:, 3].float().expand(camera.n_pixels, 3).contiguous()
+    return o, (d @ R.T).contiguous()
+
+
+def make_near_far_from_bound(bound: float, o: torch.Tensor, d: torch.Tensor):
+    """AABB entry/exit distances (cuda.py:56-97)."""
+    eps = 1e-15
+    d = torch.where(torch.signbit(d), torch.clamp(d, max=-eps), torch.clamp(d, min=eps))
+    t0 = (-bound - o) / d
+    t1 = (bound - o) / d
+    ts = torch.minimum(t0, t1).amax(-1)
+    te = torch.maximum(t0, t1).amin(-1)
+    return torch.clamp(ts, min=0.0).contiguous(), te.contiguous()
+
+
+def render_rays_train(o_world, d_world, bg, total_samples: int, model, occupancy_bitfield, *, K=1, G=128,
+                      diagonal_n_steps=1024, stepsize_portion=0.0, near=0.3, perturb=True, generator=None):
+    """Differentiable training render (cuda.py:100-162).  Returns (batch_metrics, final_rgbds, tv)."""
+    bound = model.bound
+    t_starts, t_ends = make_near_far_from_bound(bound, o_world, d_world)
+    noises = (torch.rand(t_starts.shape, device=o_world.device, generator=generator) if perturb else 0.0)
+    measured0, valid, ns, start, idcs, xyzs, dirs, dss, z_vals = volrend.march_rays(
+        total_samples, diagonal_n_steps, K, G, bound, stepsize_portion, o_world, d_world, t_starts, t_ends, noises,
+        occupancy_bitfield)
+    drgbs, tv = model(xyzs, dirs, None)
+    effective, final_rgbds, final_op = volrend.integrate_rays(near, start, ns, bg, dss, z_vals, drgbs)
+    metrics = {
+        "n_valid_rays": valid.sum(),
+        "ray_is_valid": valid,
+        "measured_batch_size_before_compaction": measured0,
+        "measured_batch_size": effective,
+    }
+    return metrics, final_rgbds, tv
+
+
+@torch.no_grad()
+def march_and_integrate_inference(next_ray_index, terminated, indices, rays_rgbd, rays_T, t_starts, *, model,
+                                  rays_o, rays_d, t_ends, rays_bg, occupancy_bitfield, march_steps_cap, K, G,
+                                  diagonal_n_steps, stepsize_portion):
+    """One pool round (cuda.py:176-241)."""
+    next_ray_index, indices, n_samples, t_starts, xyzs, dss, z_vals = volrend.march_rays_inference(
+        diagonal_n_steps, K, G, march_steps_cap, model.bound, stepsize_portion, rays_o, rays_d, t_starts, t_ends,
+        occupancy_bitfield, next_ray_index, terminated, indices)
+    idx = indices.view(torch.int32).long().clamp(max=rays_d.shape[0] - 1)
+    dirs = rays_d[idx][:, None, :].expand(xyzs.shape).contiguous()
+    drgbs, _ = model(xyzs.reshape(-1, 3), dirs.reshape(-1, 3), None)
+    terminate_cnt, terminated, rays_rgbd, rays_T = volrend.integrate_rays_inference(
+        rays_bg, rays_rgbd, rays_T, n_samples, indices, dss, z_vals, drgbs.reshape(*dss.shape, 4))
+    return terminate_cnt, next_ray_index, terminated, indices, rays_rgbd, rays_T, t_starts
+
+
+@torch.no_grad()
+def render_image_inference(model, camera: Camera, pose: torch.Tensor, occupancy_bitfield, bg=(1.0, 1.0, 1.0), *,
+                           K=1, G=128, diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8,
+                           pool: int = 8192):
+    """Renders one frame (cuda.py:244-373).  Returns (bg [H,W,3] f32, rgb u8 [H,W,3],
+    depth u8 [H,W])."""
+    dev = pose.device
+    o, d = make_rays_worldspace(camera, pose)
+    t_starts, t_ends = make_near_far_from_bound(model.bound, o, d)
+    n_px = camera.n_pixels
+    rays_rgbd = torch.zeros(n_px, 4, device=dev)
+    rays_T = torch.ones(n_px, device=dev)
+    rays_bg = torch.as_tensor(bg, dtype=torch.float32, device=dev).expand(n_px, 3).contiguous()
+    n_rays = min(pool, n_px)
+    next_ray_index = torch.zeros(1, dtype=torch.int32, device=dev).view(torch.uint32)
+    terminated = torch.ones(n_rays, dtype=torch.bool, device=dev)
+    indices = torch.zeros(n_rays, dtype=torch.int32, device=dev).view(torch.uint32)
+    n_rendered = 0
+    while n_rendered < n_px:
+        iters = max(1, (n_px - n_rendered) // n_rays)
+        iters = 2 ** int(math.log2(iters) + 1)
+        cnt_total = torch.zeros((), dtype=torch.int64, device=dev)
+        for _ in range(iters):
+            cnt, next_ray_index, terminated, indices, rays_rgbd, rays_T, t_starts = march_and_integrate_inference(
+                next_ray_index, terminated, indices, rays_rgbd, rays_T, t_starts, model=model, rays_o=o, rays_d=d,
+                t_ends=t_ends, rays_bg=rays_bg, occupancy_bitfield=occupancy_bitfield,
+                march_steps_cap=march_steps_cap, K=K, G=G, diagonal_n_steps=diagonal_n_steps,
+                stepsize_portion=stepsize_portion)
+            cnt_total += cnt.view(torch.int32).long()
+        n_rendered += int(cnt_total.item())
+    rgb = f32_to_u8(rays_rgbd[:, :3]).reshape(camera.height, camera.width, 3)
+    depth = rays_rgbd[:, 3]
+    depth_u8 = f32_to_u8((depth - depth.min()) / (depth.max() - depth.min() + 1e-15)).reshape(camera.height,
+                                                                                              camera.width)
+    return rays_bg.reshape(camera.height, camera.width, 3), rgb, depth_u8

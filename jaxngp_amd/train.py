@@ -1,0 +1,267 @@
+"""Training step and occupancy grid on the gfx950 kernels.
+
+Mirrors the reference's training loop (app/nerf/train.py:28-96) and step
+(app/nerf/_utils.py:80-170) for the NeRF-synthetic style setup of the workload:
+bound 1 (one cascade), 128^3 occupancy grid, diagonal_n_steps 1024, perturbed
+marching, random background, Huber(0.1) loss over valid rays, Adam(1e-2, 0.9, 0.99,
+eps 1e-15, eps_root 1e-15) with the staircase decay, the weight-decay term added
+after the learning-rate scaling as optax.chain does there, occupancy-grid updates
+every min(16, step//16 + 1) steps (all cells before step 256).
+
+A step is the sequence of C-ABI targets below on preallocated device buffers, all
+enqueued on the current stream with no host synchronisation:
+  train_make_rays -> march_rays -> hashgrid_encode -> nerf_mlp_forward ->
+  integrate_rays -> train_loss_grad -> integrate_rays_backward ->
+  nerf_mlp_backward -> hashgrid_encode_backward -> adam_update (table, MLP)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .nerfs import N_PARAMS, NeRF, mlp_workspace_blocks
+from .scene import SceneData
+
+SQRT3 = 3 ** 0.5
+
+
+@dataclass
+class TrainConfig:
+    total_samples: int = 1 << 18
+    n_rays: int | None = None  # rays per batch; None: the reference's n_rays = total_samples
+    lr: float = 1e-2
+    diagonal_n_steps: int = 1024
+    grid_res: int = 128
+    perturb: bool = True
+    random_bg: bool = True
+    bg: tuple = (1.0, 1.0, 1.0)
+    near: float = 0.3
+    seed: int = 1_000_000_007  # utils/args.py:18-27
+
+
+def lr_at(step: int, lr: float) -> float:
+    """optax.exponential_decay(lr, 10000, 1/3, staircase, transition_begin=10000, end=lr/100)
+    (app/nerf/_utils.py:20-27)."""
+    if step <= 10000:
+        return lr
+    return max(lr * (1 / 3) ** ((step - 10000) // 10000), lr / 100)
+
+
+class OccupancyGrid:
+    """Occupancy density grid (utils/types.py:93-145): f32 density, bool mask and u8
+    bitfield over K cascades of G^3 cells in Morton order, plus the alive cells."""
+
+    def __init__(self, K: int, G: int, device):
+        self.K, self.G = K, G
+        n = K * G ** 3
+        self.density = torch.zeros(n, dtype=torch.float32, device=device)
+        self.occ_mask = torch.zeros(n, dtype=torch.bool, device=device)
+        self.bitfield = torch.full((n // 8,), 255, dtype=torch.uint8, device=device)
+        self.alive = torch.arange(n, dtype=torch.int64, device=device)
+        self.alive_offsets = [i * G ** 3 for i in range(K + 1)]
+
+    def mean_density_up_to_cascade(self, cas: int) -> torch.Tensor:
+        return self.density[self.alive[: self.alive_offsets[cas]]].mean()
+
+
+def density_threshold_from_min_step_size(diagonal_n_steps: int, bound: float) -> float:
+    """utils/types.py:1367-1369."""
+    return 0.01 * diagonal_n_steps / (2 * min(bound, 1) * SQRT3)
+
+
+def cell_centers(idx: torch.Tensor, G: int, cas: int, bound: float) -> torch.Tensor:
+    """World-space centres of Morton cells `idx` of cascade `cas` (types.py:1193-1197)."""
+    from .volrend import morton3d_invert
+    xyz = morton3d_invert(idx.to(torch.int32).view(torch.uint32)).view(torch.int32).float()
+    xyz = xyz / (G - 1) * 2 - 1
+    mip = min(bound, 2 ** cas)
+    return xyz * (mip - mip / G)
+
+
+class Trainer:
+    def __init__(self, data: SceneData, cfg: TrainConfig = TrainConfig(), device=None, table_dtype=torch.float32):
+        self.cfg = cfg
+        self.data = data
+        self.dev = device or data.rgba.device
+        self.bound = float(data.scene.bound)
+        self.K = max(1, int(1 + math.ceil(math.log2(self.bound))))  # SceneMeta.cascades, types.py:858-860
+        self.portion = 0.0 if self.bound <= 1 else 1 / 256  # stepsize_portion (types.py:874-886), bound in (1, 4)
+        self.model = NeRF(bound=self.bound, device=self.dev, seed=cfg.seed % 2 ** 31, table_dtype=table_dtype)
+        self.grid = OccupancyGrid(self.K, cfg.grid_res, self.dev)
+        self.step_count = 0
+        self.rng = torch.Generator(device=self.dev).manual_seed(cfg.seed % 2 ** 31)
+        S = cfg.total_samples
+        n = cfg.n_rays or S
+        self.n_rays = n
+        f32, u32 = torch.float32, torch.uint32
+        z = lambda *shape, dtype=f32: torch.zeros(*shape, dtype=dtype, device=self.dev)
+        self.b = dict(
+            o=z(n, 3), d=z(n, 3), t0=z(n), t1=z(n), noise=z(n), bgs=z(n, 3), gt=z(n, 3), pix=z(n, dtype=u32),
+            next_loc=z(1, dtype=u32), exceeded=z(1, dtype=u32), valid=z(n, dtype=torch.bool), ns=z(n, dtype=u32),
+            start=z(n, dtype=u32), idcs=z(S, dtype=u32), xyzs=z(S, 3), dirs=z(S, 3), dss=z(S), zs=z(S),
+            enc=z(S, 32), drgbs=z(S, 4), measured=z(1, dtype=u32), rgbd=z(n, 4), op=z(n), n_valid=z(1, dtype=u32),
+            loss=z(1), g_rgbd=z(n, 4), dbgs=z(n, 3), dz=z(S), ddrgbs=z(S, 4), denc=z(S, 32), dparams=z(N_PARAMS),
+            ws=z(mlp_workspace_blocks(S), N_PARAMS), dtable=z(*self.model.position_encoder.table.shape),
+        )
+        table = self.model.position_encoder.table
+        self.table_master = table.detach().float().clone()
+        self.opt = dict(tm=torch.zeros_like(self.table_master), tv=torch.zeros_like(self.table_master),
+                        mm=torch.zeros(N_PARAMS, device=self.dev), mv=torch.zeros(N_PARAMS, device=self.dev))
+        self.mark_untrained_density_grid()
+
+    # -- occupancy grid -------------------------------------------------------------------
+
+    def mark_untrained_density_grid(self):
+        """Cells seen by no training camera are untrainable (types.py:1241-1362)."""
+        G, K = self.cfg.grid_res, self.K
+        G3 = G ** 3
+        dev = self.dev
+        cam = self.data.camera
+        all_idx = torch.arange(K * G3, device=dev)
+        alive = torch.zeros(K * G3, dtype=torch.bool, device=dev)
+        from .volrend import morton3d_invert
+        base = morton3d_invert(torch.arange(G3, device=dev, dtype=torch.int32).view(torch.uint32))
+        base = base.view(torch.int32).float()
+        offs = torch.tensor([[0, 0, 0], [0, 0, 1], [0, 1, 0], [0, 1, 1], [1, 0, 0], [1, 0, 1], [1, 1, 0], [1, 1, 1]],
+                            dtype=torch.float32, device=dev)
+        for cas in range(K):
+            mip = min(2 ** cas, self.bound)
+            xyz = (base / G - 0.5) * 2 * mip
+            verts = xyz[:, None, :] + (2 * mip / G) * offs[None]  # [G3, 8, 3]
+            seen = torch.zeros(G3, dtype=torch.bool, device=dev)
+            for v in range(self.data.n_views):
+                pose = torch.as_tensor(self.data.poses[v], dtype=torch.float32, device=dev)
+                p_cam = (verts - pose[:, 3]) @ pose[:, :3]  # R^T (p - t)
+                front = p_cam[..., 2] < 0
+                u = p_cam[..., 0] / (-p_cam[..., 2]) * cam.fx + cam.cx
+                vv = -p_cam[..., 1] / (-p_cam[..., 2]) * cam.fy + cam.cy
+                inside = (u >= 0) & (u < cam.width) & (vv >= 0) & (vv < cam.height)
+                seen |= (front & inside).any(-1)
+                if bool(seen.all()):
+                    break
+            alive[cas * G3:(cas + 1) * G3] = seen
+        g = self.grid
+        g.density = torch.where(alive, g.density, torch.full_like(g.density, -1.0))
+        thr = -0.5 if self.step_count == 0 else min(
+            density_threshold_from_min_step_size(self.cfg.diagonal_n_steps, self.bound),
+            float(g.mean_density_up_to_cascade(1)))
+        from .volrend import packbits
+        g.occ_mask, g.bitfield = packbits(thr, g.density)
+        g.alive = all_idx[alive]
+        counts = [int(alive[c * G3:(c + 1) * G3].sum()) for c in range(K)]
+        g.alive_offsets = np.cumsum([0] + counts).tolist()
+
+    def update_ogrid_density(self, cas: int, update_all: bool, max_inference: int):
+        """types.py:1149-1225: decay alive cells by 0.95, then max with the field's
+        density at a random point of each selected cell."""
+        g, G = self.grid, self.cfg.grid_res
+        G3 = G ** 3
+        sl = slice(cas * G3, (cas + 1) * G3)
+        aligned = g.alive[g.alive_offsets[cas]:g.alive_offsets[cas + 1]] % G3
+        n_grids = aligned.shape[0]
+        if n_grids == 0:
+            return
+        dens = g.density[sl]
+        dens[aligned] = dens[aligned] * 0.95
+        if update_all:
+            upd = aligned
+        else:
+            M = max(1, n_grids // 2)
+            h = max(1, M // 2)
+            first = aligned[torch.randint(0, n_grids, (h,), device=self.dev, generator=self.rng)]
+            occ = g.occ_mask[sl][aligned].float()
+            if bool(occ.sum() > 0):
+                second = aligned[torch.multinomial(occ, h, replacement=True, generator=self.rng)]
+            else:
+                second = aligned[torch.randint(0, n_grids, (h,), device=self.dev, generator=self.rng)]
+            upd = torch.cat([first, second])
+        mip = min(self.bound, 2 ** cas)
+        hcw = mip / G
+        coords = cell_centers(upd, G, cas, self.bound)
+        coords = coords + (torch.rand(coords.shape, device=self.dev, generator=self.rng) * 2 - 1) * hcw
+        new = torch.empty(upd.shape[0], dtype=torch.float32, device=self.dev)
+        with torch.no_grad():
+            for a in range(0, upd.shape[0], max_inference):
+                new[a:a + max_inference] = self.model(coords[a:a + max_inference].contiguous(), None)[0].reshape(-1)
+        dens[upd] = torch.maximum(dens[upd], new)
+
+    def threshold_ogrid(self):
+        """types.py:1227-1239."""
+        g = self.grid
+        thr = torch.minimum(
+            torch.tensor(density_threshold_from_min_step_size(self.cfg.diagonal_n_steps, self.bound),
+                         device=self.dev), g.mean_density_up_to_cascade(1))
+        from .volrend import packbits
+        g.occ_mask, g.bitfield = packbits(thr, g.density)
+
+    @property
+    def update_ogrid_interval(self) -> int:
+        return min(16, self.step_count // 16 + 1)
+
+    def maybe_update_grid(self):
+        """train.py:75-85 with the cadence of types.py:1379-1392."""
+        s = self.step_count
+        if s > 0 and s % self.update_ogrid_interval == 0:
+            for cas in range(self.K):
+                self.update_ogrid_density(cas, update_all=s < 256, max_inference=self.cfg.total_samples)
+            self.threshold_ogrid()
+
+    # -- one step -----------------------------------------------------------------------
+
+    def step(self) -> None:
+        """One training step (app/nerf/_utils.py:80-170), then the grid cadence."""
+        c, b, m = self.cfg, self.b, self.model
+        enc_mod = m.position_encoder
+        table = enc_mod.table
+        S, n = c.total_samples, self.n_rays
+        cam, data = self.data.camera, self.data
+        flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg else 0)
+        _lib.call("train_make_rays", [data.transforms, data.rgba, b["o"], b["d"], b["t0"], b["t1"], b["noise"],
+                                      b["bgs"], b["gt"], b["pix"]],
+                  _lib.train_rays_descriptor(n, data.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy,
+                                             self.bound, c.seed, self.step_count, flags, c.bg))
+        _lib.call("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield, b["next_loc"],
+                                 b["exceeded"], b["valid"], b["ns"], b["start"], b["idcs"], b["xyzs"], b["dirs"],
+                                 b["dss"], b["zs"]],
+                  _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion))
+        hdesc = enc_mod.descriptor(S, self.bound)
+        _lib.call("hashgrid_encode", [b["xyzs"], table.detach(), b["enc"]], hdesc)
+        params = m.mlp_params.detach()
+        _lib.call("nerf_mlp_forward", [b["enc"], b["dirs"], params, b["drgbs"]], _lib.nerf_mlp_descriptor(S))
+        _lib.call("integrate_rays", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"], b["measured"],
+                                     b["rgbd"], b["op"]], _lib.integrating_descriptor(n, S))
+        _lib.call("train_loss_grad", [b["rgbd"], b["gt"], b["valid"], b["n_valid"], b["g_rgbd"], b["loss"]],
+                  _lib.loss_descriptor(n, 0.1))
+        _lib.call("integrate_rays_backward", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"],
+                                              b["rgbd"], b["op"], b["g_rgbd"], b["dbgs"], b["dz"], b["ddrgbs"]],
+                  _lib.integrating_backward_descriptor(n, S, c.near))
+        _lib.call("nerf_mlp_backward", [b["enc"], b["dirs"], params, b["ddrgbs"], b["denc"], b["dparams"], b["ws"]],
+                  _lib.nerf_mlp_descriptor(S, b["ws"].shape[0]))
+        _lib.call("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"]], hdesc)
+        self.step_count += 1
+        t = self.step_count
+        lr = lr_at(t - 1, c.lr)
+        b1, b2 = 0.9, 0.99
+        bp1, bp2 = float(np.float32(b1) ** t), float(np.float32(b2) ** t)
+        o = self.opt
+        mirror = table.detach() if table.dtype == torch.float16 else None
+        tab_master = table.detach() if mirror is None else self.table_master
+        _lib.call("adam_update", [tab_master, b["dtable"], o["tm"], o["tv"]] + ([mirror] if mirror is not None else []),
+                  _lib.adam_descriptor(tab_master.numel(), lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
+                                       mirror_dtype=_lib.DTYPE_F16 if mirror is not None else _lib.DTYPE_F32))
+        _lib.call("adam_update", [params, b["dparams"], o["mm"], o["mv"]],
+                  _lib.adam_descriptor(N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6))
+        self.maybe_update_grid()
+
+    # -- metrics (these synchronise) -------------------------------------------------------
+
+    def metrics(self) -> dict:
+        b = self.b
+        u = lambda t: int(t.view(torch.int32)[0].item())
+        return dict(loss=float(b["loss"][0].item()), n_valid_rays=u(b["n_valid"]),
+                    measured_batch_size_before_compaction=u(b["next_loc"]) - u(b["exceeded"]),
+                    measured_batch_size=u(b["measured"]))

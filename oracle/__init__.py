@@ -417,3 +417,66 @@ def mlp_backward64(enc, dirs, params, g):
     dX = dh1 @ W1.T
     grads = [a["X"].T @ dh1, a["h1"].T @ do, a["cin"].T @ dh2, a["h2"].T @ dh3, a["h3"].T @ dr]
     return dX, np.concatenate([gw.reshape(-1) for gw in grads])
+
+
+# ---------------------------------------------------------------------------
+# float32 CPU training step (the timed CPU baseline; same algorithm, host cores)
+# ---------------------------------------------------------------------------
+
+def mlp_forward32(enc, dirs, params):
+    W = [w.astype(np.float32) for w in unpack_mlp(params)]
+    X = np.asarray(enc, np.float32)
+    h1 = np.maximum(X @ W[0], 0)
+    o = h1 @ W[1]
+    cin = np.concatenate([o, sh4_64(dirs).astype(np.float32)], 1)
+    h2 = np.maximum(cin @ W[2], 0)
+    h3 = np.maximum(h2 @ W[3], 0)
+    rgb = 1 / (1 + np.exp(-(h3 @ W[4])))
+    return np.concatenate([np.exp(o[:, :1]), rgb], 1), (X, h1, o, cin, h2, h3, rgb, W)
+
+
+def mlp_backward32(cache, g):
+    X, h1, o, cin, h2, h3, rgb, W = cache
+    dr = g[:, 1:] * rgb * (1 - rgb)
+    dh3 = (dr @ W[4].T) * (h3 > 0)
+    dh2 = (dh3 @ W[3].T) * (h2 > 0)
+    dcin = dh2 @ W[2].T
+    do = dcin[:, :16].copy()
+    do[:, 0] += g[:, 0] * np.exp(np.clip(o[:, 0], -15, 15))
+    dh1 = (do @ W[1].T) * (h1 > 0)
+    dX = dh1 @ W[0].T
+    dP = np.concatenate([(X.T @ dh1).ravel(), (h1.T @ do).ravel(), (cin.T @ dh2).ravel(), (h2.T @ dh3).ravel(),
+                         (h3.T @ dr).ravel()])
+    return dX, dP
+
+
+def cpu_train_step(rays_o, rays_d, t_starts, t_ends, noises, bits, bgs, gt, table, params, lv, T, total_samples):
+    """One plain training step on the CPU with the oracle's ops (marcher, hash
+    encoder, MLP, integrator, Huber gradient, backward passes).  Returns the number
+    of marched samples (the samples/s numerator)."""
+    L, F = len(lv["scales_f32"]), table.shape[1]
+    m = march_rays(total_samples, 1024, 1, 128, 1.0, 0.0, rays_o, rays_d, t_starts, t_ends, noises, bits)
+    S = total_samples
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    scales = np.ascontiguousarray(lv["scales_f32"], np.float32)
+    res = np.ascontiguousarray(lv["resolutions"], np.uint32)
+    offs = np.ascontiguousarray(lv["offsets"], np.uint32)
+    enc = np.zeros((S, L * F), np.float32)
+    lib().oracle_hashgrid_fwd(u32(S), u32(L), u32(F), u32(T), u32(lv["first_hash_level"]), u32(lv["n_rows"]),
+                              f32(1.0), P(scales), P(res), P(offs), P(m["xyzs"]), P(table), P(enc))
+    drgbs, cache = mlp_forward32(enc, m["dirs"], params)
+    drgbs = np.ascontiguousarray(drgbs, np.float32)
+    _, rgbd, op = integrate_rays(m["rays_sample_startidx"], m["rays_n_samples"], bgs, m["dss"], m["z_vals"], drgbs)
+    valid = m["ray_is_valid"]
+    nv = max(int(valid.sum()), 1)
+    e = rgbd[:, :3] - gt
+    g = np.zeros((len(valid), 4), np.float32)
+    g[:, :3] = np.where(valid[:, None], np.clip(e, -0.1, 0.1) / (3 * nv), 0)
+    _, _, dd = integrate_rays_backward(0.3, m["rays_sample_startidx"], m["rays_n_samples"], bgs, m["dss"],
+                                       m["z_vals"], drgbs, rgbd, op, g)
+    dX, _ = mlp_backward32(cache, dd)
+    dt = np.zeros((lv["n_rows"], F), np.float64)
+    dX = np.ascontiguousarray(dX, np.float32)
+    lib().oracle_hashgrid_bwd(u32(S), u32(L), u32(F), u32(T), u32(lv["first_hash_level"]), u32(lv["n_rows"]),
+                              f32(1.0), P(scales), P(res), P(offs), P(m["xyzs"]), P(dX), P(dt))
+    return int(m["measured"])

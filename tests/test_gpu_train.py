@@ -1,0 +1,112 @@
+"""GPU tests of the training harness: the fused ray generation against a torch
+restatement, the loss gradient, Adam against the optax update rule, and that a few
+hundred steps on the synthetic scene reduce the loss."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from jaxngp_amd import _lib, scene as sc
+from jaxngp_amd.train import TrainConfig, Trainer
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_data():
+    dev = torch.device("cuda:0")
+    cam = sc.Camera(width=100, height=100, fx=139.0, fy=139.0, cx=50.0, cy=50.0)
+    blobs = sc.BlobScene.random(seed=3)
+    poses = sc.look_at_poses(12, seed=1)
+    return sc.make_views(blobs, cam, poses, dev, n_steps=256)
+
+
+def test_make_rays_matches_torch(small_data):
+    d = small_data
+    dev = d.rgba.device
+    n = 4096
+    out = {k: torch.empty(n, w, device=dev) for k, w in [("o", 3), ("d", 3), ("bgs", 3), ("gt", 3)]}
+    t0, t1, noise = (torch.empty(n, device=dev) for _ in range(3))
+    pix = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    cam = d.camera
+    _lib.call("train_make_rays", [d.transforms, d.rgba, out["o"], out["d"], t0, t1, noise, out["bgs"], out["gt"], pix],
+              _lib.train_rays_descriptor(n, d.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy, 1.0,
+                                         123, 7, _lib.TRAIN_PERTURB | _lib.TRAIN_RANDOM_BG, (1.0, 1.0, 1.0)))
+    p = pix.view(torch.int32).long()
+    view, px = p // cam.n_pixels, p % cam.n_pixels
+    assert (view < d.n_views).all()
+    from jaxngp_amd.renderer import make_near_far_from_bound, make_rays_worldspace
+    for v in range(d.n_views):
+        sel = view == v
+        if not sel.any():
+            continue
+        o_all, d_all = make_rays_worldspace(cam, torch.as_tensor(d.poses[v], dtype=torch.float32, device=dev))
+        torch.testing.assert_close(out["o"][sel], o_all[px[sel]], rtol=0, atol=1e-6)
+        torch.testing.assert_close(out["d"][sel], d_all[px[sel]], rtol=0, atol=2e-6)
+    ts, te = make_near_far_from_bound(1.0, out["o"], out["d"])
+    torch.testing.assert_close(t0, ts, rtol=1e-5, atol=1e-5)
+    torch.testing.assert_close(t1, te, rtol=1e-5, atol=1e-5)
+    assert ((noise >= 0) & (noise < 1)).all() and ((out["bgs"] >= 0) & (out["bgs"] < 1)).all()
+    rgba = d.rgba.reshape(-1, 4)[p].float() / 255
+    want = rgba[:, :3] * rgba[:, 3:] + out["bgs"] * (1 - rgba[:, 3:])
+    torch.testing.assert_close(out["gt"], want, rtol=1e-6, atol=1e-6)
+    # uniform over pixels and views
+    assert abs(float(noise.mean()) - 0.5) < 0.03 and abs(float(view.float().mean()) - (d.n_views - 1) / 2) < 0.5
+
+
+def test_loss_grad_and_adam(small_data):
+    dev = small_data.rgba.device
+    rng = np.random.default_rng(0)
+    n = 1000
+    rgbd = torch.from_numpy(rng.uniform(0, 1, (n, 4)).astype(np.float32)).to(dev)
+    gt = torch.from_numpy(rng.uniform(0, 1, (n, 3)).astype(np.float32)).to(dev)
+    valid = torch.from_numpy(rng.uniform(size=n) < 0.7).to(dev)
+    nv = torch.empty(1, dtype=torch.int32, device=dev).view(torch.uint32)
+    g = torch.empty(n, 4, device=dev)
+    loss = torch.empty(1, device=dev)
+    _lib.call("train_loss_grad", [rgbd, gt, valid, nv, g, loss], _lib.loss_descriptor(n, 0.1))
+    pred = rgbd[:, :3].clone().requires_grad_(True)
+    err = pred - gt
+    hub = torch.where(err.abs() <= 0.1, 0.5 * err ** 2, 0.1 * (err.abs() - 0.05))
+    ref = torch.where(valid, hub.mean(-1), torch.zeros(n, device=dev)).sum() / valid.sum()
+    ref.backward()
+    assert int(nv.view(torch.int32)[0]) == int(valid.sum())
+    torch.testing.assert_close(loss[0], ref.detach(), rtol=1e-5, atol=1e-7)
+    torch.testing.assert_close(g[:, :3], pred.grad, rtol=1e-5, atol=1e-9)
+    assert (g[:, 3] == 0).all()
+    # Adam: optax scale_by_adam + scale(-lr) + add_decayed_weights
+    N = 5000
+    p = torch.from_numpy(rng.normal(size=N).astype(np.float32)).to(dev)
+    m = torch.zeros(N, device=dev)
+    v = torch.zeros(N, device=dev)
+    p_ref = p.double().clone()
+    m_ref = torch.zeros(N, dtype=torch.float64, device=dev)
+    v_ref = torch.zeros(N, dtype=torch.float64, device=dev)
+    for t in range(1, 6):
+        gr = torch.from_numpy(rng.normal(size=N).astype(np.float32)).to(dev)
+        gr[::7] = 0
+        _lib.call("adam_update", [p, gr, m, v], _lib.adam_descriptor(N, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9 ** t,
+                                                                     0.99 ** t, 1e-6))
+        m_ref = 0.9 * m_ref + 0.1 * gr.double()
+        v_ref = 0.99 * v_ref + 0.01 * gr.double() ** 2
+        u = -1e-2 * (m_ref / (1 - 0.9 ** t)) / (torch.sqrt(v_ref / (1 - 0.99 ** t) + 1e-15) + 1e-15) + 1e-6 * p_ref
+        p_ref = p_ref + u
+        torch.testing.assert_close(p.double(), p_ref, rtol=1e-5, atol=1e-6)
+
+
+def test_training_reduces_loss(small_data):
+    cfg = TrainConfig(total_samples=1 << 16)
+    tr = Trainer(small_data, cfg)
+    losses = []
+    for i in range(300):
+        tr.step()
+        if i % 50 == 0 or i == 299:
+            m = tr.metrics()
+            losses.append(m["loss"])
+            assert m["n_valid_rays"] > 0 and m["measured_batch_size_before_compaction"] > 0
+            assert math.isfinite(m["loss"])
+    assert losses[-1] < 0.5 * losses[0], losses
+    # the occupancy grid was thresholded: some cells empty, some occupied
+    occ = tr.grid.occ_mask.float().mean().item()
+    assert 0.0 < occ < 1.0
