@@ -96,6 +96,12 @@ typedef struct PackbitsDescriptor {
 #define NGP_HASHGRID_MAX_LEVELS 32
 #define NGP_DTYPE_F32 0u
 #define NGP_DTYPE_F16 1u
+/* Layout of encodings and their gradients: point-major [N, L*F] (the reference's
+ * HashGridEncoder output, enc[n, l*F + f]) or level-major [L, N, F]
+ * (enc[(l*N + n)*F + f]; used inside the fused training step, where it keeps the
+ * per-level work of the encoder on one XCD with coalesced accesses). */
+#define NGP_ENC_POINT_MAJOR 0u
+#define NGP_ENC_LEVEL_MAJOR 1u
 typedef struct HashGridDescriptor {
     uint32_t n_points;          /* N */
     uint32_t L;                 /* levels, 1..NGP_HASHGRID_MAX_LEVELS */
@@ -105,6 +111,7 @@ typedef struct HashGridDescriptor {
     uint32_t table_dtype;       /* NGP_DTYPE_F32 or NGP_DTYPE_F16 */
     uint32_t n_rows;            /* offsets[L], rows of the table */
     float bound;                /* positions are normalised as (pos + bound) / (2 bound) */
+    uint32_t enc_layout;        /* NGP_ENC_POINT_MAJOR or NGP_ENC_LEVEL_MAJOR */
     float scales[NGP_HASHGRID_MAX_LEVELS];
     uint32_t resolutions[NGP_HASHGRID_MAX_LEVELS];
     uint32_t offsets[NGP_HASHGRID_MAX_LEVELS + 1];
@@ -121,6 +128,7 @@ typedef struct HashGridDescriptor {
 typedef struct NerfMlpDescriptor {
     uint32_t n_samples;
     uint32_t n_workspace_blocks;  /* rows of the backward workspace (0 for forward) */
+    uint32_t enc_layout;          /* layout of encodings / dL_dencodings (L=16, F=2) */
 } NerfMlpDescriptor;
 
 /* ---------------------------------------------------------------------------
@@ -186,26 +194,37 @@ void integrate_rays_inference(ngp_stream_t stream, void **buffers, const char *o
 /* Forward of HashGridEncoder.__call__ (models/encoders.py:82-233).
  * opaque: HashGridDescriptor.
  * in : pos f32[N,3], table f32|f16[n_rows,F]
- * out: encodings f32[N, L*F]   (level-major per point: enc[n, l*F + f]) */
+ * out: encodings f32[N, L*F] (enc_layout NGP_ENC_POINT_MAJOR) or f32[L, N, F]
+ *      (NGP_ENC_LEVEL_MAJOR) */
 void hashgrid_encode(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* Table gradient of HashGridEncoder (the VJP of the gather at encoders.py:226 and
  * of the weighted sum at encoders.py:231).  opaque: HashGridDescriptor.
- * in : pos f32[N,3], dL_dencodings f32[N, L*F]
- * out: dL_dtable f32[n_rows, F]   (overwritten, not accumulated into) */
+ * in : pos f32[N,3], dL_dencodings f32 in the descriptor's enc_layout
+ * out: dL_dtable f32[n_rows, F]   (overwritten, not accumulated into),
+ *      workspace u8[hashgrid_encode_backward_workspace_size(opaque)], 256-byte
+ *      aligned (unused, may be NULL, when N == 0)
+ * The gradient is scattered without global atomics: contributions are sorted
+ * into row slices that one workgroup sums in LDS. */
 void hashgrid_encode_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* Bytes of the workspace hashgrid_encode_backward needs for the descriptor
+ * `opaque` (no counterpart in the reference, whose scatter uses atomics on the
+ * output).  Returns 0 and sets the last error for an invalid descriptor. */
+size_t hashgrid_encode_backward_workspace_size(const char *opaque, size_t opaque_len);
 
 /* Forward of NeRF.__call__ after the position encoder (models/nerfs.py:66-86).
  * opaque: NerfMlpDescriptor.
- * in : encodings f32[S,32], dirs f32[S,3], params f32[NGP_MLP_N_PARAMS]
+ * in : encodings f32[S,32] or f32[16,S,2] (enc_layout), dirs f32[S,3],
+ *      params f32[NGP_MLP_N_PARAMS]
  * out: drgbs f32[S,4]   (density, r, g, b) */
 void nerf_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* Backward of nerf_mlp_forward (JAX autodiff of nerfs.py:66-86 with trunc_exp's
  * custom VJP, nerfs.py:221-238).  opaque: NerfMlpDescriptor.
- * in : encodings f32[S,32], dirs f32[S,3], params f32[NGP_MLP_N_PARAMS],
+ * in : encodings (as forward), dirs f32[S,3], params f32[NGP_MLP_N_PARAMS],
  *      dL_ddrgbs f32[S,4]
- * out: dL_dencodings f32[S,32], dL_dparams f32[NGP_MLP_N_PARAMS],
+ * out: dL_dencodings (same layout as encodings), dL_dparams f32[NGP_MLP_N_PARAMS],
  *      workspace f32[n_workspace_blocks, NGP_MLP_N_PARAMS] */
 void nerf_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 

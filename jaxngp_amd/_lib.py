@@ -67,6 +67,8 @@ def load() -> ctypes.CDLL:
     lib.ngp_get_last_error.restype = ctypes.c_int
     lib.ngp_version.argtypes = []
     lib.ngp_version.restype = ctypes.c_char_p
+    lib.hashgrid_encode_backward_workspace_size.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.hashgrid_encode_backward_workspace_size.restype = ctypes.c_size_t
     _lib = lib
     return lib
 
@@ -116,6 +118,15 @@ def call(name: str, tensors: Sequence[torch.Tensor], opaque: bytes, stream: int 
     _check()
 
 
+def hashgrid_backward_workspace_bytes(opaque: bytes) -> int:
+    """Workspace bytes of hashgrid_encode_backward for descriptor `opaque`."""
+    n = load().hashgrid_encode_backward_workspace_size(opaque, len(opaque))
+    if n == 0:
+        _check()
+        raise NGPError("hashgrid_encode_backward_workspace_size: invalid descriptor")
+    return n
+
+
 def call_raw(name: str, ptrs: Sequence[int], opaque: bytes, stream: int = 0) -> None:
     """Calls target `name` with raw pointers (host-side error tests use NULLs)."""
     arr = (ctypes.c_void_p * max(1, len(ptrs)))(*ptrs)
@@ -157,20 +168,24 @@ def packbits_descriptor(n_bytes) -> bytes:
     return struct.pack("<I", n_bytes)
 
 
+ENC_POINT_MAJOR = 0
+ENC_LEVEL_MAJOR = 1
+
+
 def hashgrid_descriptor(n_points, L, F, T, first_hash_level, table_dtype, n_rows, bound, scales, resolutions,
-                        offsets) -> bytes:
+                        offsets, enc_layout=ENC_POINT_MAJOR) -> bytes:
     M = HASHGRID_MAX_LEVELS
     if not (1 <= L <= M):
         raise ValueError(f"L={L} must be in [1, {M}]")
     sc = list(scales) + [0.0] * (M - L)
     rs = list(resolutions) + [0] * (M - L)
     of = list(offsets) + [offsets[-1]] * (M + 1 - len(offsets))
-    return struct.pack(f"<7If{M}f{M}I{M + 1}I", n_points, L, F, T, first_hash_level, table_dtype, n_rows, bound,
-                       *sc, *rs, *of)
+    return struct.pack(f"<7IfI{M}f{M}I{M + 1}I", n_points, L, F, T, first_hash_level, table_dtype, n_rows, bound,
+                       enc_layout, *sc, *rs, *of)
 
 
-def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0) -> bytes:
-    return struct.pack("<2I", n_samples, n_workspace_blocks)
+def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0, enc_layout=ENC_POINT_MAJOR) -> bytes:
+    return struct.pack("<3I", n_samples, n_workspace_blocks, enc_layout)
 
 
 TRAIN_PERTURB = 1

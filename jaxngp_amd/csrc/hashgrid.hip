@@ -6,14 +6,14 @@
 // registers: normalise, scale, floor, 8 vertex indices (dense x + y res + z res^2
 // below first_hash_level, else x ^ y*2654435761 ^ z*805459861, both mod T, plus the
 // level's row offset), trilinear weights, weighted sum of the gathered rows.
-// Threads are laid out level-fastest, so the L*F outputs of a point are written by
-// L consecutive lanes as one contiguous run.
+// Point-major output: threads are laid out level-fastest, so the L*F outputs of a
+// point are written by L consecutive lanes as one contiguous run.  Level-major
+// output: workgroups are level-affine per XCD (see level_block) so each level's
+// gathers hit one L2.
 //
 // Backward is the VJP of the gather (encoders.py:226) and of the weighted sum
-// (encoders.py:231): dtable[idx] += w * g, with hardware f32 atomics.  Before the
-// atomic, lanes of the same level whose points fall on the same table row (the
-// common case on coarse levels, where consecutive samples of a ray share a cell)
-// pre-add their contributions across the wave, so one atomic carries them.
+// (encoders.py:231): dtable[idx] += w * g.  It is computed by row owners in LDS (see
+// hashgrid_bwd_owner_kernel), without global atomics.
 //
 // Arithmetic: f32, no contraction (matches the CPU oracle bit for bit on indices
 // and weights).  Indices past the last row (only possible when the last level is
@@ -98,68 +98,424 @@ __global__ void __launch_bounds__(kBlock)
     for (int f = 0; f < F; ++f) out[f] = acc[f];
 }
 
-template <int F>
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip (block b runs
+// on XCD b % 8), each with its own 4 MiB L2.  Level-affine kernels therefore give
+// block b the level 8 * phase + b % 8: all blocks of one XCD work on the same level
+// in a phase, so that level's rows (at most T * F * 4 B = 4 MiB for T = 2^19, F = 2)
+// stay resident in that XCD's L2 for the gathers / atomics, instead of every L2
+// streaming the whole 46.5 MiB table.
+constexpr uint32_t kXcds = 8;
+constexpr int kPointsPerBlock = kBlock;
+
+struct LevelBlock {
+    uint32_t level, point0;
+    bool live;
+};
+
+__device__ __forceinline__ LevelBlock level_block(uint32_t n_points, uint32_t L) {
+    const uint32_t n_pg = (n_points + kPointsPerBlock - 1) / kPointsPerBlock;
+    const uint32_t b = blockIdx.x;
+    const uint32_t x = b % kXcds, rest = b / kXcds;
+    const uint32_t pg = rest % n_pg, phase = rest / n_pg;
+    LevelBlock lb;
+    lb.level = phase * kXcds + x;
+    lb.point0 = pg * kPointsPerBlock;
+    lb.live = lb.level < L;
+    return lb;
+}
+
+inline uint32_t level_grid(uint32_t n_points, uint32_t L) {
+    const uint32_t n_pg = div_up(n_points, kPointsPerBlock);
+    return kXcds * n_pg * div_up(L, kXcds);
+}
+
+// Level-major forward: one thread per point of the block's level, output
+// enc[(l * N + n) * F + f] written as one contiguous run per wavefront.
+template <int F, typename TT>
 __global__ void __launch_bounds__(kBlock)
-    hashgrid_bwd_kernel(HashGridDescriptor d, const float *__restrict__ pos, const float *__restrict__ dL_denc,
-                        float *__restrict__ dtable) {
-    __shared__ LevelTables lt;
-    if (threadIdx.x < d.L) {
-        lt.scale[threadIdx.x] = d.scales[threadIdx.x];
-        lt.res[threadIdx.x] = d.resolutions[threadIdx.x];
-        lt.offset[threadIdx.x] = d.offsets[threadIdx.x];
-    }
-    __syncthreads();
-    const uint64_t total = (uint64_t)d.n_points * d.L;
-    const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool active = gid < total;
-    const uint32_t n = active ? (uint32_t)(gid / d.L) : 0u;
-    const uint32_t l = active ? (uint32_t)(gid - (uint64_t)n * d.L) : 0u;
+    hashgrid_fwd_lm_kernel(HashGridDescriptor d, const float *__restrict__ pos, const TT *__restrict__ table,
+                           float *__restrict__ enc) {
+    const LevelBlock lb = level_block(d.n_points, d.L);
+    if (!lb.live) return;
+    const uint32_t n = lb.point0 + threadIdx.x;
+    if (n >= d.n_points) return;
+    const uint32_t l = lb.level;
     const float two_b = 2.0f * d.bound;
     const float px = (pos[3 * (size_t)n + 0] + d.bound) / two_b;
     const float py = (pos[3 * (size_t)n + 1] + d.bound) / two_b;
     const float pz = (pos[3 * (size_t)n + 2] + d.bound) / two_b;
     uint32_t idx[8];
     float w[8];
-    level_vertices(px, py, pz, lt.scale[l], lt.res[l], l < d.first_hash_level, d.T, (d.T & (d.T - 1u)) == 0,
-                   lt.offset[l], d.n_rows - 1u, idx, w);
-    float g[F];
+    level_vertices(px, py, pz, d.scales[l], d.resolutions[l], l < d.first_hash_level, d.T, (d.T & (d.T - 1u)) == 0,
+                   d.offsets[l], d.n_rows - 1u, idx, w);
+    float acc[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) g[f] = active ? dL_denc[gid * F + f] : 0.0f;
-    // Lanes l, l+L, l+2L, ... of a wave hold the same level for consecutive points.
-    // When L divides 64, fold lane pairs at distances 32, 16, ... >= L that carry
-    // the same row: the lower lane adds the upper lane's contribution.
-    const int lane = lane_id();
-    const bool fold = active && (kWave % d.L) == 0;
+    for (int f = 0; f < F; ++f) acc[f] = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        float v[F];
+        const TT *row = table + (size_t)idx[c] * F;
 #pragma unroll
-        for (int f = 0; f < F; ++f) v[f] = w[c] * g[f];
-        bool owner = active;
-        uint32_t my = idx[c];
-        for (int dist = kWave / 2; dist >= (int)d.L; dist >>= 1) {
-            // every lane takes part in the shuffles; `fold` decides whether to merge
-            const uint32_t other_idx = __shfl_xor(my, dist, kWave);
-            const int other_owner = __shfl_xor(owner ? 1 : 0, dist, kWave);
-            float ov[F];
+        for (int f = 0; f < F; ++f) acc[f] = acc[f] + to_f32<TT>(row[f]) * w[c];
+    }
+    float *out = enc + ((size_t)l * d.n_points + n) * F;
 #pragma unroll
-            for (int f = 0; f < F; ++f) ov[f] = __shfl_xor(v[f], dist, kWave);
-            if (fold && other_idx == my && other_owner && owner) {
-                if ((lane & dist) == 0) {
+    for (int f = 0; f < F; ++f) out[f] = acc[f];
+}
+
+// Backward without global atomics.  Device-scope atomics on this chip (float or
+// integer, any scope) retire at ~21-27 G/s for the whole device on scattered
+// addresses, two orders of magnitude below the gather rate, so contributions are
+// routed to LDS by a deterministic counting sort over row slices of R rows
+// (R rows of F floats fill 128 KiB of LDS):
+//   1. count:   work item = (level, 1024-point chunk); LDS histogram of the item's
+//               contributions per slice, stored slice-major;
+//   2. scan:    exclusive scan of the counts -> each item's write offset per slice;
+//   3. scatter: the item recomputes its contributions (with dL_dencodings) and
+//               writes (row within slice, values) entries at its offsets;
+//   4. reduce:  workgroup s sums slice s's contiguous entries with LDS atomics and
+//               stores its rows once (so no table zero-fill either).
+// Within an item, lanes are 64 consecutive samples of one level; runs of equal rows
+// (coarse levels: consecutive samples of a ray share cells) are summed with a
+// segmented scan first, so one entry carries a run.
+constexpr int kChunk = 1024;         // points per work item
+constexpr int kItemThreads = 256;    // 4 points per thread per item
+constexpr int kReduceThreads = 1024;
+constexpr uint32_t kPart = 32768;    // entries one reduce workgroup sums
+constexpr uint32_t kMaxSlices = 4096;
+constexpr uint32_t kSliceBytes = 131072;
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 4;  // elements per thread in the scan kernels
+
+// An entry is (row within slice, F values), padded to a power of two of words so
+// that it is written and read with one or two vector accesses.
+template <int F>
+constexpr int entry_words() {
+    return F == 1 ? 2 : (F == 2 ? 4 : (F == 4 ? 8 : 16));
+}
+
+struct BwdPlan {
+    uint32_t R, n_slices, n_chunks, n_items;
+    uint64_t M;             // n_slices * n_items counters
+    uint64_t scan_blocks;   // blocks of the counter scan
+    uint32_t max_parts;     // upper bound of the reduce grid
+    uint64_t counts_off, sums_off, meta_off, parts_off, partials_off, entries_off, bytes;
+};
+
+inline BwdPlan make_plan(const HashGridDescriptor &d) {
+    auto align = [](uint64_t x) { return (x + 255) / 256 * 256; };
+    BwdPlan p;
+    p.R = std::max<uint32_t>(64, kSliceBytes / (4 * d.F));
+    p.n_slices = div_up(d.n_rows, p.R);
+    p.n_chunks = std::max<uint32_t>(1, div_up(d.n_points, kChunk));
+    p.n_items = d.L * p.n_chunks;
+    p.M = (uint64_t)p.n_slices * p.n_items;
+    p.scan_blocks = (p.M + kScanThreads * kScanPer - 1) / (kScanThreads * kScanPer);
+    const uint64_t max_entries = (uint64_t)d.n_points * d.L * 8;
+    // a slice with c entries takes max(1, ceil(c / kPart)) parts
+    p.max_parts = (uint32_t)(p.n_slices + max_entries / kPart);
+    const int ew = d.F == 1 ? 2 : (d.F == 2 ? 4 : (d.F == 4 ? 8 : 16));
+    p.counts_off = 0;
+    p.sums_off = align(p.counts_off + 4 * p.M);
+    p.meta_off = align(p.sums_off + 4 * p.scan_blocks);        // [0] total entries, [1] total parts
+    p.parts_off = align(p.meta_off + 8);                          // first part of each slice
+    p.partials_off = align(p.parts_off + 4 * ((uint64_t)p.n_slices + 1));
+    p.entries_off = align(p.partials_off + (uint64_t)p.max_parts * p.R * d.F * 4);
+    p.bytes = p.entries_off + max_entries * 4 * ew;
+    return p;
+}
+
+// Visits the run-merged contributions of work item `item` and calls
+// emit(row, v) on each run's last lane.  The same inputs always give the same
+// multiset of emitted rows, so counting and scattering agree.
+template <int F, bool kValues, typename Emit>
+__device__ __forceinline__ void item_contributions(const HashGridDescriptor &d, const BwdPlan &p, uint32_t item,
+                                                   const float *__restrict__ pos, const float *__restrict__ dL_denc,
+                                                   Emit emit) {
+    const uint32_t l = item / p.n_chunks, k = item % p.n_chunks;
+    const int lane = lane_id();
+    const float two_b = 2.0f * d.bound;
+    const bool pow2 = (d.T & (d.T - 1u)) == 0;
+    const bool level_major = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
+    const float scale = d.scales[l];
+    const uint32_t res = d.resolutions[l];
+    const bool dense = l < d.first_hash_level;
+    for (int sub = 0; sub < kChunk / kItemThreads; ++sub) {
+        const uint32_t n = k * kChunk + sub * kItemThreads + threadIdx.x;
+        const bool active = n < d.n_points;
+        const uint32_t np = active ? n : d.n_points - 1;
+        const float px = (pos[3 * (size_t)np + 0] + d.bound) / two_b;
+        const float py = (pos[3 * (size_t)np + 1] + d.bound) / two_b;
+        const float pz = (pos[3 * (size_t)np + 2] + d.bound) / two_b;
+        uint32_t idx[8];
+        float w[8];
+        level_vertices(px, py, pz, scale, res, dense, d.T, pow2, d.offsets[l], d.n_rows - 1u, idx, w);
+        float g[F];
 #pragma unroll
-                    for (int f = 0; f < F; ++f) v[f] += ov[f];
-                } else {
-                    owner = false;
-                }
-            }
+        for (int f = 0; f < F; ++f) g[f] = 0.0f;
+        if (kValues && active) {
+            const size_t gb = level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
+#pragma unroll
+            for (int f = 0; f < F; ++f) g[f] = dL_denc[gb + f];
         }
-        if (owner) {
-            float *dst = dtable + (size_t)my * F;
 #pragma unroll
-            for (int f = 0; f < F; ++f)
-                if (v[f] != 0.0f) unsafeAtomicAdd(dst + f, v[f]);
+        for (int c = 0; c < 8; ++c) {
+            // inactive lanes get keys no active lane has, so they form no runs
+            const uint32_t key = active ? idx[c] : 0xffffffffu - (uint32_t)lane;
+            float v[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) v[f] = w[c] * g[f];
+            const uint32_t prev = __shfl_up(key, 1, kWave);
+            const bool head = lane == 0 || prev != key;
+            bool tail = true;
+            if (__ballot(!head) != 0) {
+                if (kValues) {
+                    int seg = head ? lane : 0;
+#pragma unroll
+                    for (int dd = 1; dd < kWave; dd <<= 1) seg = max(seg, __shfl_up(seg, dd, kWave));
+#pragma unroll
+                    for (int dd = 1; dd < kWave; dd <<= 1) {
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            const float o = __shfl_up(v[f], dd, kWave);
+                            if (lane - dd >= seg) v[f] += o;
+                        }
+                    }
+                }
+                const int next_head = __shfl_down(head ? 1 : 0, 1, kWave);
+                tail = lane == kWave - 1 || next_head != 0;
+            }
+            if (tail && active) emit(key, v);
         }
     }
+}
+
+// 1. per item, entries per slice (slice-major counters)
+template <int F>
+__global__ void __launch_bounds__(kItemThreads)
+    bwd_count_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos, uint32_t *__restrict__ counts) {
+    __shared__ uint32_t hist[kMaxSlices];
+    for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) hist[s] = 0u;
+    __syncthreads();
+    const uint32_t item = blockIdx.x;
+    item_contributions<F, false>(d, p, item, pos, nullptr,
+                                 [&](uint32_t row, const float *) { atomicAdd(&hist[row / p.R], 1u); });
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) counts[(size_t)s * p.n_items + item] = hist[s];
+}
+
+// 2a. per-block sums of kScanThreads * kScanPer counters
+__global__ void __launch_bounds__(kScanThreads) scan_sums_kernel(uint64_t M, const uint32_t *__restrict__ a,
+                                                                 uint32_t *__restrict__ sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanThreads * kScanPer;
+    uint32_t v = 0;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) {
+        const uint64_t i = base + (uint64_t)q * kScanThreads + threadIdx.x;
+        if (i < M) v += a[i];
+    }
+    __shared__ uint32_t ws[kScanThreads / kWave];
+    v = wave_sum(v);
+    if (lane_id() == 0) ws[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) t += ws[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// 2b. one workgroup: exclusive scan of the block sums, total entries into meta[0]
+__global__ void __launch_bounds__(kScanThreads) scan_block_sums_kernel(uint64_t n, uint32_t *__restrict__ sums,
+                                                                       uint32_t *__restrict__ meta) {
+    uint32_t carry = 0;
+    for (uint64_t c = 0; c < n; c += kScanThreads) {
+        const uint64_t i = c + threadIdx.x;
+        const uint32_t v = i < n ? sums[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kScanThreads>(v, &tot);
+        if (i < n) sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) meta[0] = carry;
+}
+
+// 2c. exclusive scan inside each block plus the block's prefix, in place
+__global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(uint64_t M, uint32_t *__restrict__ a,
+                                                                  const uint32_t *__restrict__ sums) {
+    const uint64_t base = (uint64_t)blockIdx.x * kScanThreads * kScanPer;
+    uint32_t carry = sums[blockIdx.x];
+    for (int q = 0; q < kScanPer; ++q) {
+        const uint64_t i = base + (uint64_t)q * kScanThreads + threadIdx.x;
+        const uint32_t v = i < M ? a[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kScanThreads>(v, &tot);
+        if (i < M) a[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__device__ __forceinline__ uint32_t slice_begin(const BwdPlan &p, const uint32_t *offsets, const uint32_t *meta,
+                                                uint32_t s) {
+    return s < p.n_slices ? offsets[(size_t)s * p.n_items] : meta[0];
+}
+
+// 2d. one workgroup: parts per slice (kPart entries each, at least one) and their
+// prefix; parts[n_slices] and meta[1] hold the total.
+__global__ void __launch_bounds__(kScanThreads) plan_parts_kernel(BwdPlan p, const uint32_t *__restrict__ offsets,
+                                                                  uint32_t *__restrict__ meta,
+                                                                  uint32_t *__restrict__ parts) {
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < p.n_slices; c += kScanThreads) {
+        const uint32_t s = c + threadIdx.x;
+        uint32_t k = 0;
+        if (s < p.n_slices) {
+            const uint32_t cnt = slice_begin(p, offsets, meta, s + 1) - slice_begin(p, offsets, meta, s);
+            k = cnt == 0 ? 1u : (cnt + kPart - 1) / kPart;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kScanThreads>(k, &tot);
+        if (s < p.n_slices) parts[s] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        parts[p.n_slices] = carry;
+        meta[1] = carry;
+    }
+}
+
+// 3. entries at each item's offsets
+template <int F>
+__global__ void __launch_bounds__(kItemThreads)
+    bwd_scatter_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos, const float *__restrict__ dL_denc,
+                       const uint32_t *__restrict__ offsets, uint32_t *__restrict__ entries) {
+    constexpr int EW = entry_words<F>();
+    __shared__ uint32_t cursor[kMaxSlices];
+    const uint32_t item = blockIdx.x;
+    for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) cursor[s] = offsets[(size_t)s * p.n_items + item];
+    __syncthreads();
+    item_contributions<F, true>(d, p, item, pos, dL_denc, [&](uint32_t row, const float *v) {
+        const uint32_t s = row / p.R;
+        const uint32_t at = atomicAdd(&cursor[s], 1u);
+        uint32_t w[EW];
+        w[0] = row - s * p.R;
+#pragma unroll
+        for (int f = 0; f < F; ++f) w[1 + f] = __float_as_uint(v[f]);
+#pragma unroll
+        for (int f = F + 1; f < EW; ++f) w[f] = 0u;
+        uint4 *e = reinterpret_cast<uint4 *>(entries + (size_t)at * EW);
+        if constexpr (EW == 2) {
+            *reinterpret_cast<uint2 *>(entries + (size_t)at * EW) = make_uint2(w[0], w[1]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < EW / 4; ++q) e[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+        }
+    });
+}
+
+// 4. workgroup b sums part b's entries in LDS.  A slice of one part writes its
+// rows of the gradient; a slice of several writes partial rows, summed in order
+// by combine_kernel.
+template <int F>
+__global__ void __launch_bounds__(kReduceThreads)
+    bwd_reduce_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ offsets,
+                      const uint32_t *__restrict__ meta, const uint32_t *__restrict__ parts,
+                      const uint32_t *__restrict__ entries, float *__restrict__ partials, float *__restrict__ dtable) {
+    constexpr int EW = entry_words<F>();
+    extern __shared__ float acc[];
+    const uint32_t b = blockIdx.x;
+    if (b >= meta[1]) return;
+    // slice s: the last with parts[s] <= b
+    uint32_t lo = 0, hi = p.n_slices - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (parts[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t s = lo;
+    const uint32_t part = b - parts[s];
+    const uint32_t n_parts = parts[s + 1] - parts[s];
+    const uint32_t s_begin = slice_begin(p, offsets, meta, s), s_end = slice_begin(p, offsets, meta, s + 1);
+    const uint32_t begin = s_begin + part * kPart;
+    const uint32_t end = min(s_end, begin + kPart);
+    const uint32_t rows = min(d.n_rows - s * p.R, p.R);
+    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) acc[t] = 0.0f;
+    __syncthreads();
+    for (uint32_t e = begin + threadIdx.x; e < end; e += kReduceThreads) {
+        uint32_t w[EW];
+        if constexpr (EW == 2) {
+            const uint2 q = *reinterpret_cast<const uint2 *>(entries + (size_t)e * EW);
+            w[0] = q.x;
+            w[1] = q.y;
+        } else {
+            const uint4 *src = reinterpret_cast<const uint4 *>(entries + (size_t)e * EW);
+#pragma unroll
+            for (int q = 0; q < EW / 4; ++q) {
+                const uint4 u = src[q];
+                w[4 * q] = u.x;
+                w[4 * q + 1] = u.y;
+                w[4 * q + 2] = u.z;
+                w[4 * q + 3] = u.w;
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            const float v = __uint_as_float(w[1 + f]);
+            if (v != 0.0f) atomicAdd(&acc[w[0] * F + f], v);
+        }
+    }
+    __syncthreads();
+    float *out = n_parts == 1 ? dtable + (size_t)s * p.R * F : partials + (size_t)b * p.R * F;
+    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) out[t] = acc[t];
+}
+
+// 5. slices of several parts: rows = sum of the parts' partial rows, in part order
+template <int F>
+__global__ void __launch_bounds__(kReduceThreads)
+    bwd_combine_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ parts,
+                       const float *__restrict__ partials, float *__restrict__ dtable) {
+    const uint32_t s = blockIdx.x;
+    const uint32_t p0 = parts[s], n_parts = parts[s + 1] - p0;
+    if (n_parts <= 1) return;
+    const uint32_t rows = min(d.n_rows - s * p.R, p.R);
+    float *out = dtable + (size_t)s * p.R * F;
+    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) {
+        float a = 0.0f;
+        for (uint32_t q = 0; q < n_parts; ++q) a += partials[(size_t)(p0 + q) * p.R * F + t];
+        out[t] = a;
+    }
+}
+
+template <int F>
+bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const float *pos, const float *g, float *dtable,
+                     uint8_t *ws, const char *target) {
+    const BwdPlan p = make_plan(d);
+    uint32_t *counts = reinterpret_cast<uint32_t *>(ws + p.counts_off);
+    uint32_t *sums = reinterpret_cast<uint32_t *>(ws + p.sums_off);
+    uint32_t *meta = reinterpret_cast<uint32_t *>(ws + p.meta_off);
+    uint32_t *parts = reinterpret_cast<uint32_t *>(ws + p.parts_off);
+    float *partials = reinterpret_cast<float *>(ws + p.partials_off);
+    uint32_t *entries = reinterpret_cast<uint32_t *>(ws + p.entries_off);
+    const size_t lds = (size_t)p.R * F * sizeof(float);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_reduce_kernel<F>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) {
+        set_error(kLaunchFailed, "%s: cannot reserve %zu bytes of LDS: %s", target, lds, hipGetErrorString(e));
+        return false;
+    }
+    hipLaunchKernelGGL(bwd_count_kernel<F>, dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, counts);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3((uint32_t)p.scan_blocks), dim3(kScanThreads), 0, stream, p.M, counts,
+                       sums);
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kScanThreads), 0, stream, p.scan_blocks, sums, meta);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((uint32_t)p.scan_blocks), dim3(kScanThreads), 0, stream, p.M, counts,
+                       sums);
+    hipLaunchKernelGGL(plan_parts_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, counts, meta, parts);
+    hipLaunchKernelGGL(bwd_scatter_kernel<F>, dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, g, counts,
+                       entries);
+    hipLaunchKernelGGL(bwd_reduce_kernel<F>, dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts, meta,
+                       parts, entries, partials, dtable);
+    hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices), dim3(kReduceThreads), 0, stream, d, p, parts, partials,
+                       dtable);
+    return true;
 }
 
 __global__ void zero_table_kernel(float4 *__restrict__ a, size_t n4, float *__restrict__ tail, size_t n_tail) {
@@ -190,6 +546,10 @@ bool validate(const HashGridDescriptor &d, const char *target) {
             return false;
         }
     }
+    if (d.enc_layout != NGP_ENC_POINT_MAJOR && d.enc_layout != NGP_ENC_LEVEL_MAJOR) {
+        set_error(kBadDescriptor, "%s: enc_layout=%u must be 0 (point-major) or 1 (level-major)", target, d.enc_layout);
+        return false;
+    }
     if (d.table_dtype != NGP_DTYPE_F32 && d.table_dtype != NGP_DTYPE_F16) {
         set_error(kBadDescriptor, "%s: table_dtype=%u must be 0 (f32) or 1 (f16)", target, d.table_dtype);
         return false;
@@ -199,8 +559,18 @@ bool validate(const HashGridDescriptor &d, const char *target) {
 
 template <typename TT>
 void launch_fwd(const HashGridDescriptor &d, hipStream_t s, const float *pos, const void *table, float *enc) {
-    const uint32_t grid = div_up((uint64_t)d.n_points * d.L, kBlock);
     const TT *t = static_cast<const TT *>(table);
+    if (d.enc_layout == NGP_ENC_LEVEL_MAJOR) {
+        const uint32_t g = level_grid(d.n_points, d.L);
+        switch (d.F) {
+            case 1: hipLaunchKernelGGL((hashgrid_fwd_lm_kernel<1, TT>), dim3(g), dim3(kBlock), 0, s, d, pos, t, enc); break;
+            case 2: hipLaunchKernelGGL((hashgrid_fwd_lm_kernel<2, TT>), dim3(g), dim3(kBlock), 0, s, d, pos, t, enc); break;
+            case 4: hipLaunchKernelGGL((hashgrid_fwd_lm_kernel<4, TT>), dim3(g), dim3(kBlock), 0, s, d, pos, t, enc); break;
+            default: hipLaunchKernelGGL((hashgrid_fwd_lm_kernel<8, TT>), dim3(g), dim3(kBlock), 0, s, d, pos, t, enc); break;
+        }
+        return;
+    }
+    const uint32_t grid = div_up((uint64_t)d.n_points * d.L, kBlock);
     switch (d.F) {
         case 1: hipLaunchKernelGGL((hashgrid_fwd_kernel<1, TT>), dim3(grid), dim3(kBlock), 0, s, d, pos, t, enc); break;
         case 2: hipLaunchKernelGGL((hashgrid_fwd_kernel<2, TT>), dim3(grid), dim3(kBlock), 0, s, d, pos, t, enc); break;
@@ -229,30 +599,58 @@ extern "C" void hashgrid_encode(ngp_stream_t stream_, void **buffers, const char
     check_launch(target);
 }
 
+extern "C" size_t hashgrid_encode_backward_workspace_size(const char *opaque, size_t opaque_len) {
+    HashGridDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, "hashgrid_encode_backward_workspace_size")) return 0;
+    if (!validate(d, "hashgrid_encode_backward_workspace_size")) return 0;
+    return (size_t)make_plan(d).bytes;
+}
+
 extern "C" void hashgrid_encode_backward(ngp_stream_t stream_, void **buffers, const char *opaque, size_t opaque_len) {
     const char *target = "hashgrid_encode_backward";
     HashGridDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
     if (!validate(d, target)) return;
-    if (!check_buffers(buffers, 3, target)) return;
+    const BwdPlan plan = make_plan(d);
+    if (plan.n_slices > kMaxSlices) {
+        set_error(kBadDescriptor, "%s: table of %u rows exceeds %u row slices", target, d.n_rows, kMaxSlices);
+        return;
+    }
+    if ((uint64_t)d.n_points * d.L * 8 >= 0xffffffffull) {
+        set_error(kBadDescriptor, "%s: n_points * L * 8 must stay below 2^32", target);
+        return;
+    }
+    if (!check_buffer_list(buffers, {2}, target)) return;
+    if (d.n_points > 0 && !check_buffers(buffers, 4, target)) return;
     hipStream_t stream = (hipStream_t)stream_;
     const float *pos = (const float *)buffers[0];
     const float *g = (const float *)buffers[1];
     float *dtable = (float *)buffers[2];
-    const size_t n = (size_t)d.n_rows * d.F;
-    const bool vec = (reinterpret_cast<uintptr_t>(dtable) & 15u) == 0;
-    const size_t n4 = vec ? n / 4 : 0;
-    const size_t n_tail = n - 4 * n4;
-    const uint32_t zgrid = (uint32_t)std::max<size_t>(1, std::min<size_t>((std::max(n4, n_tail) + 255) / 256, 16384));
-    hipLaunchKernelGGL(zero_table_kernel, dim3(zgrid), dim3(256), 0, stream, reinterpret_cast<float4 *>(dtable), n4,
-                       dtable + 4 * n4, n_tail);
-    if (!check_launch(target) || d.n_points == 0) return;
-    const uint32_t grid = div_up((uint64_t)d.n_points * d.L, kBlock);
-    switch (d.F) {
-        case 1: hipLaunchKernelGGL(hashgrid_bwd_kernel<1>, dim3(grid), dim3(kBlock), 0, stream, d, pos, g, dtable); break;
-        case 2: hipLaunchKernelGGL(hashgrid_bwd_kernel<2>, dim3(grid), dim3(kBlock), 0, stream, d, pos, g, dtable); break;
-        case 4: hipLaunchKernelGGL(hashgrid_bwd_kernel<4>, dim3(grid), dim3(kBlock), 0, stream, d, pos, g, dtable); break;
-        default: hipLaunchKernelGGL(hashgrid_bwd_kernel<8>, dim3(grid), dim3(kBlock), 0, stream, d, pos, g, dtable); break;
+    if (d.n_points == 0) {
+        // no contributions: the gradient is zero
+        const size_t n = (size_t)d.n_rows * d.F;
+        const bool vec = (reinterpret_cast<uintptr_t>(dtable) & 15u) == 0;
+        const size_t n4 = vec ? n / 4 : 0;
+        const size_t n_tail = n - 4 * n4;
+        const uint32_t zgrid =
+            (uint32_t)std::max<size_t>(1, std::min<size_t>((std::max(n4, n_tail) + 255) / 256, 16384));
+        hipLaunchKernelGGL(zero_table_kernel, dim3(zgrid), dim3(256), 0, stream, reinterpret_cast<float4 *>(dtable),
+                           n4, dtable + 4 * n4, n_tail);
+        check_launch(target);
+        return;
     }
+    uint8_t *ws = (uint8_t *)buffers[3];
+    if ((reinterpret_cast<uintptr_t>(ws) & 255u) != 0) {
+        set_error(kBadArgument, "%s: workspace must be 256-byte aligned", target);
+        return;
+    }
+    bool ok = false;
+    switch (d.F) {
+        case 1: ok = launch_backward<1>(d, stream, pos, g, dtable, ws, target); break;
+        case 2: ok = launch_backward<2>(d, stream, pos, g, dtable, ws, target); break;
+        case 4: ok = launch_backward<4>(d, stream, pos, g, dtable, ws, target); break;
+        default: ok = launch_backward<8>(d, stream, pos, g, dtable, ws, target); break;
+    }
+    if (!ok) return;
     check_launch(target);
 }

@@ -9,7 +9,11 @@
 // output deterministic and equal to processing rays one after another:
 //   1. march_count:  one thread per ray marches and counts occupied steps; the
 //                    block total is parked in rays_sample_startidx[first ray of
-//                    the block] (slot owned by the block, so no scratch memory);
+//                    the block] (slot owned by the block, so no scratch memory).
+//                    Rays are counted in three ray-ordered launches (8192 rays,
+//                    up to 65536, the rest); a launch whose predecessors already
+//                    filled the sample buffer skips marching, which changes no
+//                    output (those rays are invalid regardless of their counts);
 //   2. march_scan:   one workgroup scans the block totals, finds the ray where the
 //                    sample buffer fills up and writes the two helper counters;
 //   3. march_write:  one thread per ray takes its offset (block prefix + wave/LDS
@@ -29,11 +33,13 @@ constexpr int kScanThreads = 1024;
 
 // Counts (or, with `write`, emits) the samples of one ray (marching.cu:162-192, 224-267).
 template <bool kWrite>
-__device__ uint32_t march_ray(const MarchParams &p, const uint8_t *__restrict__ bits, float ox, float oy, float oz,
+__device__ uint32_t march_ray(const MarchParams &p, const uint8_t *__restrict__ bits, bool word_ok, float ox, float oy,
+                              float oz,
                               float dx, float dy, float dz, float t0, float t1, float noise, uint32_t max_samples,
                               uint32_t ray, uint32_t *__restrict__ idcs, float *__restrict__ xyzs,
                               float *__restrict__ dirs, float *__restrict__ dss, float *__restrict__ zs) {
     const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+    BitCache cache = make_bit_cache(bits, word_ok);
     float t = t0;
     t = t + step_size(p, t) * noise;
     uint32_t n = 0;
@@ -41,7 +47,7 @@ __device__ uint32_t march_ray(const MarchParams &p, const uint8_t *__restrict__ 
         const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
         const float ds = step_size(p, t);
         float gx, gy, gz, mb;
-        const bool occ = lookup_cell(p, bits, px, py, pz, ds, gx, gy, gz, mb);
+        const bool occ = lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb);
         float nt;
         if (occ) {
             if (kWrite) {
@@ -66,19 +72,34 @@ __device__ uint32_t march_ray(const MarchParams &p, const uint8_t *__restrict__ 
     return n;
 }
 
+// Counts rays of blocks [block_begin, block_end).  Blocks of later chunks first
+// add up the parked totals of all earlier blocks (final: earlier chunks ran to
+// completion before this launch); when they already reach total_samples, every ray
+// here starts past the filled buffer and is invalid whatever its count, so the
+// block records zero samples without marching.
 __global__ void __launch_bounds__(kBlock)
-    march_count_kernel(MarchParams p, uint32_t n_rays, const float *__restrict__ rays_o,
-                       const float *__restrict__ rays_d, const float *__restrict__ t_starts,
-                       const float *__restrict__ t_ends, const float *__restrict__ noises,
-                       const uint8_t *__restrict__ bits, uint32_t *__restrict__ rays_n_samples,
+    march_count_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples, uint32_t block_begin,
+                       const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+                       const float *__restrict__ t_starts, const float *__restrict__ t_ends,
+                       const float *__restrict__ noises, const uint8_t *__restrict__ bits,
+                       uint32_t *__restrict__ rays_n_samples,
                        uint32_t *__restrict__ block_totals /* = rays_sample_startidx */) {
     __shared__ uint32_t wave_totals[kBlock / kWave];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ uint32_t skip;
+    const uint32_t blk = block_begin + blockIdx.x;
+    if (threadIdx.x < kWave) {
+        uint64_t prior = 0;
+        for (uint32_t b = threadIdx.x; b < block_begin; b += kWave) prior += block_totals[(size_t)b * kBlock];
+        prior = wave_sum64(prior);
+        if (threadIdx.x == 0) skip = prior >= total_samples ? 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t i = blk * kBlock + threadIdx.x;
     uint32_t n = 0;
     if (i < n_rays) {
         const float t0 = t_starts[i], t1 = t_ends[i];
-        if (t1 > t0) {
-            n = march_ray<false>(p, bits, rays_o[3 * (size_t)i], rays_o[3 * (size_t)i + 1], rays_o[3 * (size_t)i + 2],
+        if (!skip && t1 > t0) {
+            n = march_ray<false>(p, bits, word_ok, rays_o[3 * (size_t)i], rays_o[3 * (size_t)i + 1], rays_o[3 * (size_t)i + 2],
                                  rays_d[3 * (size_t)i], rays_d[3 * (size_t)i + 1], rays_d[3 * (size_t)i + 2], t0, t1,
                                  noises[i], 0xffffffffu, i, nullptr, nullptr, nullptr, nullptr, nullptr);
         }
@@ -91,7 +112,7 @@ __global__ void __launch_bounds__(kBlock)
         uint32_t s = 0;
 #pragma unroll
         for (int w = 0; w < kBlock / kWave; ++w) s += wave_totals[w];
-        block_totals[(size_t)blockIdx.x * kBlock] = s;
+        block_totals[(size_t)blk * kBlock] = s;
     }
 }
 
@@ -160,7 +181,8 @@ __global__ void __launch_bounds__(kScanThreads)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    march_write_kernel(MarchParams p, uint32_t n_rays, uint32_t total_samples, const float *__restrict__ rays_o,
+    march_write_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples,
+                       const float *__restrict__ rays_o,
                        const float *__restrict__ rays_d, const float *__restrict__ t_starts,
                        const float *__restrict__ t_ends, const float *__restrict__ noises,
                        const uint8_t *__restrict__ bits, const uint32_t *__restrict__ next_loc,
@@ -184,7 +206,7 @@ __global__ void __launch_bounds__(kBlock)
         rays_sample_startidx[i] = (valid && n > 0) ? (uint32_t)start : 0u;
         if (valid && n > 0) {
             const uint32_t s = (uint32_t)start;
-            march_ray<true>(p, bits, rays_o[3 * (size_t)i], rays_o[3 * (size_t)i + 1], rays_o[3 * (size_t)i + 2],
+            march_ray<true>(p, bits, word_ok, rays_o[3 * (size_t)i], rays_o[3 * (size_t)i + 1], rays_o[3 * (size_t)i + 2],
                             rays_d[3 * (size_t)i], rays_d[3 * (size_t)i + 1], rays_d[3 * (size_t)i + 2], t0, t1,
                             noises[i], n, i, idcs + s, xyzs + 3 * (size_t)s, dirs + 3 * (size_t)s, dss + s, zs + s);
         }
@@ -247,7 +269,7 @@ __global__ void __launch_bounds__(kScanThreads)
 }
 
 __global__ void __launch_bounds__(kBlock)
-    march_inference_kernel(MarchParams p, uint32_t n_total_rays, uint32_t n_rays, uint32_t cap,
+    march_inference_kernel(MarchParams p, bool word_ok, uint32_t n_total_rays, uint32_t n_rays, uint32_t cap,
                            const float *__restrict__ rays_o, const float *__restrict__ rays_d,
                            const float *__restrict__ t_starts, const float *__restrict__ t_ends,
                            const uint8_t *__restrict__ bits, const uint32_t *__restrict__ indices,
@@ -267,13 +289,14 @@ __global__ void __launch_bounds__(kBlock)
             const float ox = rays_o[3 * (size_t)r], oy = rays_o[3 * (size_t)r + 1], oz = rays_o[3 * (size_t)r + 2];
             const float dx = rays_d[3 * (size_t)r], dy = rays_d[3 * (size_t)r + 1], dz = rays_d[3 * (size_t)r + 2];
             const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
+            BitCache cache = make_bit_cache(bits, word_ok);
             t = t0;
             while (steps < cap && t < t1) {
                 const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
                 const float ds = step_size(p, t);
                 float gx, gy, gz, mb;
                 float nt;
-                if (lookup_cell(p, bits, px, py, pz, ds, gx, gy, gz, mb)) {
+                if (lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb)) {
                     rx[3 * steps + 0] = px;
                     rx[3 * steps + 1] = py;
                     rx[3 * steps + 2] = pz;
@@ -291,7 +314,7 @@ __global__ void __launch_bounds__(kBlock)
                 const float px = ox + t1 * dx, py = oy + t1 * dy, pz = oz + t1 * dz;
                 const float ds = step_size(p, t1);
                 float gx, gy, gz, mb;
-                if (lookup_cell(p, bits, px, py, pz, ds, gx, gy, gz, mb)) {
+                if (lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb)) {
                     if (steps > 0 && rds[steps - 1] + rz[steps - 1] >= t1) rds[steps - 1] = t1 - rz[steps - 1];
                     if (steps < cap) {
                         rx[3 * steps + 0] = px;
@@ -317,6 +340,12 @@ __global__ void __launch_bounds__(kBlock)
     }
     n_samples[i] = steps;
     t_out[i] = t;
+}
+
+// 64-bit words need an 8-byte aligned bitfield of a whole number of words.
+bool bitfield_words_ok(const uint8_t *bits, uint32_t K, uint32_t G) {
+    const uint64_t n_bytes = (uint64_t)K * G * G * G / 8;
+    return (reinterpret_cast<uintptr_t>(bits) & 7u) == 0 && n_bytes % 8 == 0 && n_bytes > 0;
 }
 
 bool valid_march_statics(uint32_t diag, uint32_t K, uint32_t G, float bound, float portion, const char *target) {
@@ -368,6 +397,7 @@ extern "C" void march_rays(ngp_stream_t stream_, void **buffers, const char *opa
     float *dss = (float *)buffers[14];
     float *zs = (float *)buffers[15];
     const MarchParams p = make_march_params(d.diagonal_n_steps, d.K, d.G, d.bound, d.stepsize_portion);
+    const bool word_ok = bitfield_words_ok(bits, d.K, d.G);
     if (d.n_rays == 0) {
         hipLaunchKernelGGL(march_empty_kernel, dim3(min(div_up(d.total_samples, 256), 4096u)), dim3(256), 0, stream,
                            d.total_samples, next_loc, exceeded, idcs, xyzs, dirs, dss, zs);
@@ -375,13 +405,21 @@ extern "C" void march_rays(ngp_stream_t stream_, void **buffers, const char *opa
         return;
     }
     const uint32_t n_blocks = div_up(d.n_rays, kBlock);
-    hipLaunchKernelGGL(march_count_kernel, dim3(n_blocks), dim3(kBlock), 0, stream, p, d.n_rays, rays_o, rays_d,
-                       t_starts, t_ends, noises, bits, n_samples, startidx);
-    if (!check_launch(target)) return;
+    // count in ray-ordered chunks of 32, 256 and all remaining blocks
+    const uint32_t bounds[4] = {0, std::min(n_blocks, 32u), std::min(n_blocks, 256u), n_blocks};
+    for (int c = 0; c < 3; ++c) {
+        if (bounds[c + 1] <= bounds[c]) continue;
+        hipLaunchKernelGGL(march_count_kernel, dim3(bounds[c + 1] - bounds[c]), dim3(kBlock), 0, stream, p, word_ok,
+                           d.n_rays,
+                           d.total_samples, bounds[c], rays_o, rays_d, t_starts, t_ends, noises, bits, n_samples,
+                           startidx);
+        if (!check_launch(target)) return;
+    }
     hipLaunchKernelGGL(march_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, d.n_rays, n_blocks, d.total_samples,
                        n_samples, startidx, next_loc, exceeded);
     if (!check_launch(target)) return;
-    hipLaunchKernelGGL(march_write_kernel, dim3(n_blocks), dim3(kBlock), 0, stream, p, d.n_rays, d.total_samples,
+    hipLaunchKernelGGL(march_write_kernel, dim3(n_blocks), dim3(kBlock), 0, stream, p, word_ok, d.n_rays,
+                       d.total_samples,
                        rays_o, rays_d, t_starts, t_ends, noises, bits, next_loc, exceeded, valid, n_samples, startidx,
                        idcs, xyzs, dirs, dss, zs);
     check_launch(target);
@@ -419,7 +457,7 @@ extern "C" void march_rays_inference(ngp_stream_t stream_, void **buffers, const
                        indices_in, next_out, indices_out);
     if (!check_launch(target)) return;
     hipLaunchKernelGGL(march_inference_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, p,
-                       d.n_total_rays, d.n_rays, d.march_steps_cap, rays_o, rays_d, t_starts, t_ends, bits,
+                       bitfield_words_ok(bits, d.K, d.G), d.n_total_rays, d.n_rays, d.march_steps_cap, rays_o, rays_d, t_starts, t_ends, bits,
                        indices_out, n_samples, t_out, xyzs, dss, zs);
     check_launch(target);
 }

@@ -145,16 +145,37 @@ struct Fwd {
     v4f x[2], h1[4], c[2], h2[4], h3[4], rr;  // c[0] = o (density MLP output), c[1] = SH
 };
 
+// Features 16b + 4g .. 16b + 4g + 3 of sample s are levels 8b + 2g and 8b + 2g + 1
+// (two features each).  Point-major: one 16-byte vector at enc[s*32 + 16b + 4g];
+// level-major [16, n, 2]: two 8-byte pairs at enc[(level*n + s)*2].
+__device__ __forceinline__ v4f load_enc4(const float *__restrict__ enc, bool level_major, uint32_t n, size_t s,
+                                         int b, int g) {
+    if (!level_major) return *reinterpret_cast<const v4f *>(enc + s * 32 + 16 * b + 4 * g);
+    const size_t l0 = 8 * b + 2 * g;
+    const float2 a = *reinterpret_cast<const float2 *>(enc + (l0 * n + s) * 2);
+    const float2 c = *reinterpret_cast<const float2 *>(enc + ((l0 + 1) * n + s) * 2);
+    return v4f{a.x, a.y, c.x, c.y};
+}
+
+__device__ __forceinline__ void store_enc4(float *__restrict__ enc, bool level_major, uint32_t n, size_t s, int b,
+                                           int g, v4f v) {
+    if (!level_major) {
+        *reinterpret_cast<v4f *>(enc + s * 32 + 16 * b + 4 * g) = v;
+        return;
+    }
+    const size_t l0 = 8 * b + 2 * g;
+    *reinterpret_cast<float2 *>(enc + (l0 * n + s) * 2) = make_float2(v[0], v[1]);
+    *reinterpret_cast<float2 *>(enc + ((l0 + 1) * n + s) * 2) = make_float2(v[2], v[3]);
+}
+
 // Forward of one 16-sample tile starting at sample `s0` (samples >= n are zero).
-__device__ __forceinline__ void forward_tile(const float *lds, const float *__restrict__ enc,
+__device__ __forceinline__ void forward_tile(const float *lds, const float *__restrict__ enc, bool level_major,
                                              const float *__restrict__ dirs, uint32_t n, size_t s0, int i, int g,
                                              Fwd &f) {
     const size_t s = s0 + i;
     const bool live = s < n;
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        f.x[b] = live ? *reinterpret_cast<const v4f *>(enc + s * 32 + 16 * b + 4 * g) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    }
+    for (int b = 0; b < 2; ++b) f.x[b] = live ? load_enc4(enc, level_major, n, s, b, g) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
     layer_fwd<32, 64, S_W1>(lds + L_W1, f.x, f.h1, i, g);
     relu<4>(f.h1);
     layer_fwd<64, 16, S_W2>(lds + L_W2, f.h1, &f.c[0], i, g);
@@ -173,7 +194,7 @@ __device__ __forceinline__ void forward_tile(const float *lds, const float *__re
 }
 
 __global__ void __launch_bounds__(kThreads)
-    mlp_forward_kernel(uint32_t n, const float *__restrict__ enc, const float *__restrict__ dirs,
+    mlp_forward_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
                        const float *__restrict__ params, float *__restrict__ drgbs) {
     __shared__ float lds[L_END];
     stage_weights(lds, params);
@@ -185,7 +206,7 @@ __global__ void __launch_bounds__(kThreads)
     for (uint32_t t = wave; t < n_tiles; t += n_waves) {
         const size_t s0 = (size_t)t * kTile;
         Fwd f;
-        forward_tile(lds, enc, dirs, n, s0, i, g, f);
+        forward_tile(lds, enc, level_major, dirs, n, s0, i, g, f);
         const size_t s = s0 + i;
         if (g == 0 && s < n) {
             const float sigma = det_expf(f.c[0][0]);
@@ -238,7 +259,7 @@ __device__ __forceinline__ void dw_to_lds(float *dst, const v4f *acc, int out_di
 }
 
 __global__ void __launch_bounds__(kThreads)
-    mlp_backward_kernel(uint32_t n, const float *__restrict__ enc, const float *__restrict__ dirs,
+    mlp_backward_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
                         const float *__restrict__ params, const float *__restrict__ dL_ddrgbs,
                         float *__restrict__ dL_denc, float *__restrict__ workspace) {
     __shared__ float lds[L_END];
@@ -266,7 +287,7 @@ __global__ void __launch_bounds__(kThreads)
         const size_t s = s0 + i;
         const bool live = s < n;
         Fwd f;
-        forward_tile(lds, enc, dirs, n, s0, i, g, f);
+        forward_tile(lds, enc, level_major, dirs, n, s0, i, g, f);
         const v4f gin = live ? *reinterpret_cast<const v4f *>(dL_ddrgbs + 4 * s) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
         // colour head: d(pre-sigmoid), features 0..2 live in lanes g == 0
         v4f dr = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -324,7 +345,7 @@ __global__ void __launch_bounds__(kThreads)
         layer_bwd<32, 64, S_W1>(lds + L_W1, d1, dx, i, g);
         if (live) {
 #pragma unroll
-            for (int b = 0; b < 2; ++b) *reinterpret_cast<v4f *>(dL_denc + s * 32 + 16 * b + 4 * g) = dx[b];
+            for (int b = 0; b < 2; ++b) store_enc4(dL_denc, level_major, n, s, b, g, dx[b]);
         }
         wave_lds_sync();
     }
@@ -375,6 +396,10 @@ extern "C" void nerf_mlp_forward(ngp_stream_t stream_, void **buffers, const cha
     const char *target = "nerf_mlp_forward";
     NerfMlpDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.enc_layout > NGP_ENC_LEVEL_MAJOR) {
+        set_error(kBadDescriptor, "%s: enc_layout=%u must be 0 or 1", target, d.enc_layout);
+        return;
+    }
     if (d.n_samples == 0) return;
     if (!check_buffers(buffers, 4, target)) return;
     if (!aligned16(buffers[0]) || !aligned16(buffers[3])) {
@@ -382,7 +407,7 @@ extern "C" void nerf_mlp_forward(ngp_stream_t stream_, void **buffers, const cha
         return;
     }
     hipLaunchKernelGGL(mlp_forward_kernel, dim3(forward_grid(d.n_samples)), dim3(kThreads), 0, (hipStream_t)stream_,
-                       d.n_samples, (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
+                       d.n_samples, d.enc_layout == NGP_ENC_LEVEL_MAJOR, (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
                        (float *)buffers[3]);
     check_launch(target);
 }
@@ -391,8 +416,8 @@ extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const ch
     const char *target = "nerf_mlp_backward";
     NerfMlpDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
-    if (d.n_workspace_blocks == 0) {
-        set_error(kBadDescriptor, "%s: n_workspace_blocks must be positive", target);
+    if (d.n_workspace_blocks == 0 || d.enc_layout > NGP_ENC_LEVEL_MAJOR) {
+        set_error(kBadDescriptor, "%s: n_workspace_blocks must be positive and enc_layout 0 or 1", target);
         return;
     }
     if (!check_buffer_list(buffers, {2, 5, 6}, target)) return;
@@ -403,7 +428,7 @@ extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const ch
     }
     hipStream_t stream = (hipStream_t)stream_;
     hipLaunchKernelGGL(mlp_backward_kernel, dim3(d.n_workspace_blocks), dim3(kThreads), 0, stream, d.n_samples,
-                       (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
+                       d.enc_layout == NGP_ENC_LEVEL_MAJOR, (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
                        (const float *)buffers[3], (float *)buffers[4], (float *)buffers[6]);
     if (!check_launch(target)) return;
     hipLaunchKernelGGL(sum_rows_kernel, dim3(div_up(P_END, 256)), dim3(256), 0, stream, d.n_workspace_blocks,

@@ -71,10 +71,39 @@ __device__ __forceinline__ uint32_t clamp_cascade(int e, uint32_t K) {
     return (uint32_t)(e < 0 ? 0 : (e > hi ? hi : e));
 }
 
+// The last 64-bit word of the occupancy bitfield a ray read.  A word holds 64
+// Morton-consecutive cells (a 4x4x4 block); consecutive steps of a ray mostly stay
+// inside one, so most lookups need no memory access.  `words` is NULL when the
+// bitfield is too small or not 8-byte aligned; then single bytes are read.
+struct BitCache {
+    const uint64_t *__restrict__ words;
+    uint64_t word;
+    uint32_t key;
+};
+
+__device__ __forceinline__ BitCache make_bit_cache(const uint8_t *bits, bool word_ok) {
+    BitCache c;
+    c.words = word_ok ? reinterpret_cast<const uint64_t *>(bits) : nullptr;
+    c.word = 0;
+    c.key = 0xffffffffu;
+    return c;
+}
+
+__device__ __forceinline__ bool occupied(BitCache &c, const uint8_t *__restrict__ bits, uint32_t idx) {
+    if (c.words == nullptr) return (bits[idx >> 3] >> (idx & 7u)) & 1u;
+    const uint32_t key = idx >> 6;
+    if (key != c.key) {
+        c.word = c.words[key];
+        c.key = key;
+    }
+    return (c.word >> (idx & 63u)) & 1u;
+}
+
 // Occupancy lookup of one position (marching.cu:79-98 and 170-176).  Fills the
 // float grid coordinate and the mip bound used by the empty-cell skip.
-__device__ __forceinline__ bool lookup_cell(const MarchParams &p, const uint8_t *__restrict__ bits, float px, float py,
-                                            float pz, float ds, float &gx, float &gy, float &gz, float &mip_bound) {
+__device__ __forceinline__ bool lookup_cell(const MarchParams &p, const uint8_t *__restrict__ bits, BitCache &cache,
+                                            float px, float py, float pz, float ds, float &gx, float &gy, float &gz,
+                                            float &mip_bound) {
     uint32_t cascade = 0;
     if (p.K > 1) {
         float m = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
@@ -93,7 +122,7 @@ __device__ __forceinline__ bool lookup_cell(const MarchParams &p, const uint8_t 
     uint32_t iy = (uint32_t)fminf(fmaxf(floorf(gy), 0.0f), gmax);
     uint32_t iz = (uint32_t)fminf(fmaxf(floorf(gz), 0.0f), gmax);
     uint32_t idx = cascade * p.G3 + morton3(ix, iy, iz);
-    return (bits[idx >> 3] >> (idx & 7u)) & 1u;
+    return occupied(cache, bits, idx);
 }
 
 // Exit distance of the current cell along the ray (marching.cu:182-185).

@@ -80,7 +80,9 @@ class _HashGridFn(torch.autograd.Function):
     def backward(ctx, g):
         (pos,) = ctx.saved_tensors
         dtable = torch.empty(ctx.table_shape, dtype=torch.float32, device=pos.device)
-        _lib.call("hashgrid_encode_backward", [pos, g.float().contiguous(), dtable], ctx.desc_fn(pos.shape[0]))
+        desc = ctx.desc_fn(pos.shape[0])
+        ws = torch.empty(_lib.hashgrid_backward_workspace_bytes(desc), dtype=torch.uint8, device=pos.device)
+        _lib.call("hashgrid_encode_backward", [pos, g.float().contiguous(), dtable, ws], desc)
         return None, dtable.to(ctx.table_dtype), None, None
 
 
@@ -117,11 +119,14 @@ class HashGridEncoder(torch.nn.Module):
     def table(self) -> torch.nn.Parameter:
         return getattr(self, PARAM_NAME)
 
-    def descriptor(self, n_points: int, bound: float) -> bytes:
+    def descriptor(self, n_points: int, bound: float, enc_layout: int = _lib.ENC_POINT_MAJOR) -> bytes:
+        """HashGridDescriptor bytes; enc_layout ENC_LEVEL_MAJOR selects the [L, N, F]
+        encoding layout the fused training step uses."""
         lv = self.levels
         dt = _lib.DTYPE_F16 if self.table.dtype == torch.float16 else _lib.DTYPE_F32
         return _lib.hashgrid_descriptor(n_points, self.L, self.F, self.T, lv.first_hash_level, dt, lv.n_rows,
-                                        float(bound), [float(s) for s in lv.scales], lv.resolutions, lv.offsets)
+                                        float(bound), [float(s) for s in lv.scales], lv.resolutions, lv.offsets,
+                                        enc_layout)
 
     def forward(self, pos: torch.Tensor, bound: float):
         if pos.dim() != 2 or pos.shape[1] != 3:

@@ -9,7 +9,8 @@ after the learning-rate scaling as optax.chain does there, occupancy-grid update
 every min(16, step//16 + 1) steps (all cells before step 256).
 
 A step is the sequence of C-ABI targets below on preallocated device buffers, all
-enqueued on the current stream with no host synchronisation:
+enqueued on the current stream with no host synchronisation (encodings and their
+gradients use the level-major [16, S, 2] layout between the encoder and the MLP):
   train_make_rays -> march_rays -> hashgrid_encode -> nerf_mlp_forward ->
   integrate_rays -> train_loss_grad -> integrate_rays_backward ->
   nerf_mlp_backward -> hashgrid_encode_backward -> adam_update (table, MLP)
@@ -103,10 +104,14 @@ class Trainer:
             o=z(n, 3), d=z(n, 3), t0=z(n), t1=z(n), noise=z(n), bgs=z(n, 3), gt=z(n, 3), pix=z(n, dtype=u32),
             next_loc=z(1, dtype=u32), exceeded=z(1, dtype=u32), valid=z(n, dtype=torch.bool), ns=z(n, dtype=u32),
             start=z(n, dtype=u32), idcs=z(S, dtype=u32), xyzs=z(S, 3), dirs=z(S, 3), dss=z(S), zs=z(S),
-            enc=z(S, 32), drgbs=z(S, 4), measured=z(1, dtype=u32), rgbd=z(n, 4), op=z(n), n_valid=z(1, dtype=u32),
-            loss=z(1), g_rgbd=z(n, 4), dbgs=z(n, 3), dz=z(S), ddrgbs=z(S, 4), denc=z(S, 32), dparams=z(N_PARAMS),
+            enc=z(16, S, 2), drgbs=z(S, 4), measured=z(1, dtype=u32), rgbd=z(n, 4), op=z(n), n_valid=z(1, dtype=u32),
+            loss=z(1), g_rgbd=z(n, 4), dbgs=z(n, 3), dz=z(S), ddrgbs=z(S, 4), denc=z(16, S, 2), dparams=z(N_PARAMS),
             ws=z(mlp_workspace_blocks(S), N_PARAMS), dtable=z(*self.model.position_encoder.table.shape),
         )
+        enc_mod = self.model.position_encoder
+        self.b["hg_ws"] = torch.empty(
+            _lib.hashgrid_backward_workspace_bytes(enc_mod.descriptor(S, self.bound, _lib.ENC_LEVEL_MAJOR)),
+            dtype=torch.uint8, device=self.dev)
         table = self.model.position_encoder.table
         self.table_master = table.detach().float().clone()
         self.opt = dict(tm=torch.zeros_like(self.table_master), tv=torch.zeros_like(self.table_master),
@@ -228,10 +233,11 @@ class Trainer:
                                  b["exceeded"], b["valid"], b["ns"], b["start"], b["idcs"], b["xyzs"], b["dirs"],
                                  b["dss"], b["zs"]],
                   _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion))
-        hdesc = enc_mod.descriptor(S, self.bound)
+        LM = _lib.ENC_LEVEL_MAJOR
+        hdesc = enc_mod.descriptor(S, self.bound, LM)
         _lib.call("hashgrid_encode", [b["xyzs"], table.detach(), b["enc"]], hdesc)
         params = m.mlp_params.detach()
-        _lib.call("nerf_mlp_forward", [b["enc"], b["dirs"], params, b["drgbs"]], _lib.nerf_mlp_descriptor(S))
+        _lib.call("nerf_mlp_forward", [b["enc"], b["dirs"], params, b["drgbs"]], _lib.nerf_mlp_descriptor(S, 0, LM))
         _lib.call("integrate_rays", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"], b["measured"],
                                      b["rgbd"], b["op"]], _lib.integrating_descriptor(n, S))
         _lib.call("train_loss_grad", [b["rgbd"], b["gt"], b["valid"], b["n_valid"], b["g_rgbd"], b["loss"]],
@@ -240,8 +246,8 @@ class Trainer:
                                               b["rgbd"], b["op"], b["g_rgbd"], b["dbgs"], b["dz"], b["ddrgbs"]],
                   _lib.integrating_backward_descriptor(n, S, c.near))
         _lib.call("nerf_mlp_backward", [b["enc"], b["dirs"], params, b["ddrgbs"], b["denc"], b["dparams"], b["ws"]],
-                  _lib.nerf_mlp_descriptor(S, b["ws"].shape[0]))
-        _lib.call("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"]], hdesc)
+                  _lib.nerf_mlp_descriptor(S, b["ws"].shape[0], LM))
+        _lib.call("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"], b["hg_ws"]], hdesc)
         self.step_count += 1
         t = self.step_count
         lr = lr_at(t - 1, c.lr)

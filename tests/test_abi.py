@@ -17,13 +17,14 @@ HEADER = REPO / "include" / "ngp_hip.h"
 
 def _declared_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:void|int|const char \*)\s*\*?\s*([a-z_0-9]+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:void|int|size_t|const char \*)\s*\*?\s*([a-z_0-9]+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
     lib = _lib.load()
     declared = _declared_functions()
     assert "march_rays" in declared and "ngp_get_last_error" in declared
+    assert "hashgrid_encode_backward_workspace_size" in declared
     for name in declared:
         assert hasattr(lib, name), name
     nm = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
@@ -45,8 +46,8 @@ def test_descriptor_sizes_match_c_structs():
     assert len(_lib.marching_inference_descriptor(1, 2, 3, 4, 5, 6, 1.0, 0.0)) == 32
     assert len(_lib.morton3d_descriptor(5)) == 4
     assert len(_lib.packbits_descriptor(5)) == 4
-    assert len(_lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])) == 8 * 4 + 32 * 4 * 2 + 33 * 4
-    assert len(_lib.nerf_mlp_descriptor(5, 0)) == 8
+    assert len(_lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])) == 9 * 4 + 32 * 4 * 2 + 33 * 4
+    assert len(_lib.nerf_mlp_descriptor(5, 0)) == 12
     assert len(_lib.train_rays_descriptor(1, 2, 3, 4, 1.0, 1.0, 1.0, 1.0, 1.0, 5, 6, 3, (1.0, 1.0, 1.0))) == 64
     assert len(_lib.loss_descriptor(4, 0.1)) == 8
     assert len(_lib.adam_descriptor(10, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0)) == 48
@@ -66,8 +67,8 @@ _Static_assert(sizeof(IntegratingBackwardDescriptor) == 12, "");
 _Static_assert(sizeof(IntegratingInferenceDescriptor) == 12, "");
 _Static_assert(sizeof(PackbitsDescriptor) == 4, "");
 _Static_assert(sizeof(Morton3DDescriptor) == 4, "");
-_Static_assert(sizeof(HashGridDescriptor) == 8 * 4 + 32 * 8 + 33 * 4, "");
-_Static_assert(sizeof(NerfMlpDescriptor) == 8, "");
+_Static_assert(sizeof(HashGridDescriptor) == 9 * 4 + 32 * 8 + 33 * 4, "");
+_Static_assert(sizeof(NerfMlpDescriptor) == 12, "");
 _Static_assert(sizeof(NgpTrainRaysDescriptor) == 64, "");
 _Static_assert(sizeof(NgpLossDescriptor) == 8, "");
 _Static_assert(sizeof(NgpAdamDescriptor) == 48, "");
@@ -106,4 +107,19 @@ def test_bad_statics_are_errors():
     with pytest.raises(_lib.NGPError, match="level table"):
         _lib.call_raw("hashgrid_encode", [0] * 3, _lib.hashgrid_descriptor(4, 1, 2, 8, 1, 0, 9, 1.0, [1.0], [2], [0, 8]))
     # the error is cleared once read
+    assert _lib.last_error() == (0, "")
+
+
+def test_hashgrid_backward_workspace_size():
+    """Host-only query: the workspace holds 16-byte entries for up to 8 rows per
+    point and level (F=2), and an invalid descriptor is an error, not 0 bytes."""
+    from jaxngp_amd.encoders import level_table
+    lv = level_table(16, 2 ** 19, 16, 2048)
+    desc = lambda n: _lib.hashgrid_descriptor(n, 16, 2, 2 ** 19, lv.first_hash_level, 0, lv.n_rows, 1.0,
+                                              list(lv.scales), lv.resolutions, lv.offsets)
+    small, big = _lib.hashgrid_backward_workspace_bytes(desc(1000)), _lib.hashgrid_backward_workspace_bytes(desc(2 ** 18))
+    assert big - small >= (2 ** 18 - 1000) * 16 * 8 * 16
+    assert big < 2 ** 30
+    with pytest.raises(_lib.NGPError, match="F="):
+        _lib.hashgrid_backward_workspace_bytes(_lib.hashgrid_descriptor(4, 1, 3, 8, 1, 0, 8, 1.0, [1.0], [2], [0, 8]))
     assert _lib.last_error() == (0, "")

@@ -118,3 +118,36 @@ def test_hashgrid_fp16_table(dev):
     out, _ = enc(torch.from_numpy(pos).to(dev), 1.0)
     table32 = enc.table.detach().float().cpu().numpy()
     np.testing.assert_array_equal(out.detach().cpu().numpy(), _c_forward(enc, pos, table32))
+
+
+def test_level_major_layout_matches_point_major(dev):
+    """The level-major [L, N, F] path (XCD-affine kernels) gives the point-major
+    encodings transposed, and the same table gradient from the transposed cotangent."""
+    from jaxngp_amd import _lib
+    rng = np.random.default_rng(9)
+    enc = HashGridEncoder(**CONFIGS["c0"], generator=torch.Generator().manual_seed(1)).to(dev)
+    with torch.no_grad():
+        enc.table.uniform_(-1, 1)
+    N = 30001
+    pos = torch.from_numpy(rng.uniform(-1, 1, (N, 3)).astype(np.float32)).to(dev)
+    L, F = 16, 2
+    pm = torch.empty(N, L * F, device=dev)
+    lm = torch.empty(L, N, F, device=dev)
+    t = enc.table.detach()
+    _lib.call("hashgrid_encode", [pos, t, pm], enc.descriptor(N, 1.0))
+    _lib.call("hashgrid_encode", [pos, t, lm], enc.descriptor(N, 1.0, _lib.ENC_LEVEL_MAJOR))
+    torch.testing.assert_close(lm.permute(1, 0, 2).reshape(N, L * F), pm, rtol=0, atol=0)
+    g = torch.from_numpy(rng.normal(size=(N, L * F)).astype(np.float32)).to(dev)
+    g_lm = g.reshape(N, L, F).permute(1, 0, 2).contiguous()
+    d1 = torch.empty_like(t)
+    d2 = torch.empty_like(t)
+    for d, gg, layout in ((d1, g, _lib.ENC_POINT_MAJOR), (d2, g_lm, _lib.ENC_LEVEL_MAJOR)):
+        desc = enc.descriptor(N, 1.0, layout)
+        ws = torch.empty(_lib.hashgrid_backward_workspace_bytes(desc), dtype=torch.uint8, device=dev)
+        _lib.call("hashgrid_encode_backward", [pos, gg, d, ws], desc)
+    want = _c_backward(enc, pos.cpu().numpy(), g.cpu().numpy())
+    absw = _c_backward(enc, pos.cpu().numpy(), np.abs(g.cpu().numpy()))
+    for d in (d1, d2):
+        dt = d.cpu().numpy().astype(np.float64)
+        assert ((dt != 0) == (want != 0)).all()
+        assert (np.abs(dt - want) <= 1e-5 * absw + 1e-30).all()

@@ -67,3 +67,29 @@ def test_mlp_zero_cotangent_and_padding(dev):
     _, dP = oracle.mlp_backward64(enc[:60], d[:60], params, g[:60])
     gP = par_t.grad.cpu().numpy()
     assert (np.abs(gP - dP) <= 2e-4 * np.abs(dP).max() + 1e-5 * np.abs(dP)).all()
+
+
+def test_mlp_level_major_encodings(dev):
+    from jaxngp_amd import _lib
+    from jaxngp_amd.nerfs import mlp_workspace_blocks
+    S = 5000
+    enc, d, params, g = _inputs(S, 11)
+    enc_t = torch.from_numpy(enc).to(dev)
+    lm = enc_t.reshape(S, 16, 2).permute(1, 0, 2).contiguous()
+    d_t = torch.from_numpy(d).to(dev)
+    p_t = torch.from_numpy(params).to(dev)
+    g_t = torch.from_numpy(g).to(dev)
+    outs = []
+    for layout, e in ((_lib.ENC_POINT_MAJOR, enc_t), (_lib.ENC_LEVEL_MAJOR, lm)):
+        out = torch.empty(S, 4, device=dev)
+        _lib.call("nerf_mlp_forward", [e, d_t, p_t, out], _lib.nerf_mlp_descriptor(S, 0, layout))
+        denc = torch.empty_like(e)
+        dp = torch.empty(N_PARAMS, device=dev)
+        blocks = mlp_workspace_blocks(S)
+        ws = torch.empty(blocks, N_PARAMS, device=dev)
+        _lib.call("nerf_mlp_backward", [e, d_t, p_t, g_t, denc, dp, ws], _lib.nerf_mlp_descriptor(S, blocks, layout))
+        if layout == _lib.ENC_LEVEL_MAJOR:
+            denc = denc.permute(1, 0, 2).reshape(S, 32)
+        outs.append((out, denc, dp))
+    for a, b in zip(outs[0], outs[1]):
+        torch.testing.assert_close(a, b, rtol=0, atol=0)
