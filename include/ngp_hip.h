@@ -201,7 +201,7 @@ void hashgrid_encode(ngp_stream_t stream, void **buffers, const char *opaque, si
 /* Table gradient of HashGridEncoder (the VJP of the gather at encoders.py:226 and
  * of the weighted sum at encoders.py:231).  opaque: HashGridDescriptor.
  * in : pos f32[N,3], dL_dencodings f32 in the descriptor's enc_layout
- * out: dL_dtable f32[n_rows, F]   (overwritten, not accumulated into),
+ * out: dL_dtable f32[n_rows, F]   (overwritten, not accumulated into; 16-byte aligned),
  *      workspace u8[hashgrid_encode_backward_workspace_size(opaque)], 256-byte
  *      aligned (unused, may be NULL, when N == 0)
  * The gradient is scattered without global atomics: contributions are sorted

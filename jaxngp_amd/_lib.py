@@ -118,6 +118,42 @@ def call(name: str, tensors: Sequence[torch.Tensor], opaque: bytes, stream: int 
     _check()
 
 
+class Prepared:
+    """A target call with its buffers bound once: the pointer array and the stream
+    are built at construction, so a call costs one ctypes call.  The tensors are
+    kept referenced; `set_buffer` swaps one.  Errors are left for `check()`."""
+
+    __slots__ = ("name", "fn", "ptrs", "opaque", "stream", "keep")
+
+    def __init__(self, name: str, tensors: Sequence[torch.Tensor], opaque: bytes, stream: int | None = None):
+        for i, t in enumerate(tensors):
+            if not t.is_cuda or not t.is_contiguous():
+                raise ValueError(f"{name}: buffer {i} is not a contiguous device tensor")
+        self.name = name
+        self.fn = target(name)
+        self.keep = list(tensors)
+        self.ptrs = (ctypes.c_void_p * max(1, len(tensors)))(*[t.data_ptr() for t in tensors])
+        self.opaque = opaque
+        if stream is None:
+            stream = current_stream(tensors[0].device if tensors else None)
+        self.stream = ctypes.c_void_p(stream)
+
+    def set_buffer(self, i: int, t: torch.Tensor) -> None:
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{self.name}: buffer {i} is not a contiguous device tensor")
+        self.keep[i] = t
+        self.ptrs[i] = t.data_ptr()
+
+    def __call__(self, opaque: bytes | None = None) -> None:
+        o = self.opaque if opaque is None else opaque
+        self.fn(self.stream, self.ptrs, o, len(o))
+
+
+def check() -> None:
+    """Raises NGPError if a target recorded an error on this thread."""
+    _check()
+
+
 def hashgrid_backward_workspace_bytes(opaque: bytes) -> int:
     """Workspace bytes of hashgrid_encode_backward for descriptor `opaque`."""
     n = load().hashgrid_encode_backward_workspace_size(opaque, len(opaque))

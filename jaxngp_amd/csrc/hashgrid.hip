@@ -412,6 +412,24 @@ __global__ void __launch_bounds__(kItemThreads)
     });
 }
 
+template <int EW>
+__device__ __forceinline__ void load_entry(const uint32_t *__restrict__ src, uint32_t *w) {
+    if constexpr (EW == 2) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(src);
+        w[0] = q.x;
+        w[1] = q.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < EW / 4; ++q) {
+            const uint4 u = reinterpret_cast<const uint4 *>(src)[q];
+            w[4 * q] = u.x;
+            w[4 * q + 1] = u.y;
+            w[4 * q + 2] = u.z;
+            w[4 * q + 3] = u.w;
+        }
+    }
+}
+
 // 4. workgroup b sums part b's entries in LDS.  A slice of one part writes its
 // rows of the gradient; a slice of several writes partial rows, summed in order
 // by combine_kernel.
@@ -440,27 +458,28 @@ __global__ void __launch_bounds__(kReduceThreads)
     const uint32_t rows = min(d.n_rows - s * p.R, p.R);
     for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) acc[t] = 0.0f;
     __syncthreads();
-    for (uint32_t e = begin + threadIdx.x; e < end; e += kReduceThreads) {
-        uint32_t w[EW];
-        if constexpr (EW == 2) {
-            const uint2 q = *reinterpret_cast<const uint2 *>(entries + (size_t)e * EW);
-            w[0] = q.x;
-            w[1] = q.y;
-        } else {
-            const uint4 *src = reinterpret_cast<const uint4 *>(entries + (size_t)e * EW);
+    // four entries in flight per thread
+    constexpr int kU = 4;
+    for (uint32_t e0 = begin; e0 < end; e0 += kU * kReduceThreads) {
+        uint32_t w[kU][EW];
 #pragma unroll
-            for (int q = 0; q < EW / 4; ++q) {
-                const uint4 u = src[q];
-                w[4 * q] = u.x;
-                w[4 * q + 1] = u.y;
-                w[4 * q + 2] = u.z;
-                w[4 * q + 3] = u.w;
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t e = e0 + u * kReduceThreads + threadIdx.x;
+            if (e < end) {
+                load_entry<EW>(entries + (size_t)e * EW, w[u]);
+            } else {
+                w[u][0] = 0u;
+#pragma unroll
+                for (int f = 1; f < EW; ++f) w[u][f] = 0u;  // zero values add nothing
             }
         }
 #pragma unroll
-        for (int f = 0; f < F; ++f) {
-            const float v = __uint_as_float(w[1 + f]);
-            if (v != 0.0f) atomicAdd(&acc[w[0] * F + f], v);
+        for (int u = 0; u < kU; ++u) {
+#pragma unroll
+            for (int f = 0; f < F; ++f) {
+                const float v = __uint_as_float(w[u][1 + f]);
+                if (v != 0.0f) atomicAdd(&acc[w[u][0] * F + f], v);
+            }
         }
     }
     __syncthreads();
@@ -468,20 +487,40 @@ __global__ void __launch_bounds__(kReduceThreads)
     for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) out[t] = acc[t];
 }
 
-// 5. slices of several parts: rows = sum of the parts' partial rows, in part order
+// 5. slices of several parts: rows = sum of the parts' partial rows, in part
+// order.  Block (s, c) covers 1/kCombineSplit of slice s.
+constexpr uint32_t kCombineSplit = 16;
 template <int F>
-__global__ void __launch_bounds__(kReduceThreads)
+__global__ void __launch_bounds__(256)
     bwd_combine_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ parts,
                        const float *__restrict__ partials, float *__restrict__ dtable) {
     const uint32_t s = blockIdx.x;
     const uint32_t p0 = parts[s], n_parts = parts[s + 1] - p0;
     if (n_parts <= 1) return;
-    const uint32_t rows = min(d.n_rows - s * p.R, p.R);
+    const uint32_t n = min(d.n_rows - s * p.R, p.R) * F;  // floats of the slice
+    const uint32_t per = (p.R * F) / kCombineSplit;      // a multiple of 4
+    const uint32_t lo = blockIdx.y * per, hi = min(n, lo + per);
+    const size_t pstride = (size_t)p.R * F;
+    const float *src = partials + (size_t)p0 * pstride;
     float *out = dtable + (size_t)s * p.R * F;
-    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) {
-        float a = 0.0f;
-        for (uint32_t q = 0; q < n_parts; ++q) a += partials[(size_t)(p0 + q) * p.R * F + t];
-        out[t] = a;
+    for (uint32_t t = lo + 4 * threadIdx.x; t < hi; t += 4 * 256) {
+        if (t + 4 <= hi) {
+            float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            for (uint32_t q = 0; q < n_parts; ++q) {
+                const float4 v = *reinterpret_cast<const float4 *>(src + q * pstride + t);
+                a.x += v.x;
+                a.y += v.y;
+                a.z += v.z;
+                a.w += v.w;
+            }
+            *reinterpret_cast<float4 *>(out + t) = a;
+        } else {
+            for (uint32_t u = t; u < hi; ++u) {
+                float a = 0.0f;
+                for (uint32_t q = 0; q < n_parts; ++q) a += src[q * pstride + u];
+                out[u] = a;
+            }
+        }
     }
 }
 
@@ -513,8 +552,8 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
                        entries);
     hipLaunchKernelGGL(bwd_reduce_kernel<F>, dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts, meta,
                        parts, entries, partials, dtable);
-    hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices), dim3(kReduceThreads), 0, stream, d, p, parts, partials,
-                       dtable);
+    hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices, kCombineSplit), dim3(256), 0, stream, d, p, parts,
+                       partials, dtable);
     return true;
 }
 
@@ -640,8 +679,8 @@ extern "C" void hashgrid_encode_backward(ngp_stream_t stream_, void **buffers, c
         return;
     }
     uint8_t *ws = (uint8_t *)buffers[3];
-    if ((reinterpret_cast<uintptr_t>(ws) & 255u) != 0) {
-        set_error(kBadArgument, "%s: workspace must be 256-byte aligned", target);
+    if ((reinterpret_cast<uintptr_t>(ws) & 255u) != 0 || (reinterpret_cast<uintptr_t>(dtable) & 15u) != 0) {
+        set_error(kBadArgument, "%s: dL_dtable must be 16-byte and the workspace 256-byte aligned", target);
         return;
     }
     bool ok = false;

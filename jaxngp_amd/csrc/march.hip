@@ -7,21 +7,33 @@
 // atomicAdd in scheduling order and marches every ray twice.  Here the ranges are
 // an exclusive prefix sum of the per-ray counts in ray order, which makes the
 // output deterministic and equal to processing rays one after another:
-//   1. march_count:  one thread per ray marches and counts occupied steps; the
-//                    block total is parked in rays_sample_startidx[first ray of
-//                    the block] (slot owned by the block, so no scratch memory).
-//                    Rays are counted in three ray-ordered launches (8192 rays,
-//                    up to 65536, the rest); a launch whose predecessors already
-//                    filled the sample buffer skips marching, which changes no
-//                    output (those rays are invalid regardless of their counts);
+//   1. march_count:  one wavefront per ray counts its samples; the block total is
+//                    parked in rays_sample_startidx[first ray of the block] (slot
+//                    owned by the block, so no scratch memory).  Rays are counted
+//                    in ray-ordered launches (1024 rays, up to 8192, up to 65536,
+//                    the rest); between launches the running total goes to
+//                    next_sample_write_location, and a launch that finds the
+//                    sample buffer already full skips marching, which changes no
+//                    output (its rays are invalid regardless of their counts);
 //   2. march_scan:   one workgroup scans the block totals, finds the ray where the
 //                    sample buffer fills up and writes the two helper counters;
-//   3. march_write:  one thread per ray takes its offset (block prefix + wave/LDS
-//                    scan), decides validity, re-marches valid rays writing their
-//                    samples, and zero-fills the unused tail of the sample arrays.
+//   3. march_write:  one wavefront per ray takes its offset, decides validity,
+//                    re-marches valid rays writing their samples, and zero-fills
+//                    the unused tail of the sample arrays.
 // A ray is valid iff the samples of the rays before it leave room (prefix <
 // total_samples), t_end > t_start, and its own samples fit; rays after the buffer
 // fills are invalid, as the reference's early exit at marching.cu:135 makes them.
+//
+// Wavefront marching.  Every t the reference's loop visits (marching.cu:162-192)
+// lies on the sequence c_0 = t_start, c_{k+1} = c_k + step_size(c_k): an occupied
+// step moves to the next element, an empty-cell skip to the first later element
+// at or past the cell's exit.  So a wavefront generates 64 consecutive elements
+// (one per lane, a serial chain of adds), looks up all 64 cells and exit distances
+// at once, and walks the visited elements with ballots: runs of occupied lanes
+// are emitted together, a skip is a search for the first lane past the exit.
+// Samples go to their slots by lane prefix counts.  The walk visits exactly the
+// reference's t values in its order; where the chain stops growing (t + ds == t)
+// the ray finishes on one lane with the serial loop, which handles that case.
 #include "ngp_common.h"
 #include "ngp_device.h"
 
@@ -29,144 +41,372 @@ namespace ngp {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kRaysPerBlock = kBlock / kWave;  // one wavefront per ray
 constexpr int kScanThreads = 1024;
+// parked total of a block whose launch found the sample buffer already full
+constexpr uint32_t kPastFill = 0xffffffffu;
 
-// Counts (or, with `write`, emits) the samples of one ray (marching.cu:162-192, 224-267).
+struct Ray {
+    float ox, oy, oz, dx, dy, dz, idx, idy, idz, t1;
+};
+
+__device__ __forceinline__ Ray load_ray(const float *__restrict__ rays_o, const float *__restrict__ rays_d, uint32_t i,
+                                        float t1) {
+    Ray r;
+    r.ox = rays_o[3 * (size_t)i];
+    r.oy = rays_o[3 * (size_t)i + 1];
+    r.oz = rays_o[3 * (size_t)i + 2];
+    r.dx = rays_d[3 * (size_t)i];
+    r.dy = rays_d[3 * (size_t)i + 1];
+    r.dz = rays_d[3 * (size_t)i + 2];
+    r.idx = 1.0f / r.dx;
+    r.idy = 1.0f / r.dy;
+    r.idz = 1.0f / r.dz;
+    r.t1 = t1;
+    return r;
+}
+
+// Output arrays of one ray, offset to its first sample.
+struct RaySamples {
+    uint32_t *idcs;
+    float *xyzs, *dirs, *dss, *zs;
+};
+
+__device__ __forceinline__ void put_sample(const RaySamples &o, uint32_t k, uint32_t ray, const Ray &r, float t,
+                                           float ds, float px, float py, float pz) {
+    o.idcs[k] = ray;
+    o.xyzs[3 * k + 0] = px;
+    o.xyzs[3 * k + 1] = py;
+    o.xyzs[3 * k + 2] = pz;
+    o.dirs[3 * k + 0] = r.dx;
+    o.dirs[3 * k + 1] = r.dy;
+    o.dirs[3 * k + 2] = r.dz;
+    o.dss[k] = ds;
+    o.zs[k] = t;
+}
+
+// The reference's loop (marching.cu:162-192) from position t with n samples taken,
+// on one lane.
 template <bool kWrite>
-__device__ uint32_t march_ray(const MarchParams &p, const uint8_t *__restrict__ bits, bool word_ok, float ox, float oy,
-                              float oz,
-                              float dx, float dy, float dz, float t0, float t1, float noise, uint32_t max_samples,
-                              uint32_t ray, uint32_t *__restrict__ idcs, float *__restrict__ xyzs,
-                              float *__restrict__ dirs, float *__restrict__ dss, float *__restrict__ zs) {
-    const float idx_ = 1.0f / dx, idy = 1.0f / dy, idz = 1.0f / dz;
-    BitCache cache = make_bit_cache(bits, word_ok);
-    float t = t0;
-    t = t + step_size(p, t) * noise;
-    uint32_t n = 0;
-    while ((float)n < p.sample_limit && n < max_samples && t < t1) {
-        const float px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
+__device__ uint32_t march_serial(const MarchParams &p, const uint8_t *__restrict__ bits, BitCache &cache, const Ray &r,
+                                 float t, uint32_t n, uint32_t cap, uint32_t ray, const RaySamples &o) {
+    while (n < cap && t < r.t1) {
+        const float px = r.ox + t * r.dx, py = r.oy + t * r.dy, pz = r.oz + t * r.dz;
         const float ds = step_size(p, t);
         float gx, gy, gz, mb;
-        const bool occ = lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb);
-        float nt;
-        if (occ) {
-            if (kWrite) {
-                idcs[n] = ray;
-                xyzs[3 * n + 0] = px;
-                xyzs[3 * n + 1] = py;
-                xyzs[3 * n + 2] = pz;
-                dirs[3 * n + 0] = dx;
-                dirs[3 * n + 1] = dy;
-                dirs[3 * n + 2] = dz;
-                dss[n] = ds;
-                zs[n] = t;
-            }
+        if (lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb)) {
+            if (kWrite) put_sample(o, n, ray, r, t, ds, px, py, pz);
             ++n;
-            nt = t + ds;
+            t = t + ds;
         } else {
-            const float next_t = cell_exit_t(p, t, px, py, pz, gx, gy, gz, mb, dx, dy, dz, idx_, idy, idz);
-            nt = skip_to(p, t, ds, next_t);
+            const float next_t = cell_exit_t(p, t, px, py, pz, gx, gy, gz, mb, r.dx, r.dy, r.dz, r.idx, r.idy, r.idz);
+            t = skip_to(p, t, ds, next_t);
         }
-        t = nt;
     }
     return n;
 }
 
-// Counts rays of blocks [block_begin, block_end).  Blocks of later chunks first
-// add up the parked totals of all earlier blocks (final: earlier chunks ran to
-// completion before this launch); when they already reach total_samples, every ray
-// here starts past the filled buffer and is invalid whatever its count, so the
-// block records zero samples without marching.
+// Lane j's element of the chain c_0 = base, c_{j+1} = c_j + k (k > 0 constant).
+// While the chain stays in base's binade, where every float is a multiple of the
+// ulp u, each step adds the same m = k rounded to a multiple of u, so
+// c_j = base + (j*m)*u with exact arithmetic.  Every lane then checks the
+// recurrence c_{j+1} == c_j + k against its neighbour; the values are used only
+// if all checks pass (by induction they are the chain), which fails where the
+// chain leaves the binade, at rounding ties and for non-positive or non-normal
+// base.  On success also returns c_64 in `next`; a chain that stops growing
+// (next == c_63) is left to the serial generation, which detects it.
+__device__ __forceinline__ bool chain_in_binade(float base, float k, int lane, float &w, float &next) {
+    const uint32_t e8 = __float_as_uint(base) >> 23;  // sign bit included: negative base fails
+    if (e8 < 24u || e8 >= 255u) return false;
+    const float u = __uint_as_float((e8 - 23u) << 23);
+    const float m = rintf(k / u);  // division by a power of two: exact
+    if (!(m >= 1.0f && m <= 65536.0f)) return false;
+    const float x = base + ((float)lane * m) * u;
+    const float nx = x + k;
+    const float up = __shfl_down(x, 1, kWave);
+    const bool ok = lane == kWave - 1 || (up == nx && up != x);
+    if (__ballot(!ok) != 0ull) return false;
+    const float n63 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nx), kWave - 1));
+    const float x63 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), kWave - 1));
+    if (n63 == x63) return false;
+    w = x;
+    next = n63;
+    return true;
+}
+
+// Counts (kWrite: also writes) the samples of one ray with the whole wavefront,
+// starting at t_start, taking at most cap samples.  Call with all 64 lanes active.
+template <bool kWrite>
+__device__ uint32_t march_ray_wave(const MarchParams &p, const uint8_t *__restrict__ bits, bool word_ok, const Ray &r,
+                                   float t_start, uint32_t cap, uint32_t ray, const RaySamples &o) {
+    const int lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    BitCache cache = make_bit_cache(bits, word_ok);
+    // with stepsize_portion 0 every step is the same constant, for every t
+    const bool const_step = p.portion == 0.0f;
+    const float k_step = step_size(p, 0.0f);
+    uint32_t n = 0;
+    float base = t_start;  // sequence element of lane 0
+    bool pending = false;  // a skip is searching for the first element >= thr
+    float thr = 0.0f;
+    while (n < cap) {
+        float w = base, c = base;
+        bool stall = false;
+        if (const_step && chain_in_binade(base, k_step, lane, w, c)) {
+            // 64 elements at once
+        } else if (const_step) {
+#pragma unroll
+            for (int q = 0; q < kWave; ++q) {
+                if (lane == q) w = c;
+                const float nx = c + k_step;
+                stall |= nx == c;
+                c = nx;
+            }
+        } else {
+#pragma unroll 8
+            for (int q = 0; q < kWave; ++q) {
+                if (lane == q) w = c;
+                const float nx = c + step_size(p, c);
+                stall |= nx == c;
+                c = nx;
+            }
+        }
+        if (__ballot(stall) != 0ull) {
+            uint32_t nn = n;
+            if (lane == 0) {
+                float t = base;
+                if (pending) {
+                    // rest of skip_to: t is the first step past the skipped element
+                    while (t < thr) {
+                        const float s2 = t + step_size(p, t);
+                        if (s2 == t) {
+                            t = __builtin_huge_valf();
+                            break;
+                        }
+                        t = s2;
+                    }
+                }
+                nn = march_serial<kWrite>(p, bits, cache, r, t, n, cap, ray, o);
+            }
+            return (uint32_t)__builtin_amdgcn_readlane((int)nn, 0);
+        }
+        const bool live = w < r.t1;
+        const float px = r.ox + w * r.dx, py = r.oy + w * r.dy, pz = r.oz + w * r.dz;
+        const float ds = step_size(p, w);
+        bool occ = false;
+        float ex = 0.0f;
+        if (live) {
+            float gx, gy, gz, mb;
+            occ = lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb);
+            if (!occ) ex = cell_exit_t(p, w, px, py, pz, gx, gy, gz, mb, r.dx, r.dy, r.dz, r.idx, r.idy, r.idz);
+        }
+        const uint64_t live_m = __ballot(live);
+        const uint64_t occ_m = __ballot(occ);
+        const uint32_t n0 = n;
+        uint64_t emit = 0ull;
+        bool done = false;
+        int cur = 0;
+        while (cur < kWave) {
+            if (pending) {
+                // skip_to stops at the first element with !(t < thr)
+                const uint64_t m = __ballot(!(w < thr)) & (~0ull << cur);
+                if (m == 0ull) {
+                    // the target lies past this window; if the window already
+                    // reached t_end, so does the target and the ray is done
+                    done = ((live_m >> (kWave - 1)) & 1ull) == 0ull;
+                    break;
+                }
+                cur = __builtin_ctzll(m);
+                pending = false;
+            }
+            if (((live_m >> cur) & 1ull) == 0ull) {
+                done = true;
+                break;
+            }
+            if ((occ_m >> cur) & 1ull) {
+                const uint64_t from = occ_m >> cur;
+                const int run = ~from == 0ull ? kWave - cur : __builtin_ctzll(~from);
+                const uint32_t take = min((uint32_t)run, cap - n);
+                emit |= (take >= 64u ? ~0ull : ((1ull << take) - 1ull)) << cur;
+                n += take;
+                if (n >= cap) {
+                    done = true;
+                    break;
+                }
+                cur += run;
+            } else {
+                thr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex), cur));
+                pending = true;
+                cur += 1;
+            }
+        }
+        if (kWrite && ((emit >> lane) & 1ull)) {
+            put_sample(o, n0 + (uint32_t)__popcll(emit & below), ray, r, w, ds, px, py, pz);
+        }
+        if (done) break;
+        base = c;
+    }
+    return n;
+}
+
+__device__ __forceinline__ float start_t(const MarchParams &p, float t0, float noise) {
+    return t0 + step_size(p, t0) * noise;  // marching.cu:159-160
+}
+
+// Counts the rays of blocks [block_begin, block_end) (grid-stride).  With
+// check_running, a launch whose predecessors already took total_samples samples
+// (running total in *running) marks its blocks kPastFill without marching.
 __global__ void __launch_bounds__(kBlock)
     march_count_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples, uint32_t block_begin,
+                       uint32_t block_end, bool check_running, const uint32_t *__restrict__ running,
                        const float *__restrict__ rays_o, const float *__restrict__ rays_d,
                        const float *__restrict__ t_starts, const float *__restrict__ t_ends,
                        const float *__restrict__ noises, const uint8_t *__restrict__ bits,
                        uint32_t *__restrict__ rays_n_samples,
                        uint32_t *__restrict__ block_totals /* = rays_sample_startidx */) {
-    __shared__ uint32_t wave_totals[kBlock / kWave];
-    __shared__ uint32_t skip;
-    const uint32_t blk = block_begin + blockIdx.x;
-    if (threadIdx.x < kWave) {
-        uint64_t prior = 0;
-        for (uint32_t b = threadIdx.x; b < block_begin; b += kWave) prior += block_totals[(size_t)b * kBlock];
-        prior = wave_sum64(prior);
-        if (threadIdx.x == 0) skip = prior >= total_samples ? 1u : 0u;
+    if (check_running && *running >= total_samples) {
+        for (uint32_t b = block_begin + blockIdx.x * kBlock + threadIdx.x; b < block_end; b += gridDim.x * kBlock)
+            block_totals[(size_t)b * kRaysPerBlock] = kPastFill;
+        return;
     }
-    __syncthreads();
-    const uint32_t i = blk * kBlock + threadIdx.x;
-    uint32_t n = 0;
-    if (i < n_rays) {
-        const float t0 = t_starts[i], t1 = t_ends[i];
-        if (!skip && t1 > t0) {
-            n = march_ray<false>(p, bits, word_ok, rays_o[3 * (size_t)i], rays_o[3 * (size_t)i + 1], rays_o[3 * (size_t)i + 2],
-                                 rays_d[3 * (size_t)i], rays_d[3 * (size_t)i + 1], rays_d[3 * (size_t)i + 2], t0, t1,
-                                 noises[i], 0xffffffffu, i, nullptr, nullptr, nullptr, nullptr, nullptr);
+    __shared__ uint32_t wave_n[kRaysPerBlock];
+    const int wave = (int)threadIdx.x / kWave;
+    for (uint32_t blk = block_begin + blockIdx.x; blk < block_end; blk += gridDim.x) {
+        const uint32_t i = blk * kRaysPerBlock + wave;
+        uint32_t n = 0;
+        if (i < n_rays) {
+            const float t0 = t_starts[i], t1 = t_ends[i];
+            if (t1 > t0) {
+                const Ray r = load_ray(rays_o, rays_d, i, t1);
+                n = march_ray_wave<false>(p, bits, word_ok, r, start_t(p, t0, noises[i]), p.n_cap, i, RaySamples{});
+            }
+            if (lane_id() == 0) rays_n_samples[i] = n;
         }
-        rays_n_samples[i] = n;
-    }
-    const uint32_t ws = wave_sum(n);
-    if (lane_id() == 0) wave_totals[threadIdx.x / kWave] = ws;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t s = 0;
+        if (lane_id() == 0) wave_n[wave] = n;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t s = 0;
 #pragma unroll
-        for (int w = 0; w < kBlock / kWave; ++w) s += wave_totals[w];
-        block_totals[(size_t)blk * kBlock] = s;
+            for (int w = 0; w < kRaysPerBlock; ++w) s += wave_n[w];
+            block_totals[(size_t)blk * kRaysPerBlock] = (uint32_t)min(s, (uint64_t)kPastFill - 1);
+        }
+        __syncthreads();
     }
 }
 
+// Single workgroup: *running (+)= the parked totals of blocks [b0, b1), saturated.
+__global__ void __launch_bounds__(kScanThreads)
+    march_accumulate_kernel(uint32_t b0, uint32_t b1, bool first, const uint32_t *__restrict__ block_totals,
+                            uint32_t *__restrict__ running) {
+    __shared__ uint64_t part[kScanThreads / kWave];
+    uint64_t s = 0;
+    for (uint32_t b = b0 + threadIdx.x; b < b1; b += kScanThreads) {
+        const uint32_t v = block_totals[(size_t)b * kRaysPerBlock];
+        s += v == kPastFill ? 0u : v;
+    }
+    s = wave_sum64(s);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = first ? 0ull : *running;
+        for (int w = 0; w < kScanThreads / kWave; ++w) t += part[w];
+        *running = t > 0xffffffffull ? 0xffffffffu : (uint32_t)t;
+    }
+}
+
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+    const uint32_t c = a + b;
+    return c < a ? 0xffffffffu : c;
+}
+
 // Single workgroup.  Replaces each parked block total with the block's exclusive
-// prefix (saturated to u32; only its comparison with total_samples matters) and
-// writes next_sample_write_location / number_of_exceeded_samples.
+// prefix (saturated: only its comparison with total_samples matters) and writes
+// next_sample_write_location / number_of_exceeded_samples.  Blocks are read in
+// tiles of 4096 with all loads of a tile in flight; the scan stops at the first
+// block of a skipped launch (every later block was skipped too, and their
+// kPastFill slots already mark their rays as past the filled buffer).
+constexpr int kScanPerThread = 4;
 __global__ void __launch_bounds__(kScanThreads)
     march_scan_kernel(uint32_t n_rays, uint32_t n_blocks, uint32_t total_samples,
                       const uint32_t *__restrict__ rays_n_samples, uint32_t *__restrict__ block_slots,
                       uint32_t *__restrict__ next_loc, uint32_t *__restrict__ exceeded) {
-    __shared__ uint64_t partial[kScanThreads];
+    constexpr uint32_t kTile = kScanThreads * kScanPerThread;
+    __shared__ uint32_t first_past;
     __shared__ uint32_t fill_block;
+    __shared__ uint32_t wave_tot[kScanThreads / kWave];
     const uint32_t tid = threadIdx.x;
-    const uint32_t per = (n_blocks + kScanThreads - 1) / kScanThreads;
-    const uint32_t b0 = tid * per;
-    uint64_t local = 0;
-    for (uint32_t k = 0; k < per; ++k) {
-        uint32_t b = b0 + k;
-        if (b < n_blocks) local += block_slots[(size_t)b * kBlock];
-    }
-    partial[tid] = local;
+    const int lane = lane_id(), wave = (int)tid / kWave;
     if (tid == 0) fill_block = 0xffffffffu;
-    __syncthreads();
-    // inclusive Hillis-Steele scan of the per-thread sums (u64)
-    for (uint32_t d = 1; d < kScanThreads; d <<= 1) {
-        uint64_t v = tid >= d ? partial[tid - d] : 0;
+    uint32_t carry = 0;  // saturated prefix of the tiles before
+    uint32_t scanned = 0;
+    for (uint32_t t0 = 0; t0 < n_blocks; t0 += kTile) {
+        if (tid == 0) first_past = 0xffffffffu;
+        uint32_t v[kScanPerThread];
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k) {
+            const uint32_t b = t0 + tid * kScanPerThread + k;
+            v[k] = b < n_blocks ? block_slots[(size_t)b * kRaysPerBlock] : kPastFill;
+        }
         __syncthreads();
-        partial[tid] += v;
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k)
+            if (v[k] == kPastFill) atomicMin(&first_past, t0 + tid * kScanPerThread + k);
         __syncthreads();
-    }
-    const uint64_t grand = partial[kScanThreads - 1];
-    uint64_t run = partial[tid] - local;  // exclusive prefix of this thread's first block
-    for (uint32_t k = 0; k < per; ++k) {
-        uint32_t b = b0 + k;
-        if (b >= n_blocks) break;
-        uint32_t tot = block_slots[(size_t)b * kBlock];
-        // first block whose inclusive prefix passes total_samples holds the fill point
-        if (run + tot > total_samples && run <= total_samples) atomicMin(&fill_block, b);
-        block_slots[(size_t)b * kBlock] = run > 0xffffffffull ? 0xffffffffu : (uint32_t)run;
-        run += tot;
+        const uint32_t stop = first_past;
+        uint32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k) {
+            if (t0 + tid * kScanPerThread + k >= stop) v[k] = 0;
+            mine = sat_add(mine, v[k]);
+        }
+        // inclusive saturated scan over the wave, then over the waves
+        uint32_t incl = mine;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl = sat_add(incl, o);
+        }
+        if (lane == kWave - 1) wave_tot[wave] = incl;
+        __syncthreads();
+        uint32_t before = carry, tile_total = carry;
+        for (int w = 0; w < kScanThreads / kWave; ++w) {
+            if (w < wave) before = sat_add(before, wave_tot[w]);
+            tile_total = sat_add(tile_total, wave_tot[w]);
+        }
+        uint32_t run = sat_add(before, incl - mine);  // exclusive prefix of this thread's first block
+#pragma unroll
+        for (int k = 0; k < kScanPerThread; ++k) {
+            const uint32_t b = t0 + tid * kScanPerThread + k;
+            if (b < stop && b < n_blocks) {
+                // the first block whose inclusive prefix passes total_samples holds the fill point
+                const uint32_t incl_b = sat_add(run, v[k]);
+                if (incl_b > total_samples && run <= total_samples) atomicMin(&fill_block, b);
+                block_slots[(size_t)b * kRaysPerBlock] = run;
+                run = incl_b;
+            }
+        }
+        carry = tile_total;
+        __syncthreads();  // wave_tot and first_past are rewritten by the next tile
+        if (stop != 0xffffffffu) {
+            scanned = min(stop, n_blocks);
+            break;
+        }
+        scanned = min(n_blocks, t0 + kTile);
     }
     __syncthreads();
     if (tid == 0) {
+        const uint32_t grand = carry;
         if (grand <= total_samples) {
-            *next_loc = (uint32_t)grand;
+            *next_loc = grand;
             *exceeded = 0;
         } else {
             // walk the fill block's rays: first ray whose inclusive prefix passes total_samples
-            uint32_t b = fill_block < n_blocks ? fill_block : n_blocks - 1;  // always set when grand > S
-            uint64_t pre = block_slots[(size_t)b * kBlock];
-            uint32_t first = b * kBlock, last = min(n_rays, first + kBlock);
+            const uint32_t b = fill_block < scanned ? fill_block : scanned - 1;  // set whenever grand > S
+            uint64_t pre = block_slots[(size_t)b * kRaysPerBlock];
+            const uint32_t first = b * kRaysPerBlock, last = min(n_rays, first + kRaysPerBlock);
             uint32_t measured = (uint32_t)pre, over = 0;
             for (uint32_t r = first; r < last; ++r) {
-                uint32_t n = rays_n_samples[r];
+                const uint32_t n = rays_n_samples[r];
                 if (pre + n > total_samples) {
                     measured = (uint32_t)pre;
                     over = pre < total_samples ? n : 0;
@@ -180,38 +420,71 @@ __global__ void __launch_bounds__(kScanThreads)
     }
 }
 
+// Sample range of ray i from the scanned block prefixes (slot of i's block) and
+// the counts of the rays before i in its block.
+__device__ __forceinline__ uint64_t ray_start(const uint32_t *__restrict__ rays_n_samples,
+                                              const uint32_t *__restrict__ block_slots, uint32_t i) {
+    const uint32_t first = i / kRaysPerBlock * kRaysPerBlock;
+    uint64_t start = block_slots[first];
+    for (uint32_t j = first; j < i; ++j) start += rays_n_samples[j];
+    return start;
+}
+
+// Wavefront per ray, grid-stride over rays: re-marches the valid rays, writing
+// their samples.  Stops at the first ray whose block prefix reaches
+// total_samples (every later ray is past the filled buffer).  Runs before
+// march_finish_kernel overwrites the block slots.
 __global__ void __launch_bounds__(kBlock)
-    march_write_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples,
-                       const float *__restrict__ rays_o,
-                       const float *__restrict__ rays_d, const float *__restrict__ t_starts,
-                       const float *__restrict__ t_ends, const float *__restrict__ noises,
-                       const uint8_t *__restrict__ bits, const uint32_t *__restrict__ next_loc,
-                       const uint32_t *__restrict__ exceeded, bool *__restrict__ ray_is_valid,
-                       uint32_t *__restrict__ rays_n_samples, uint32_t *__restrict__ rays_sample_startidx,
-                       uint32_t *__restrict__ idcs, float *__restrict__ xyzs, float *__restrict__ dirs,
-                       float *__restrict__ dss, float *__restrict__ zs) {
-    __shared__ uint32_t block_prefix;
+    march_emit_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples,
+                      const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+                      const float *__restrict__ t_starts, const float *__restrict__ t_ends,
+                      const float *__restrict__ noises, const uint8_t *__restrict__ bits,
+                      const uint32_t *__restrict__ rays_n_samples, const uint32_t *__restrict__ block_slots,
+                      uint32_t *__restrict__ idcs, float *__restrict__ xyzs, float *__restrict__ dirs,
+                      float *__restrict__ dss, float *__restrict__ zs) {
+    const uint32_t n_waves = gridDim.x * kRaysPerBlock;
+    for (uint32_t i = blockIdx.x * kRaysPerBlock + threadIdx.x / kWave; i < n_rays; i += n_waves) {
+        if (block_slots[i / kRaysPerBlock * kRaysPerBlock] >= total_samples) break;
+        const uint64_t start = ray_start(rays_n_samples, block_slots, i);
+        const uint32_t n = rays_n_samples[i];
+        const float t0 = t_starts[i], t1 = t_ends[i];
+        if (start < total_samples && t1 > t0 && start + n <= total_samples && n > 0) {
+            const uint32_t s = (uint32_t)start;
+            const Ray r = load_ray(rays_o, rays_d, i, t1);
+            const RaySamples o{idcs + s, xyzs + 3 * (size_t)s, dirs + 3 * (size_t)s, dss + s, zs + s};
+            march_ray_wave<true>(p, bits, word_ok, r, start_t(p, t0, noises[i]), min(p.n_cap, n), i, o);
+        }
+    }
+}
+
+// Thread per ray: validity, final counts and start indices (replacing the block
+// slots), and zeros in the sample slots no valid ray owns, [measured, total_samples).
+__global__ void __launch_bounds__(kBlock)
+    march_finish_kernel(uint32_t n_rays, uint32_t total_samples, const float *__restrict__ t_starts,
+                        const float *__restrict__ t_ends, const uint32_t *__restrict__ next_loc,
+                        const uint32_t *__restrict__ exceeded, bool *__restrict__ ray_is_valid,
+                        uint32_t *__restrict__ rays_n_samples, uint32_t *__restrict__ rays_sample_startidx,
+                        uint32_t *__restrict__ idcs, float *__restrict__ xyzs, float *__restrict__ dirs,
+                        float *__restrict__ dss, float *__restrict__ zs) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (threadIdx.x == 0) block_prefix = rays_sample_startidx[(size_t)blockIdx.x * kBlock];
+    const int lane = lane_id();
+    // a block's rays are kRaysPerBlock consecutive lanes of one wavefront
     const uint32_t n = i < n_rays ? rays_n_samples[i] : 0u;
-    __syncthreads();
-    uint32_t block_total;
-    const uint32_t local = block_exclusive_scan<kBlock>(n, &block_total);
-    const uint64_t start = (uint64_t)block_prefix + local;
+    const uint32_t slot = i < n_rays ? rays_sample_startidx[i / kRaysPerBlock * kRaysPerBlock] : 0u;
+    const int k = lane % kRaysPerBlock;
+    uint64_t start = slot;
+#pragma unroll
+    for (int d = 1; d < kRaysPerBlock; ++d) {
+        const uint32_t o = __shfl_up(n, d, kWave);
+        if (k >= d) start += o;
+    }
     if (i < n_rays) {
         const float t0 = t_starts[i], t1 = t_ends[i];
         const bool valid = start < total_samples && t1 > t0 && start + n <= total_samples;
         ray_is_valid[i] = valid;
         rays_n_samples[i] = valid ? n : 0u;
         rays_sample_startidx[i] = (valid && n > 0) ? (uint32_t)start : 0u;
-        if (valid && n > 0) {
-            const uint32_t s = (uint32_t)start;
-            march_ray<true>(p, bits, word_ok, rays_o[3 * (size_t)i], rays_o[3 * (size_t)i + 1], rays_o[3 * (size_t)i + 2],
-                            rays_d[3 * (size_t)i], rays_d[3 * (size_t)i + 1], rays_d[3 * (size_t)i + 2], t0, t1,
-                            noises[i], n, i, idcs + s, xyzs + 3 * (size_t)s, dirs + 3 * (size_t)s, dss + s, zs + s);
-        }
     }
-    // zero the sample slots no valid ray owns: [measured, total_samples)
     const uint32_t measured = *next_loc - *exceeded;
     const size_t stride = (size_t)gridDim.x * kBlock;
     for (size_t j = (size_t)measured + blockIdx.x * kBlock + threadIdx.x; j < total_samples; j += stride) {
@@ -404,24 +677,35 @@ extern "C" void march_rays(ngp_stream_t stream_, void **buffers, const char *opa
         check_launch(target);
         return;
     }
-    const uint32_t n_blocks = div_up(d.n_rays, kBlock);
-    // count in ray-ordered chunks of 32, 256 and all remaining blocks
-    const uint32_t bounds[4] = {0, std::min(n_blocks, 32u), std::min(n_blocks, 256u), n_blocks};
-    for (int c = 0; c < 3; ++c) {
+    const uint32_t n_blocks = div_up(d.n_rays, kRaysPerBlock);
+    // count in ray-ordered launches of 2048, 16384, 131072 and all rays; grids
+    // of at most kMaxCountBlocks blocks (8192 resident wavefronts) stride over them
+    constexpr uint32_t kMaxCountBlocks = 2048;
+    const uint32_t bounds[5] = {0, std::min(n_blocks, 2048u / kRaysPerBlock),
+                                std::min(n_blocks, 16384u / kRaysPerBlock), std::min(n_blocks, 131072u / kRaysPerBlock),
+                                n_blocks};
+    for (int c = 0; c < 4; ++c) {
         if (bounds[c + 1] <= bounds[c]) continue;
-        hipLaunchKernelGGL(march_count_kernel, dim3(bounds[c + 1] - bounds[c]), dim3(kBlock), 0, stream, p, word_ok,
-                           d.n_rays,
-                           d.total_samples, bounds[c], rays_o, rays_d, t_starts, t_ends, noises, bits, n_samples,
-                           startidx);
+        hipLaunchKernelGGL(march_count_kernel, dim3(std::min(bounds[c + 1] - bounds[c], kMaxCountBlocks)), dim3(kBlock),
+                           0, stream, p, word_ok, d.n_rays, d.total_samples, bounds[c], bounds[c + 1], c > 0, next_loc,
+                           rays_o, rays_d, t_starts, t_ends, noises, bits, n_samples, startidx);
         if (!check_launch(target)) return;
+        if (bounds[c + 1] < n_blocks) {
+            hipLaunchKernelGGL(march_accumulate_kernel, dim3(1), dim3(kScanThreads), 0, stream, bounds[c],
+                               bounds[c + 1], c == 0, startidx, next_loc);
+            if (!check_launch(target)) return;
+        }
     }
     hipLaunchKernelGGL(march_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, d.n_rays, n_blocks, d.total_samples,
                        n_samples, startidx, next_loc, exceeded);
     if (!check_launch(target)) return;
-    hipLaunchKernelGGL(march_write_kernel, dim3(n_blocks), dim3(kBlock), 0, stream, p, word_ok, d.n_rays,
-                       d.total_samples,
-                       rays_o, rays_d, t_starts, t_ends, noises, bits, next_loc, exceeded, valid, n_samples, startidx,
+    hipLaunchKernelGGL(march_emit_kernel, dim3(std::min(n_blocks, kMaxCountBlocks)), dim3(kBlock), 0, stream, p, word_ok,
+                       d.n_rays, d.total_samples, rays_o, rays_d, t_starts, t_ends, noises, bits, n_samples, startidx,
                        idcs, xyzs, dirs, dss, zs);
+    if (!check_launch(target)) return;
+    hipLaunchKernelGGL(march_finish_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
+                       d.total_samples, t_starts, t_ends, next_loc, exceeded, valid, n_samples, startidx, idcs, xyzs,
+                       dirs, dss, zs);
     check_launch(target);
 }
 

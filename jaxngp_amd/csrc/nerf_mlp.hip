@@ -371,13 +371,33 @@ __global__ void __launch_bounds__(kThreads)
     for (int t = threadIdx.x; t < P_END; t += kThreads) row[t] = red[t];
 }
 
-__global__ void __launch_bounds__(256)
+// out[p] = sum of workspace rows at column p.  Block: 64 columns x 16 waves; wave
+// w sums rows w, w+16, ... with 8 loads in flight, then the waves' sums are added
+// in wave order, so the result does not depend on scheduling.
+constexpr int kSumWaves = 16;
+__global__ void __launch_bounds__(kSumWaves * kWave)
     sum_rows_kernel(uint32_t rows, const float *__restrict__ workspace, float *__restrict__ out) {
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= (uint32_t)P_END) return;
-    float acc = 0.0f;
-    for (uint32_t r = 0; r < rows; ++r) acc += workspace[(size_t)r * P_END + p];
-    out[p] = acc;
+    __shared__ float part[kSumWaves][kWave];
+    const int lane = lane_id(), wave = (int)threadIdx.x / kWave;
+    const uint32_t p = blockIdx.x * kWave + lane;
+    const uint32_t pc = p < (uint32_t)P_END ? p : (uint32_t)P_END - 1;
+    float acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0f;
+    uint32_t r = wave;
+    for (; r + 7 * kSumWaves < rows; r += 8 * kSumWaves) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += workspace[(size_t)(r + u * kSumWaves) * P_END + pc];
+    }
+    for (; r < rows; r += kSumWaves) acc[0] += workspace[(size_t)r * P_END + pc];
+    part[wave][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (wave == 0 && p < (uint32_t)P_END) {
+        float s = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kSumWaves; ++w) s += part[w][lane];
+        out[p] = s;
+    }
 }
 
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
@@ -431,7 +451,8 @@ extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const ch
                        d.enc_layout == NGP_ENC_LEVEL_MAJOR, (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
                        (const float *)buffers[3], (float *)buffers[4], (float *)buffers[6]);
     if (!check_launch(target)) return;
-    hipLaunchKernelGGL(sum_rows_kernel, dim3(div_up(P_END, 256)), dim3(256), 0, stream, d.n_workspace_blocks,
+    hipLaunchKernelGGL(sum_rows_kernel, dim3(div_up(P_END, kWave)), dim3(kSumWaves * kWave), 0, stream,
+                       d.n_workspace_blocks,
                        (const float *)buffers[6], (float *)buffers[5]);
     check_launch(target);
 }

@@ -37,6 +37,9 @@ __device__ __forceinline__ uint32_t morton_compact(uint32_t x) {
 struct MarchParams {
     uint32_t diag, K, G, G3;
     float bound, portion, inv_G, ds_lo, ds_hi, sample_limit;
+    // samples a ray may take: the loop test (float)n < sample_limit holds exactly
+    // for n < n_cap
+    uint32_t n_cap;
 };
 
 inline MarchParams make_march_params(uint32_t diag, uint32_t K, uint32_t G, float bound, float portion) {
@@ -52,6 +55,14 @@ inline MarchParams make_march_params(uint32_t diag, uint32_t K, uint32_t G, floa
     p.ds_lo = ((2.0f * kSqrt3) * (bound < 1.0f ? bound : 1.0f)) / (float)diag;
     p.ds_hi = ((2.0f * kSqrt3) * bound) * p.inv_G;
     p.sample_limit = (float)diag * bound;  // marching.cu:165
+    // smallest n with (float)n >= sample_limit ((float)n is monotone in n)
+    uint64_t lo = 0, hi = 0x100000000ull;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if ((float)mid >= p.sample_limit) hi = mid;
+        else lo = mid + 1;
+    }
+    p.n_cap = lo > 0xffffffffull ? 0xffffffffu : (uint32_t)lo;
     return p;
 }
 

@@ -11,7 +11,7 @@ import torch
 
 import oracle
 from jaxngp_amd import volrend
-from tests.helpers import blob_bitfield, integrate_inputs, march_inputs
+from tests.helpers import blob_bitfield, integrate_inputs, march_inputs, near_far_np
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,30 @@ def test_march_rays_bit_exact(dev, case):
     got, want = _march_both(dev, case["S"], 1024, case["K"], case["G"], case["bound"], case["portion"], o, d, t0, t1,
                             noise, bits)
     _assert_march_equal(got, want)
+
+
+@pytest.mark.parametrize("dist", [3.0e4, 1.0e5])
+def test_march_rays_far_origins(dev, dist):
+    """Origins far outside the box: at t ~ 3e4 a step of 2*sqrt(3)/1024 rounds to one
+    ulp of t (the visited t differ from t0 + k*ds); at t ~ 1e5 it rounds to zero, so
+    t + ds == t: occupied cells then repeat the same sample up to the step cap
+    (diagonal_n_steps * bound) and empty cells end the ray (the reference loops
+    forever there; the skip ends it instead)."""
+    rng = np.random.default_rng(int(dist))
+    n = 300
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    target = rng.uniform(-0.5, 0.5, (n, 3))
+    o = (target - d * dist).astype(np.float32)
+    d = d.astype(np.float32)
+    t0, t1 = near_far_np(o, d, 1.0)
+    noise = rng.uniform(0, 1, n).astype(np.float32)
+    bits, _ = blob_bitfield(rng, G=128, K=1)
+    if dist > 5e4:
+        bits[: bits.size // 2] = 255  # occupied cells where the stalled rays start
+    got, want = _march_both(dev, 2 ** 20, 1024, 1, 128, 1.0, 0.0, o, d, t0, t1, noise, bits)
+    _assert_march_equal(got, want)
+    assert int(want["measured"]) > 0
 
 
 def test_march_rays_full_and_empty_fields(dev, rng):
