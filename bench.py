@@ -71,6 +71,96 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
                       f"occupancy grid: oracle C marcher/encoder/integrator (OpenMP) + numpy f32 MLP"}
 
 
+def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
+    """Trains W warm-up steps, then times K steps.  Per-target launch durations come
+    from HIP events around each bound call, on the stream the kernels are launched
+    on, inside the timed region (recording them costs host time only; the enqueue
+    time logged stays under the step time)."""
+    import torch
+    from jaxngp_amd.train import Trainer
+
+    tr = Trainer(data, cfg, device=dev)
+    for i in range(warmup):
+        tr.step()
+        if i % 100 == 0:
+            torch.cuda.synchronize()
+            log(f"[{cfg.precision}] warmup step {i}: {tr.metrics()}")
+    torch.cuda.synchronize()
+    timed = ["march_rays", "hashgrid_encode", "nerf_mlp_forward", "integrate_rays", "integrate_rays_backward",
+             "nerf_mlp_backward", "hashgrid_encode_backward", "adam_update"]
+    events: dict = {k: [] for k in timed}
+
+    class Timed:
+        def __init__(self, call):
+            self.call = call
+
+        def __getattr__(self, a):
+            return getattr(self.call, a)
+
+        def __call__(self, opaque=None):
+            stream = torch.cuda.current_stream()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            self.call(opaque)
+            b.record(stream)
+            events[self.call.name].append((a, b))
+
+    plain = tr._calls
+    tr._calls = {k: (Timed(v) if hasattr(v, "name") and v.name in events else v) for k, v in plain.items()}
+    samples = torch.zeros((), dtype=torch.int64, device=dev)
+    rays = torch.zeros((), dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        tr.step()
+        b = tr.b
+        samples += b["next_loc"].view(torch.int32).long()[0] - b["exceeded"].view(torch.int32).long()[0]
+        rays += b["n_valid"].view(torch.int32).long()[0]
+    t_host = time.perf_counter() - t_start
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    tr._calls = plain
+    n_samples = int(samples.item())
+    n_valid = int(rays.item())
+    ms = elapsed / steps * 1e3
+    log(f"[{cfg.precision}] timed {steps} steps: {ms:.3f} ms/step ({t_host / steps * 1e3:.3f} ms/step to "
+        f"enqueue), {n_samples / steps:.0f} samples/step, {n_valid / steps:.0f} valid rays/step, "
+        f"metrics {tr.metrics()}")
+    return dict(trainer=tr, events=events, elapsed=elapsed, samples=n_samples, valid=n_valid, ms=ms)
+
+
+def bench_inference(tr, data, n_frames: int, dev) -> dict:
+    """ms per 800x800 frame of the trained field from held-out poses, with the
+    reference's ray-pool renderer settings (8192 slots, march_steps_cap 8, white
+    background), median over the timed frames after two warm-up renders; PSNR of
+    the u8 render against the float64 ground truth blended on white."""
+    import torch
+    from jaxngp_amd import scene as sc
+    from jaxngp_amd.renderer import render_image_inference
+
+    poses = sc.look_at_poses(n_frames, seed=2, golden=False)
+    gt = sc.make_views(data.scene, data.camera, poses, dev)
+    c = tr.cfg
+    kw = dict(K=tr.K, G=c.grid_res, diagonal_n_steps=c.diagonal_n_steps, stepsize_portion=tr.portion,
+              march_steps_cap=8, pool=8192)
+    pose_t = [torch.as_tensor(p, dtype=torch.float32, device=dev) for p in poses]
+    for i in range(2):
+        render_image_inference(tr.model, data.camera, pose_t[i % n_frames], tr.grid.bitfield, **kw)
+    times, psnrs = [], []
+    for i in range(n_frames):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, rgb, _ = render_image_inference(tr.model, data.camera, pose_t[i], tr.grid.bitfield, **kw)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t) * 1e3)
+        psnrs.append(sc.psnr_u8(rgb, sc.blend_u8(gt.rgba[i], (1.0, 1.0, 1.0))))
+    times.sort()
+    log(f"inference: {times[len(times) // 2]:.2f} ms/frame (median of {n_frames}), PSNR {np.mean(psnrs):.2f} dB")
+    return {"ms_per_frame": times[len(times) // 2], "frames": n_frames, "psnr_db": float(np.mean(psnrs)),
+            "resolution": "800x800", "pool": 8192, "march_steps_cap": 8, "background": "white",
+            "train_steps": tr.step_count}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,14 +169,29 @@ def main() -> None:
     ap.add_argument("--views", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--frames", type=int, default=5, help="timed 800x800 inference frames (held-out poses)")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the f16-weight precision path")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy 5 times (the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20  # floats: 256 MiB
+        src = torch.ones(n, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     import torch
     from jaxngp_amd import scene as sc
-    from jaxngp_amd.train import TrainConfig, Trainer
+    from jaxngp_amd.train import TrainConfig
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
@@ -97,50 +202,10 @@ def main() -> None:
     torch.cuda.synchronize()
     log(f"scene: {args.views} views {cam.width}x{cam.height} in {time.perf_counter() - t0:.1f}s")
 
-    cfg = TrainConfig()
-    tr = Trainer(data, cfg, device=dev)
-    for i in range(args.warmup):
-        tr.step()
-        if i % 100 == 0:
-            torch.cuda.synchronize()
-            log(f"warmup step {i}: {tr.metrics()}")
-    torch.cuda.synchronize()
-
-    # per-target HIP events on the launch stream
-    timed = ["march_rays", "hashgrid_encode", "nerf_mlp_forward", "integrate_rays", "integrate_rays_backward",
-             "nerf_mlp_backward", "hashgrid_encode_backward", "adam_update"]
-    from jaxngp_amd import _lib
-    events: dict = {k: [] for k in timed}
-    orig_call = _lib.call
-
-    def timed_call(name, tensors, opaque, stream=None):
-        if name in events:
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            orig_call(name, tensors, opaque, stream)
-            b.record()
-            events[name].append((a, b))
-        else:
-            orig_call(name, tensors, opaque, stream)
-
-    samples = torch.zeros((), dtype=torch.int64, device=dev)
-    rays = torch.zeros((), dtype=torch.int64, device=dev)
-    _lib.call = timed_call
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        tr.step()
-        b = tr.b
-        samples += b["next_loc"].view(torch.int32).long()[0] - b["exceeded"].view(torch.int32).long()[0]
-        rays += b["n_valid"].view(torch.int32).long()[0]
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    _lib.call = orig_call
-    n_samples = int(samples.item())
-    n_valid = int(rays.item())
-    ms = elapsed / args.steps * 1e3
-    log(f"timed {args.steps} steps: {ms:.3f} ms/step, {n_samples / args.steps:.0f} samples/step, "
-        f"{n_valid / args.steps:.0f} valid rays/step, metrics {tr.metrics()}")
+    res = run_training(data, TrainConfig(), args.steps, args.warmup, dev)
+    tr, events, elapsed, n_samples, n_valid, ms = (res[k] for k in ("trainer", "events", "elapsed", "samples",
+                                                                      "valid", "ms"))
+    cfg = tr.cfg
 
     per = {}
     for k, lst in events.items():
@@ -190,9 +255,21 @@ def main() -> None:
                    "path": "plain-fp32"},
         "rays_per_s": n_valid / elapsed,
         "samples_per_step": spp,
-        "kernel_ms_per_step": {k: v for k, v in per.items()},
+        "kernel_ms_per_launch": {k: v for k, v in per.items()},
         "roofline": roof,
     }
+    if args.frames > 0:
+        out["inference"] = bench_inference(tr, data, args.frames, dev)
+    if not args.no_fp16:
+        # the precision path on the same scene: f16 table and MLP weights, loss scaling
+        r16 = run_training(data, TrainConfig(precision="fp16"), args.steps, args.warmup, dev)
+        t16 = r16["trainer"]
+        out["fp16"] = {"value": r16["samples"] / r16["elapsed"], "unit": "samples/s", "ms_per_step": r16["ms"],
+                       "kernel_ms_per_launch": {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
+                                              for k, v in r16["events"].items() if v},
+                       "loss": t16.metrics()["loss"], "loss_fp32": tr.metrics()["loss"]}
+        if args.frames > 0:
+            out["fp16"]["inference"] = bench_inference(t16, data, args.frames, dev)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

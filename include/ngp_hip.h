@@ -129,6 +129,8 @@ typedef struct NerfMlpDescriptor {
     uint32_t n_samples;
     uint32_t n_workspace_blocks;  /* rows of the backward workspace (0 for forward) */
     uint32_t enc_layout;          /* layout of encodings / dL_dencodings (L=16, F=2) */
+    uint32_t param_dtype;         /* NGP_DTYPE_F32, or NGP_DTYPE_F16: f16 weights and f16
+                                     MFMA operands, f32 accumulation and activations */
 } NerfMlpDescriptor;
 
 /* ---------------------------------------------------------------------------
@@ -216,14 +218,14 @@ size_t hashgrid_encode_backward_workspace_size(const char *opaque, size_t opaque
 /* Forward of NeRF.__call__ after the position encoder (models/nerfs.py:66-86).
  * opaque: NerfMlpDescriptor.
  * in : encodings f32[S,32] or f32[16,S,2] (enc_layout), dirs f32[S,3],
- *      params f32[NGP_MLP_N_PARAMS]
+ *      params f32 or f16 [NGP_MLP_N_PARAMS] (param_dtype)
  * out: drgbs f32[S,4]   (density, r, g, b) */
 void nerf_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* Backward of nerf_mlp_forward (JAX autodiff of nerfs.py:66-86 with trunc_exp's
  * custom VJP, nerfs.py:221-238).  opaque: NerfMlpDescriptor.
- * in : encodings (as forward), dirs f32[S,3], params f32[NGP_MLP_N_PARAMS],
- *      dL_ddrgbs f32[S,4]
+ * in : encodings (as forward), dirs f32[S,3], params (as forward),
+ *      dL_ddrgbs f32[S,4] (loss-scaled on the f16 path)
  * out: dL_dencodings (same layout as encodings), dL_dparams f32[NGP_MLP_N_PARAMS],
  *      workspace f32[n_workspace_blocks, NGP_MLP_N_PARAMS] */
 void nerf_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
@@ -257,16 +259,49 @@ typedef struct NgpTrainRaysDescriptor {
  *      noises f32[n], bgs f32[n,3], gt_rgb f32[n,3], pixel_ids u32[n] */
 void train_make_rays(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
+/* Dynamic loss scaling state, kept in device memory so that no step waits for
+ * the host (no counterpart in the reference, which trains in f32 only). */
+typedef struct NgpLossScaleState {
+    float scale;         /* current loss scale (the f16 path starts at 2^15) */
+    uint32_t overflow;   /* set by loss_scale_check when a gradient is not finite */
+    uint32_t good_steps; /* finite steps since the scale last changed */
+    uint32_t reserved;
+} NgpLossScaleState;
+
+#define NGP_LOSS_SCALED 1u /* flags: gradients are multiplied by / divided by state.scale */
+
 typedef struct NgpLossDescriptor {
     uint32_t n_rays;
-    float delta; /* Huber delta (0.1 in app/nerf/_utils.py:154) */
+    float delta;    /* Huber delta (0.1 in app/nerf/_utils.py:154) */
+    uint32_t flags; /* NGP_LOSS_SCALED: dL_dfinal_rgbds is multiplied by state.scale */
 } NgpLossDescriptor;
 
 /* Huber loss averaged over rgb and over valid rays, and its gradient with respect
  * to final_rgbds (app/nerf/_utils.py:149-158; the depth channel gets 0).
  * in : final_rgbds f32[n,4], gt_rgb f32[n,3], ray_is_valid bool[n]
- * out: n_valid_rays u32[1], dL_dfinal_rgbds f32[n,4], loss f32[1] */
+ * out: n_valid_rays u32[1], dL_dfinal_rgbds f32[n,4], loss f32[1] (unscaled)
+ * with NGP_LOSS_SCALED, one more input: NgpLossScaleState[1] */
 void train_loss_grad(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct NgpLossScaleCheckDescriptor {
+    uint64_t n;
+} NgpLossScaleCheckDescriptor;
+
+/* Sets state.overflow if any of the n gradient values is not finite (no
+ * counterpart in the reference).
+ * in : grads f32[n]   inout: NgpLossScaleState[1] */
+void loss_scale_check(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct NgpLossScaleUpdateDescriptor {
+    uint32_t growth_interval; /* finite steps before the scale doubles */
+    float min_scale, max_scale;
+} NgpLossScaleUpdateDescriptor;
+
+/* End of a loss-scaled step: after an overflow the scale halves (not below
+ * min_scale) and the count restarts; otherwise the count grows and, at
+ * growth_interval, the scale doubles (not above max_scale).  Clears overflow.
+ * inout: NgpLossScaleState[1] */
+void loss_scale_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 typedef struct NgpAdamDescriptor {
     uint64_t n;
@@ -275,12 +310,16 @@ typedef struct NgpAdamDescriptor {
     float weight_decay;       /* added to the (already lr-scaled) update: u += wd * p */
     float grad_scale;         /* gradients are multiplied by this first (loss-scale undo) */
     uint32_t mirror_dtype;    /* NGP_DTYPE_F16: also write the updated params as f16 */
+    uint32_t flags;           /* NGP_LOSS_SCALED: gradients are also divided by state.scale,
+                                 and nothing is updated when state.overflow is set */
+    uint32_t reserved;
 } NgpAdamDescriptor;
 
 /* One optax.adam(lr, b1, b2, eps, eps_root) step followed by
  * add_decayed_weights(weight_decay), in that order (app/nerf/_utils.py:19-77).
  * in/out: params f32[n]; in: grads f32[n]; in/out: m f32[n], v f32[n];
- * out: mirror f16[n] (only read when mirror_dtype is NGP_DTYPE_F16; may be NULL) */
+ * out: mirror f16[n] (only read when mirror_dtype is NGP_DTYPE_F16; may be NULL);
+ * with NGP_LOSS_SCALED, one more input after the mirror slot: NgpLossScaleState[1] */
 void adam_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* ---------------------------------------------------------------------------

@@ -37,6 +37,8 @@ TARGETS = (
     "train_make_rays",
     "train_loss_grad",
     "adam_update",
+    "loss_scale_check",
+    "loss_scale_update",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -220,8 +222,8 @@ def hashgrid_descriptor(n_points, L, F, T, first_hash_level, table_dtype, n_rows
                        enc_layout, *sc, *rs, *of)
 
 
-def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0, enc_layout=ENC_POINT_MAJOR) -> bytes:
-    return struct.pack("<3I", n_samples, n_workspace_blocks, enc_layout)
+def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0, enc_layout=ENC_POINT_MAJOR, param_dtype=DTYPE_F32) -> bytes:
+    return struct.pack("<4I", n_samples, n_workspace_blocks, enc_layout, param_dtype)
 
 
 TRAIN_PERTURB = 1
@@ -233,11 +235,27 @@ def train_rays_descriptor(n_rays, n_views, width, height, fx, fy, cx, cy, bound,
                        (seed >> 32) & 0xFFFFFFFF, step & 0xFFFFFFFF, flags, *bg)
 
 
-def loss_descriptor(n_rays, delta) -> bytes:
-    return struct.pack("<If", n_rays, delta)
+LOSS_SCALED = 1
+
+
+def loss_descriptor(n_rays, delta, flags=0) -> bytes:
+    return struct.pack("<IfI", n_rays, delta, flags)
 
 
 def adam_descriptor(n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_decay, grad_scale=1.0,
-                    mirror_dtype=DTYPE_F32) -> bytes:
-    return struct.pack("<Q9fI", n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_decay, grad_scale,
-                       mirror_dtype)
+                    mirror_dtype=DTYPE_F32, flags=0) -> bytes:
+    return struct.pack("<Q9f3I", n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_decay, grad_scale,
+                       mirror_dtype, flags, 0)
+
+
+def loss_scale_check_descriptor(n) -> bytes:
+    return struct.pack("<Q", n)
+
+
+def loss_scale_update_descriptor(growth_interval=2000, min_scale=1.0, max_scale=2.0 ** 24) -> bytes:
+    return struct.pack("<I2f", growth_interval, min_scale, max_scale)
+
+
+def loss_scale_state(device, scale=2.0 ** 15) -> torch.Tensor:
+    """NgpLossScaleState {scale f32, overflow u32, good_steps u32, reserved} as u8[16]."""
+    return torch.tensor(list(struct.pack("<f3I", scale, 0, 0, 0)), dtype=torch.uint8, device=device)

@@ -120,11 +120,13 @@ __global__ void __launch_bounds__(kBlock) count_true_kernel(uint32_t n, const bo
 __global__ void __launch_bounds__(kBlock)
     loss_grad_kernel(uint32_t n, float delta, const float *__restrict__ rgbd, const float *__restrict__ gt,
                      const bool *__restrict__ valid, const uint32_t *__restrict__ n_valid,
-                     float *__restrict__ dL_drgbd, float *__restrict__ loss) {
+                     const NgpLossScaleState *__restrict__ scale_state, float *__restrict__ dL_drgbd,
+                     float *__restrict__ loss) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     float l = 0.0f;
     if (i < n) {
         const float nv = (float)max(*n_valid, 1u);
+        const float ls = scale_state != nullptr ? scale_state->scale : 1.0f;
         float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (valid[i]) {
             float e[3], h = 0.0f;
@@ -136,8 +138,8 @@ __global__ void __launch_bounds__(kBlock)
             }
             l = (h / 3.0f) / nv;
             const float s = 1.0f / (3.0f * nv);
-            g = make_float4(fminf(fmaxf(e[0], -delta), delta) * s, fminf(fmaxf(e[1], -delta), delta) * s,
-                            fminf(fmaxf(e[2], -delta), delta) * s, 0.0f);
+            g = make_float4((fminf(fmaxf(e[0], -delta), delta) * s) * ls, (fminf(fmaxf(e[1], -delta), delta) * s) * ls,
+                            (fminf(fmaxf(e[2], -delta), delta) * s) * ls, 0.0f);
         }
         *reinterpret_cast<float4 *>(dL_drgbd + 4 * (size_t)i) = g;
     }
@@ -150,11 +152,16 @@ __global__ void __launch_bounds__(kBlock)
 template <typename TT>
 __global__ void __launch_bounds__(kBlock)
     adam_kernel(NgpAdamDescriptor d, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                float *__restrict__ v, TT *__restrict__ mirror) {
+                float *__restrict__ v, TT *__restrict__ mirror, const NgpLossScaleState *__restrict__ scale_state) {
     const size_t stride = (size_t)gridDim.x * kBlock;
     const float c1 = 1.0f - d.b1_pow_t, c2 = 1.0f - d.b2_pow_t;
+    float gs = d.grad_scale;
+    if (scale_state != nullptr) {
+        if (scale_state->overflow != 0u) return;  // skipped step: nothing changes
+        gs = gs / scale_state->scale;
+    }
     for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < d.n; k += stride) {
-        const float gk = g[k] * d.grad_scale;
+        const float gk = g[k] * gs;
         const float mk = d.b1 * m[k] + (1.0f - d.b1) * gk;
         const float vk = d.b2 * v[k] + (1.0f - d.b2) * (gk * gk);
         m[k] = mk;
@@ -165,6 +172,28 @@ __global__ void __launch_bounds__(kBlock)
         p[k] = pk;
         if (mirror != nullptr) mirror[k] = (TT)pk;
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    loss_scale_check_kernel(uint64_t n, const float *__restrict__ g, NgpLossScaleState *__restrict__ state) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    bool bad = false;
+    for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += stride) bad |= !isfinite(g[k]);
+    if (__ballot(bad) != 0ull && lane_id() == 0) atomicOr(&state->overflow, 1u);
+}
+
+__global__ void loss_scale_update_kernel(NgpLossScaleUpdateDescriptor d, NgpLossScaleState *__restrict__ state) {
+    if (threadIdx.x != 0) return;
+    NgpLossScaleState s = *state;
+    if (s.overflow != 0u) {
+        s.scale = fmaxf(s.scale * 0.5f, d.min_scale);
+        s.good_steps = 0;
+    } else if (++s.good_steps >= d.growth_interval) {
+        s.scale = fminf(s.scale * 2.0f, d.max_scale);
+        s.good_steps = 0;
+    }
+    s.overflow = 0;
+    *state = s;
 }
 
 }  // namespace
@@ -193,8 +222,14 @@ extern "C" void train_loss_grad(ngp_stream_t stream_, void **buffers, const char
     const char *target = "train_loss_grad";
     NgpLossDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.flags & ~NGP_LOSS_SCALED) {
+        set_error(kBadDescriptor, "%s: unknown flags %u", target, d.flags);
+        return;
+    }
+    const bool scaled = (d.flags & NGP_LOSS_SCALED) != 0;
     if (!check_buffer_list(buffers, {3, 5}, target)) return;
     if (d.n_rays > 0 && !check_buffers(buffers, 6, target)) return;
+    if (scaled && !check_buffer_list(buffers, {6}, target)) return;
     hipStream_t stream = (hipStream_t)stream_;
     uint32_t *n_valid = (uint32_t *)buffers[3];
     float *loss = (float *)buffers[5];
@@ -205,7 +240,8 @@ extern "C" void train_loss_grad(ngp_stream_t stream_, void **buffers, const char
                            (const bool *)buffers[2], n_valid);
         hipLaunchKernelGGL(loss_grad_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
                            d.delta, (const float *)buffers[0], (const float *)buffers[1], (const bool *)buffers[2],
-                           n_valid, (float *)buffers[4], loss);
+                           n_valid, scaled ? (const NgpLossScaleState *)buffers[6] : nullptr, (float *)buffers[4],
+                           loss);
     }
     check_launch(target);
 }
@@ -214,17 +250,51 @@ extern "C" void adam_update(ngp_stream_t stream, void **buffers, const char *opa
     const char *target = "adam_update";
     NgpAdamDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.flags & ~NGP_LOSS_SCALED) {
+        set_error(kBadDescriptor, "%s: unknown flags %u", target, d.flags);
+        return;
+    }
     if (d.n == 0) return;
     if (!check_buffers(buffers, 4, target)) return;
+    const bool scaled = (d.flags & NGP_LOSS_SCALED) != 0;
+    if (scaled && !check_buffer_list(buffers, {5}, target)) return;
+    const NgpLossScaleState *state = scaled ? (const NgpLossScaleState *)buffers[5] : nullptr;
     const uint32_t grid = (uint32_t)std::min<uint64_t>(div_up(d.n, kBlock), 8192);
     if (d.mirror_dtype == NGP_DTYPE_F16 && buffers[4] != nullptr) {
         hipLaunchKernelGGL(adam_kernel<__half>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, d,
                            (float *)buffers[0], (const float *)buffers[1], (float *)buffers[2], (float *)buffers[3],
-                           (__half *)buffers[4]);
+                           (__half *)buffers[4], state);
     } else {
         hipLaunchKernelGGL(adam_kernel<float>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, d,
                            (float *)buffers[0], (const float *)buffers[1], (float *)buffers[2], (float *)buffers[3],
-                           (float *)nullptr);
+                           (float *)nullptr, state);
     }
+    check_launch(target);
+}
+
+extern "C" void loss_scale_check(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "loss_scale_check";
+    NgpLossScaleCheckDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (!check_buffer_list(buffers, {1}, target)) return;
+    if (d.n == 0) return;
+    if (!check_buffers(buffers, 2, target)) return;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(div_up(d.n, kBlock), 4096);
+    hipLaunchKernelGGL(loss_scale_check_kernel, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, d.n,
+                       (const float *)buffers[0], (NgpLossScaleState *)buffers[1]);
+    check_launch(target);
+}
+
+extern "C" void loss_scale_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "loss_scale_update";
+    NgpLossScaleUpdateDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.growth_interval == 0 || !(d.min_scale > 0.0f) || !(d.max_scale >= d.min_scale)) {
+        set_error(kBadDescriptor, "%s: need growth_interval > 0 and 0 < min_scale <= max_scale", target);
+        return;
+    }
+    if (!check_buffers(buffers, 1, target)) return;
+    hipLaunchKernelGGL(loss_scale_update_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, d,
+                       (NgpLossScaleState *)buffers[0]);
     check_launch(target);
 }

@@ -400,6 +400,317 @@ __global__ void __launch_bounds__(kSumWaves * kWave)
     }
 }
 
+// ---------------------------------------------------------------------------
+// f16 weights (the precision path): the same network with f16 weights and f16
+// MFMA operands, f32 accumulation (v_mfma_f32_16x16x32_f16 for the layers,
+// v_mfma_f32_16x16x16_f16 for the weight gradients), f32 activations between the
+// products (ReLU and masks in f32, rounded to f16 only as MFMA inputs), f32
+// density/sigmoid heads and f32 gradients out.  Cotangents arrive loss-scaled.
+//
+// K mapping.  A 16x16x32 product takes, on lane (i, g) = (lane % 16, lane / 16),
+// eight K slots q = 0..7 of A's row i and B's column i.  Here slot q of lane group
+// g stands for feature 32 kb + 4g + q (q < 4) or 32 kb + 16 + 4g + (q - 4)
+// (q >= 4): exactly the registers lane (i, g) holds of two consecutive MFMA output
+// tiles, so a layer's outputs are the next layer's B operand as they are, and the
+// weights are read from LDS with the same mapping (two 8-byte runs per lane).
+// ---------------------------------------------------------------------------
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+// forward copies W^T [out][in + 8], backward copies W [in][out_pad + 8] (halves)
+constexpr int FT1 = 40, FT2 = 72, FT3 = 40, FT4 = 72, FT5 = 72;
+constexpr int H_F1 = 0;
+constexpr int H_F2 = H_F1 + 64 * FT1;
+constexpr int H_F3 = H_F2 + 16 * FT2;
+constexpr int H_F4 = H_F3 + 64 * FT3;
+constexpr int H_F5 = H_F4 + 64 * FT4;
+constexpr int H_FEND = H_F5 + 16 * FT5;
+constexpr int BT1 = 72, BT2 = 24, BT3 = 72, BT4 = 72, BT5 = 24;
+constexpr int H_B1 = H_FEND;
+constexpr int H_B2 = H_B1 + 32 * BT1;
+constexpr int H_B3 = H_B2 + 64 * BT2;
+constexpr int H_B4 = H_B3 + 32 * BT3;
+constexpr int H_B5 = H_B4 + 64 * BT4;
+constexpr int H_END = H_B5 + 64 * BT5;  // 24320 halves
+
+__device__ __forceinline__ v4f mfma_h32(h8 a, h8 b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_h16(h4 a, h4 b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ h8 pack_h8(v4f lo, v4f hi) {
+    return h8{(_Float16)lo[0], (_Float16)lo[1], (_Float16)lo[2], (_Float16)lo[3],
+              (_Float16)hi[0], (_Float16)hi[1], (_Float16)hi[2], (_Float16)hi[3]};
+}
+
+__device__ __forceinline__ h8 join_h4(h4 lo, h4 hi) {
+    return h8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
+// Copies the flat f16 parameters into the forward (transposed) and, with
+// kBackward, the backward LDS layouts; padding is zero.
+template <bool kBackward>
+__device__ void stage_weights_h(_Float16 *lds, const _Float16 *__restrict__ params) {
+    const int end = kBackward ? H_END : H_FEND;
+    for (int t = threadIdx.x; t < end; t += kThreads) lds[t] = (_Float16)0.0f;
+    __syncthreads();
+    // W1 [32][64], W2 [64][16], W3 [32][64], W4 [64][64], W5 [64][3] (flax [in][out])
+    for (int t = threadIdx.x; t < 32 * 64; t += kThreads) {
+        const int in = t / 64, out = t % 64;
+        lds[H_F1 + out * FT1 + in] = params[P_W1 + t];
+        lds[H_F3 + out * FT3 + in] = params[P_W3 + t];
+        if (kBackward) {
+            lds[H_B1 + in * BT1 + out] = params[P_W1 + t];
+            lds[H_B3 + in * BT3 + out] = params[P_W3 + t];
+        }
+    }
+    for (int t = threadIdx.x; t < 64 * 16; t += kThreads) {
+        const int in = t / 16, out = t % 16;
+        lds[H_F2 + out * FT2 + in] = params[P_W2 + t];
+        if (kBackward) lds[H_B2 + in * BT2 + out] = params[P_W2 + t];
+    }
+    for (int t = threadIdx.x; t < 64 * 64; t += kThreads) {
+        const int in = t / 64, out = t % 64;
+        lds[H_F4 + out * FT4 + in] = params[P_W4 + t];
+        if (kBackward) lds[H_B4 + in * BT4 + out] = params[P_W4 + t];
+    }
+    for (int t = threadIdx.x; t < 64 * 3; t += kThreads) {
+        const int in = t / 3, out = t % 3;
+        lds[H_F5 + out * FT5 + in] = params[P_W5 + t];
+        if (kBackward) lds[H_B5 + in * BT5 + out] = params[P_W5 + t];
+    }
+    __syncthreads();
+}
+
+// Y^T = W^T Z^T with KB K-slices of 32 inputs and OB output tiles of 16.
+template <int KB, int OB, int ST>
+__device__ __forceinline__ void layer_fwd_h(const _Float16 *WT, const h8 *in, v4f *out, int i, int g) {
+#pragma unroll
+    for (int ob = 0; ob < OB; ++ob) {
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            const _Float16 *row = WT + (16 * ob + i) * ST + 32 * kb + 4 * g;
+            const h8 a = join_h4(*reinterpret_cast<const h4 *>(row), *reinterpret_cast<const h4 *>(row + 16));
+            acc = mfma_h32(a, in[kb], acc);
+        }
+        out[ob] = acc;
+    }
+}
+
+// dZ^T = W dY^T with IB input tiles and KB K-slices of 32 outputs; kHalfK: the
+// layer has 16 (or fewer) outputs, the upper half of the slice is zero.
+template <int IB, int KB, int ST, bool kHalfK>
+__device__ __forceinline__ void layer_bwd_h(const _Float16 *W, const h8 *dout, v4f *din, int i, int g) {
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib) {
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            const _Float16 *row = W + (16 * ib + i) * ST + 32 * kb + 4 * g;
+            const h4 lo = *reinterpret_cast<const h4 *>(row);
+            const h4 hi = kHalfK ? h4{0, 0, 0, 0} : *reinterpret_cast<const h4 *>(row + 16);
+            acc = mfma_h32(join_h4(lo, hi), dout[kb], acc);
+        }
+        din[ib] = acc;
+    }
+}
+
+struct FwdH {
+    v4f x[2], h1[4], c[2], h2[4], h3[4], rr;  // f32 values; c[0] = o, c[1] = SH
+};
+
+__device__ __forceinline__ void forward_tile_h(const _Float16 *lds, const float *__restrict__ enc, bool level_major,
+                                               const float *__restrict__ dirs, uint32_t n, size_t s0, int i, int g,
+                                               FwdH &f) {
+    const size_t s = s0 + i;
+    const bool live = s < n;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) f.x[b] = live ? load_enc4(enc, level_major, n, s, b, g) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const h8 xin = pack_h8(f.x[0], f.x[1]);
+    layer_fwd_h<1, 4, FT1>(lds + H_F1, &xin, f.h1, i, g);
+    relu<4>(f.h1);
+    const h8 h1in[2] = {pack_h8(f.h1[0], f.h1[1]), pack_h8(f.h1[2], f.h1[3])};
+    layer_fwd_h<2, 1, FT2>(lds + H_F2, h1in, &f.c[0], i, g);
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    if (live) {
+        dx = dirs[3 * s + 0];
+        dy = dirs[3 * s + 1];
+        dz = dirs[3 * s + 2];
+    }
+    f.c[1] = sh4_part(dx, dy, dz, g);
+    const h8 cin = pack_h8(f.c[0], f.c[1]);
+    layer_fwd_h<1, 4, FT3>(lds + H_F3, &cin, f.h2, i, g);
+    relu<4>(f.h2);
+    const h8 h2in[2] = {pack_h8(f.h2[0], f.h2[1]), pack_h8(f.h2[2], f.h2[3])};
+    layer_fwd_h<2, 4, FT4>(lds + H_F4, h2in, f.h3, i, g);
+    relu<4>(f.h3);
+    const h8 h3in[2] = {pack_h8(f.h3[0], f.h3[1]), pack_h8(f.h3[2], f.h3[3])};
+    layer_fwd_h<2, 1, FT5>(lds + H_F5, h3in, &f.rr, i, g);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    mlp_forward_h_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
+                         const _Float16 *__restrict__ params, float *__restrict__ drgbs) {
+    __shared__ __align__(16) _Float16 lds[H_FEND];
+    stage_weights_h<false>(lds, params);
+    const int lane = lane_id();
+    const int i = lane & 15, g = lane >> 4;
+    const uint32_t n_tiles = (n + kTile - 1) / kTile;
+    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / kWave;
+    const uint32_t n_waves = gridDim.x * kWaves;
+    for (uint32_t t = wave; t < n_tiles; t += n_waves) {
+        const size_t s0 = (size_t)t * kTile;
+        FwdH f;
+        forward_tile_h(lds, enc, level_major, dirs, n, s0, i, g, f);
+        const size_t s = s0 + i;
+        if (g == 0 && s < n) {
+            const float sigma = det_expf(f.c[0][0]);
+            v4f out = {sigma, sigmoidf(f.rr[0]), sigmoidf(f.rr[1]), sigmoidf(f.rr[2])};
+            *reinterpret_cast<v4f *>(drgbs + 4 * s) = out;
+        }
+    }
+}
+
+// acc[ib*OB + ob] += Z^T dY over the tile's 16 samples (K = samples), f16 operands:
+// lane (i, g) takes samples 4g..4g+3 of feature 16ib + i (A) and 16ob + i (B).
+template <int IB, int OB>
+__device__ __forceinline__ void dw_accumulate_h(v4f *acc, const float *zs, const float *ys, int i, int g) {
+    h4 a[IB], bb[OB];
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[ib][q] = (_Float16)zs[(4 * g + q) * S_T + 16 * ib + i];
+#pragma unroll
+    for (int ob = 0; ob < OB; ++ob)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bb[ob][q] = (_Float16)ys[(4 * g + q) * S_T + 16 * ob + i];
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib)
+#pragma unroll
+        for (int ob = 0; ob < OB; ++ob) acc[ib * OB + ob] = mfma_h16(a[ib], bb[ob], acc[ib * OB + ob]);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    mlp_backward_h_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
+                          const _Float16 *__restrict__ params, const float *__restrict__ dL_ddrgbs,
+                          float *__restrict__ dL_denc, float *__restrict__ workspace) {
+    __shared__ __align__(16) _Float16 lds[H_END];
+    __shared__ float scratch[kWaves * 2 * T_FLOATS];
+    stage_weights_h<true>(lds, params);
+    const int lane = lane_id();
+    const int i = lane & 15, g = lane >> 4;
+    const int wv = threadIdx.x / kWave;
+    float *zs = scratch + wv * 2 * T_FLOATS;
+    float *ys = zs + T_FLOATS;
+    const uint32_t n_tiles = (n + kTile - 1) / kTile;
+    const uint32_t wave = blockIdx.x * kWaves + wv;
+    const uint32_t n_waves = gridDim.x * kWaves;
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+
+    v4f gW1[8], gW2[4], gW3[8], gW4[16], gW5[4];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gW1[k] = gW3[k] = zero;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gW2[k] = gW5[k] = zero;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) gW4[k] = zero;
+
+    for (uint32_t t = wave; t < n_tiles; t += n_waves) {
+        const size_t s0 = (size_t)t * kTile;
+        const size_t s = s0 + i;
+        const bool live = s < n;
+        FwdH f;
+        forward_tile_h(lds, enc, level_major, dirs, n, s0, i, g, f);
+        const v4f gin = live ? *reinterpret_cast<const v4f *>(dL_ddrgbs + 4 * s) : zero;
+        v4f dr = zero;
+        float dsig = 0.0f;
+        if (g == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float y = sigmoidf(f.rr[c]);
+                dr[c] = gin[c + 1] * y * (1.0f - y);
+            }
+            dsig = gin[0] * det_expf(fminf(fmaxf(f.c[0][0], -15.0f), 15.0f));
+        }
+        // W5
+        tile_to_lds<4>(zs, f.h3, i, g);
+        tile_to_lds<1>(ys, &dr, i, g);
+        wave_lds_sync();
+        dw_accumulate_h<4, 1>(gW5, zs, ys, i, g);
+        const h8 drin = pack_h8(dr, zero);
+        v4f d3[4];
+        layer_bwd_h<4, 1, BT5, true>(lds + H_B5, &drin, d3, i, g);
+        relu_mask<4>(d3, f.h3);
+        wave_lds_sync();
+        // W4
+        tile_to_lds<4>(zs, f.h2, i, g);
+        tile_to_lds<4>(ys, d3, i, g);
+        wave_lds_sync();
+        dw_accumulate_h<4, 4>(gW4, zs, ys, i, g);
+        const h8 d3in[2] = {pack_h8(d3[0], d3[1]), pack_h8(d3[2], d3[3])};
+        v4f d2[4];
+        layer_bwd_h<4, 2, BT4, false>(lds + H_B4, d3in, d2, i, g);
+        relu_mask<4>(d2, f.h2);
+        wave_lds_sync();
+        // W3
+        tile_to_lds<2>(zs, f.c, i, g);
+        tile_to_lds<4>(ys, d2, i, g);
+        wave_lds_sync();
+        dw_accumulate_h<2, 4>(gW3, zs, ys, i, g);
+        const h8 d2in[2] = {pack_h8(d2[0], d2[1]), pack_h8(d2[2], d2[3])};
+        v4f dc[2];
+        layer_bwd_h<2, 2, BT3, false>(lds + H_B3, d2in, dc, i, g);
+        if (g == 0) dc[0][0] += dsig;
+        wave_lds_sync();
+        // W2
+        tile_to_lds<4>(zs, f.h1, i, g);
+        tile_to_lds<1>(ys, &dc[0], i, g);
+        wave_lds_sync();
+        dw_accumulate_h<4, 1>(gW2, zs, ys, i, g);
+        const h8 dcin = pack_h8(dc[0], zero);
+        v4f d1[4];
+        layer_bwd_h<4, 1, BT2, true>(lds + H_B2, &dcin, d1, i, g);
+        relu_mask<4>(d1, f.h1);
+        wave_lds_sync();
+        // W1
+        tile_to_lds<2>(zs, f.x, i, g);
+        tile_to_lds<4>(ys, d1, i, g);
+        wave_lds_sync();
+        dw_accumulate_h<2, 4>(gW1, zs, ys, i, g);
+        const h8 d1in[2] = {pack_h8(d1[0], d1[1]), pack_h8(d1[2], d1[3])};
+        v4f dx[2];
+        layer_bwd_h<2, 2, BT1, false>(lds + H_B1, d1in, dx, i, g);
+        if (live) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) store_enc4(dL_denc, level_major, n, s, b, g, dx[b]);
+        }
+        wave_lds_sync();
+    }
+
+    __syncthreads();
+    float *red = reinterpret_cast<float *>(lds);  // 9408 floats fit in the 48,640 bytes
+    for (int t = threadIdx.x; t < P_END; t += kThreads) red[t] = 0.0f;
+    __syncthreads();
+    for (int w = 0; w < kWaves; ++w) {
+        if (wv == w) {
+            dw_to_lds<2, 4>(red + P_W1, gW1, 64, i, g);
+            dw_to_lds<4, 1>(red + P_W2, gW2, 16, i, g);
+            dw_to_lds<2, 4>(red + P_W3, gW3, 64, i, g);
+            dw_to_lds<4, 4>(red + P_W4, gW4, 64, i, g);
+            dw_to_lds<4, 1>(red + P_W5, gW5, 3, i, g);
+        }
+        __syncthreads();
+    }
+    float *row = workspace + (size_t)blockIdx.x * P_END;
+    for (int t = threadIdx.x; t < P_END; t += kThreads) row[t] = red[t];
+}
+
+static_assert(H_END * 2 >= P_END * 4, "the weight area holds the gradient reduction");
+
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 uint32_t forward_grid(uint32_t n) {
@@ -416,8 +727,9 @@ extern "C" void nerf_mlp_forward(ngp_stream_t stream_, void **buffers, const cha
     const char *target = "nerf_mlp_forward";
     NerfMlpDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
-    if (d.enc_layout > NGP_ENC_LEVEL_MAJOR) {
-        set_error(kBadDescriptor, "%s: enc_layout=%u must be 0 or 1", target, d.enc_layout);
+    if (d.enc_layout > NGP_ENC_LEVEL_MAJOR || d.param_dtype > NGP_DTYPE_F16) {
+        set_error(kBadDescriptor, "%s: enc_layout=%u and param_dtype=%u must be 0 or 1", target, d.enc_layout,
+                  d.param_dtype);
         return;
     }
     if (d.n_samples == 0) return;
@@ -426,9 +738,16 @@ extern "C" void nerf_mlp_forward(ngp_stream_t stream_, void **buffers, const cha
         set_error(kBadArgument, "%s: encodings and drgbs must be 16-byte aligned", target);
         return;
     }
-    hipLaunchKernelGGL(mlp_forward_kernel, dim3(forward_grid(d.n_samples)), dim3(kThreads), 0, (hipStream_t)stream_,
-                       d.n_samples, d.enc_layout == NGP_ENC_LEVEL_MAJOR, (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
-                       (float *)buffers[3]);
+    const bool lm = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
+    if (d.param_dtype == NGP_DTYPE_F16) {
+        hipLaunchKernelGGL(mlp_forward_h_kernel, dim3(forward_grid(d.n_samples)), dim3(kThreads), 0,
+                           (hipStream_t)stream_, d.n_samples, lm, (const float *)buffers[0], (const float *)buffers[1],
+                           (const _Float16 *)buffers[2], (float *)buffers[3]);
+    } else {
+        hipLaunchKernelGGL(mlp_forward_kernel, dim3(forward_grid(d.n_samples)), dim3(kThreads), 0, (hipStream_t)stream_,
+                           d.n_samples, lm, (const float *)buffers[0], (const float *)buffers[1],
+                           (const float *)buffers[2], (float *)buffers[3]);
+    }
     check_launch(target);
 }
 
@@ -436,8 +755,9 @@ extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const ch
     const char *target = "nerf_mlp_backward";
     NerfMlpDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
-    if (d.n_workspace_blocks == 0 || d.enc_layout > NGP_ENC_LEVEL_MAJOR) {
-        set_error(kBadDescriptor, "%s: n_workspace_blocks must be positive and enc_layout 0 or 1", target);
+    if (d.n_workspace_blocks == 0 || d.enc_layout > NGP_ENC_LEVEL_MAJOR || d.param_dtype > NGP_DTYPE_F16) {
+        set_error(kBadDescriptor, "%s: n_workspace_blocks must be positive, enc_layout and param_dtype 0 or 1",
+                  target);
         return;
     }
     if (!check_buffer_list(buffers, {2, 5, 6}, target)) return;
@@ -447,9 +767,16 @@ extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const ch
         return;
     }
     hipStream_t stream = (hipStream_t)stream_;
-    hipLaunchKernelGGL(mlp_backward_kernel, dim3(d.n_workspace_blocks), dim3(kThreads), 0, stream, d.n_samples,
-                       d.enc_layout == NGP_ENC_LEVEL_MAJOR, (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
-                       (const float *)buffers[3], (float *)buffers[4], (float *)buffers[6]);
+    const bool lm = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
+    if (d.param_dtype == NGP_DTYPE_F16) {
+        hipLaunchKernelGGL(mlp_backward_h_kernel, dim3(d.n_workspace_blocks), dim3(kThreads), 0, stream, d.n_samples,
+                           lm, (const float *)buffers[0], (const float *)buffers[1], (const _Float16 *)buffers[2],
+                           (const float *)buffers[3], (float *)buffers[4], (float *)buffers[6]);
+    } else {
+        hipLaunchKernelGGL(mlp_backward_kernel, dim3(d.n_workspace_blocks), dim3(kThreads), 0, stream, d.n_samples, lm,
+                           (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
+                           (const float *)buffers[3], (float *)buffers[4], (float *)buffers[6]);
+    }
     if (!check_launch(target)) return;
     hipLaunchKernelGGL(sum_rows_kernel, dim3(div_up(P_END, kWave)), dim3(kSumWaves * kWave), 0, stream,
                        d.n_workspace_blocks,

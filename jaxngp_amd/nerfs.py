@@ -107,6 +107,9 @@ class NeRF(torch.nn.Module):
                                                 tv_scale=0.0, param_dtype=table_dtype, generator=gen)
         flat = glorot_uniform_(torch.empty(N_PARAMS, dtype=torch.float32), generator=gen)
         self.mlp_params = torch.nn.Parameter(flat)
+        # optional f16 copy of the MLP weights (the f16 precision path); when set,
+        # queries without autograd run the f16-weight kernels on it
+        self.mlp_half: torch.Tensor | None = None
         if device is not None:
             self.to(device)
 
@@ -119,6 +122,11 @@ class NeRF(torch.nn.Module):
     def forward(self, xyz: torch.Tensor, dir: torch.Tensor | None, appearance_embeddings=None):
         shape = xyz.shape[:-1]
         xyz = xyz.reshape(-1, 3)
+        if not torch.is_grad_enabled():
+            out = self._query(xyz.contiguous(), None if dir is None else dir.reshape(-1, 3).contiguous())
+            if dir is None:
+                return out[:, :1].reshape(*shape, 1), 0.0
+            return out.reshape(*shape, 4), 0.0
         enc, tv = self.position_encoder(xyz, self.bound)
         if dir is None:
             d = torch.zeros_like(xyz)
@@ -126,3 +134,23 @@ class NeRF(torch.nn.Module):
             return density.reshape(*shape, 1), tv
         drgbs = nerf_mlp(enc, dir.reshape(-1, 3), self.mlp_params)
         return drgbs.reshape(*shape, 4), tv
+
+    def _query(self, xyz: torch.Tensor, dirs: torch.Tensor | None) -> torch.Tensor:
+        """Forward without autograd on the level-major encodings [L, N, F], the layout
+        the XCD-affine encoder and the MLP's coalesced loads use."""
+        N = xyz.shape[0]
+        out = torch.empty(N, 4, dtype=torch.float32, device=xyz.device)
+        if N == 0:
+            return out
+        if xyz.dtype != torch.float32:
+            raise NotImplementedError(f"positions must be float32, got {xyz.dtype}")
+        pe = self.position_encoder
+        enc = torch.empty(pe.L, N, pe.F, dtype=torch.float32, device=xyz.device)
+        _lib.call("hashgrid_encode", [xyz, pe.table.detach(), enc], pe.descriptor(N, self.bound, _lib.ENC_LEVEL_MAJOR))
+        d = torch.zeros_like(xyz) if dirs is None else dirs
+        if self.mlp_half is not None:
+            w, pd = self.mlp_half, _lib.DTYPE_F16
+        else:
+            w, pd = self.mlp_params.detach(), _lib.DTYPE_F32
+        _lib.call("nerf_mlp_forward", [enc, d, w, out], _lib.nerf_mlp_descriptor(N, 0, _lib.ENC_LEVEL_MAJOR, pd))
+        return out

@@ -42,6 +42,12 @@ class TrainConfig:
     bg: tuple = (1.0, 1.0, 1.0)
     near: float = 0.3
     seed: int = 1_000_000_007  # utils/args.py:18-27
+    # "fp32": the reference's precision.  "fp16": f16 hash table and f16 MLP weights
+    # on MFMA (f32 accumulation), f32 master weights, f32 transmittance, dynamic
+    # loss scaling from 2^15 (halve on overflow, double after 2000 finite steps)
+    precision: str = "fp32"
+    loss_scale_init: float = 2.0 ** 15
+    loss_scale_growth_interval: int = 2000
 
 
 def lr_at(step: int, lr: float) -> float:
@@ -91,6 +97,11 @@ class Trainer:
         self.bound = float(data.scene.bound)
         self.K = max(1, int(1 + math.ceil(math.log2(self.bound))))  # SceneMeta.cascades, types.py:858-860
         self.portion = 0.0 if self.bound <= 1 else 1 / 256  # stepsize_portion (types.py:874-886), bound in (1, 4)
+        if cfg.precision not in ("fp32", "fp16"):
+            raise ValueError(f"precision must be 'fp32' or 'fp16', got {cfg.precision!r}")
+        self.fp16 = cfg.precision == "fp16"
+        if self.fp16:
+            table_dtype = torch.float16
         self.model = NeRF(bound=self.bound, device=self.dev, seed=cfg.seed % 2 ** 31, table_dtype=table_dtype)
         self.grid = OccupancyGrid(self.K, cfg.grid_res, self.dev)
         self.step_count = 0
@@ -116,6 +127,10 @@ class Trainer:
         self.table_master = table.detach().float().clone()
         self.opt = dict(tm=torch.zeros_like(self.table_master), tv=torch.zeros_like(self.table_master),
                         mm=torch.zeros(N_PARAMS, device=self.dev), mv=torch.zeros(N_PARAMS, device=self.dev))
+        if self.fp16:
+            self.model.mlp_half = self.model.mlp_params.detach().half()
+            self.loss_state = _lib.loss_scale_state(self.dev, cfg.loss_scale_init)
+        self._calls = None
         self.mark_untrained_density_grid()
 
     # -- occupancy grid -------------------------------------------------------------------
@@ -178,11 +193,15 @@ class Trainer:
             M = max(1, n_grids // 2)
             h = max(1, M // 2)
             first = aligned[torch.randint(0, n_grids, (h,), device=self.dev, generator=self.rng)]
-            occ = g.occ_mask[sl][aligned].float()
-            if bool(occ.sum() > 0):
-                second = aligned[torch.multinomial(occ, h, replacement=True, generator=self.rng)]
-            else:
-                second = aligned[torch.randint(0, n_grids, (h,), device=self.dev, generator=self.rng)]
+            # uniform choice among occupied cells (sampling proportional to the 0/1
+            # occupancy, as types.py:1180-1190), uniform among all when none is
+            # occupied; on the device, without a host sync
+            cs = torch.cumsum(g.occ_mask[sl][aligned].to(torch.int64), 0)
+            total = cs[-1]
+            u = torch.rand(h, device=self.dev, generator=self.rng, dtype=torch.float64)
+            k_occ = torch.searchsorted(cs, (u * total).floor().to(torch.int64), right=True).clamp_max(n_grids - 1)
+            k_any = (u * n_grids).floor().to(torch.int64).clamp_max(n_grids - 1)
+            second = aligned[torch.where(total > 0, k_occ, k_any)]
             upd = torch.cat([first, second])
         mip = min(self.bound, 2 ** cas)
         hcw = mip / G
@@ -217,50 +236,96 @@ class Trainer:
 
     # -- one step -----------------------------------------------------------------------
 
-    def step(self) -> None:
-        """One training step (app/nerf/_utils.py:80-170), then the grid cadence."""
+    def _prepare(self) -> dict:
+        """Bound calls of one training step (buffers and static descriptors fixed)."""
         c, b, m = self.cfg, self.b, self.model
         enc_mod = m.position_encoder
-        table = enc_mod.table
+        table = enc_mod.table.detach()
+        params = m.mlp_params.detach()
         S, n = c.total_samples, self.n_rays
-        cam, data = self.data.camera, self.data
-        flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg else 0)
-        _lib.call("train_make_rays", [data.transforms, data.rgba, b["o"], b["d"], b["t0"], b["t1"], b["noise"],
-                                      b["bgs"], b["gt"], b["pix"]],
-                  _lib.train_rays_descriptor(n, data.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy,
-                                             self.bound, c.seed, self.step_count, flags, c.bg))
-        _lib.call("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield, b["next_loc"],
-                                 b["exceeded"], b["valid"], b["ns"], b["start"], b["idcs"], b["xyzs"], b["dirs"],
-                                 b["dss"], b["zs"]],
-                  _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion))
+        data = self.data
         LM = _lib.ENC_LEVEL_MAJOR
         hdesc = enc_mod.descriptor(S, self.bound, LM)
-        _lib.call("hashgrid_encode", [b["xyzs"], table.detach(), b["enc"]], hdesc)
-        params = m.mlp_params.detach()
-        _lib.call("nerf_mlp_forward", [b["enc"], b["dirs"], params, b["drgbs"]], _lib.nerf_mlp_descriptor(S, 0, LM))
-        _lib.call("integrate_rays", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"], b["measured"],
-                                     b["rgbd"], b["op"]], _lib.integrating_descriptor(n, S))
-        _lib.call("train_loss_grad", [b["rgbd"], b["gt"], b["valid"], b["n_valid"], b["g_rgbd"], b["loss"]],
-                  _lib.loss_descriptor(n, 0.1))
-        _lib.call("integrate_rays_backward", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"],
-                                              b["rgbd"], b["op"], b["g_rgbd"], b["dbgs"], b["dz"], b["ddrgbs"]],
-                  _lib.integrating_backward_descriptor(n, S, c.near))
-        _lib.call("nerf_mlp_backward", [b["enc"], b["dirs"], params, b["ddrgbs"], b["denc"], b["dparams"], b["ws"]],
-                  _lib.nerf_mlp_descriptor(S, b["ws"].shape[0], LM))
-        _lib.call("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"], b["hg_ws"]], hdesc)
+        P = _lib.Prepared
+        o = self.opt
+        mirror = table if table.dtype == torch.float16 else None
+        tab_master = table if mirror is None else self.table_master
+        if self.fp16:
+            mlp_w, pd, flags = m.mlp_half, _lib.DTYPE_F16, _lib.LOSS_SCALED
+            state = [self.loss_state]
+        else:
+            mlp_w, pd, flags, state = params, _lib.DTYPE_F32, 0, []
+        none = torch.empty(0, device=self.dev)  # a NULL mirror slot
+        calls = dict(
+            rays=P("train_make_rays", [data.transforms, data.rgba, b["o"], b["d"], b["t0"], b["t1"], b["noise"],
+                                       b["bgs"], b["gt"], b["pix"]], b""),
+            march=P("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield, b["next_loc"],
+                                   b["exceeded"], b["valid"], b["ns"], b["start"], b["idcs"], b["xyzs"], b["dirs"],
+                                   b["dss"], b["zs"]],
+                    _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion)),
+            encode=P("hashgrid_encode", [b["xyzs"], table, b["enc"]], hdesc),
+            mlp=P("nerf_mlp_forward", [b["enc"], b["dirs"], mlp_w, b["drgbs"]], _lib.nerf_mlp_descriptor(S, 0, LM, pd)),
+            integrate=P("integrate_rays", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"], b["measured"],
+                                           b["rgbd"], b["op"]], _lib.integrating_descriptor(n, S)),
+            loss=P("train_loss_grad", [b["rgbd"], b["gt"], b["valid"], b["n_valid"], b["g_rgbd"], b["loss"]] + state,
+                   _lib.loss_descriptor(n, 0.1, flags)),
+            integrate_bwd=P("integrate_rays_backward", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"],
+                                                        b["rgbd"], b["op"], b["g_rgbd"], b["dbgs"], b["dz"],
+                                                        b["ddrgbs"]],
+                            _lib.integrating_backward_descriptor(n, S, c.near)),
+            mlp_bwd=P("nerf_mlp_backward", [b["enc"], b["dirs"], mlp_w, b["ddrgbs"], b["denc"], b["dparams"],
+                                            b["ws"]], _lib.nerf_mlp_descriptor(S, b["ws"].shape[0], LM, pd)),
+            encode_bwd=P("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"], b["hg_ws"]], hdesc),
+            adam_table=P("adam_update", [tab_master, b["dtable"], o["tm"], o["tv"]] +
+                         [mirror if mirror is not None else none] + state, b""),
+            adam_mlp=P("adam_update", [params, b["dparams"], o["mm"], o["mv"]] +
+                       [m.mlp_half if self.fp16 else none] + state, b""),
+            mirror=mirror is not None,
+            flags=flags,
+        )
+        if self.fp16:
+            calls["check_table"] = P("loss_scale_check", [b["dtable"], self.loss_state],
+                                     _lib.loss_scale_check_descriptor(b["dtable"].numel()))
+            calls["check_mlp"] = P("loss_scale_check", [b["dparams"], self.loss_state],
+                                   _lib.loss_scale_check_descriptor(N_PARAMS))
+            calls["scale_update"] = P("loss_scale_update", [self.loss_state],
+                                      _lib.loss_scale_update_descriptor(c.loss_scale_growth_interval))
+        return calls
+
+    def step(self) -> None:
+        """One training step (app/nerf/_utils.py:80-170), then the grid cadence.
+        Kernel order: train_make_rays -> march_rays -> hashgrid_encode ->
+        nerf_mlp_forward -> integrate_rays -> train_loss_grad -> integrate_rays_backward
+        -> nerf_mlp_backward -> hashgrid_encode_backward -> adam_update (table, MLP)."""
+        c = self.cfg
+        if self._calls is None:
+            self._calls = self._prepare()
+        k = self._calls
+        k["march"].set_buffer(5, self.grid.bitfield)
+        cam, data = self.data.camera, self.data
+        flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg else 0)
+        k["rays"](_lib.train_rays_descriptor(self.n_rays, data.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx,
+                                             cam.cy, self.bound, c.seed, self.step_count, flags, c.bg))
+        for name in ("march", "encode", "mlp", "integrate", "loss", "integrate_bwd", "mlp_bwd", "encode_bwd"):
+            k[name]()
         self.step_count += 1
         t = self.step_count
         lr = lr_at(t - 1, c.lr)
         b1, b2 = 0.9, 0.99
         bp1, bp2 = float(np.float32(b1) ** t), float(np.float32(b2) ** t)
-        o = self.opt
-        mirror = table.detach() if table.dtype == torch.float16 else None
-        tab_master = table.detach() if mirror is None else self.table_master
-        _lib.call("adam_update", [tab_master, b["dtable"], o["tm"], o["tv"]] + ([mirror] if mirror is not None else []),
-                  _lib.adam_descriptor(tab_master.numel(), lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
-                                       mirror_dtype=_lib.DTYPE_F16 if mirror is not None else _lib.DTYPE_F32))
-        _lib.call("adam_update", [params, b["dparams"], o["mm"], o["mv"]],
-                  _lib.adam_descriptor(N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6))
+        if self.fp16:
+            k["check_table"]()
+            k["check_mlp"]()
+        n_table = k["adam_table"].keep[0].numel()
+        k["adam_table"](_lib.adam_descriptor(n_table, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
+                                             mirror_dtype=_lib.DTYPE_F16 if k["mirror"] else _lib.DTYPE_F32,
+                                             flags=k["flags"]))
+        k["adam_mlp"](_lib.adam_descriptor(N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
+                                           mirror_dtype=_lib.DTYPE_F16 if self.fp16 else _lib.DTYPE_F32,
+                                           flags=k["flags"]))
+        if self.fp16:
+            k["scale_update"]()
+        _lib.check()
         self.maybe_update_grid()
 
     # -- metrics (these synchronise) -------------------------------------------------------
@@ -268,6 +333,9 @@ class Trainer:
     def metrics(self) -> dict:
         b = self.b
         u = lambda t: int(t.view(torch.int32)[0].item())
-        return dict(loss=float(b["loss"][0].item()), n_valid_rays=u(b["n_valid"]),
-                    measured_batch_size_before_compaction=u(b["next_loc"]) - u(b["exceeded"]),
-                    measured_batch_size=u(b["measured"]))
+        out = dict(loss=float(b["loss"][0].item()), n_valid_rays=u(b["n_valid"]),
+                   measured_batch_size_before_compaction=u(b["next_loc"]) - u(b["exceeded"]),
+                   measured_batch_size=u(b["measured"]))
+        if self.fp16:
+            out["loss_scale"] = float(self.loss_state[:4].view(torch.float32)[0].item())
+        return out

@@ -401,19 +401,35 @@ def mlp_forward64(enc, dirs, params):
     return np.concatenate([sigma[:, None], rgb], axis=1), dict(X=X, h1=h1, o=o, cin=cin, h2=h2, h3=h3, rgb=rgb)
 
 
-def mlp_backward64(enc, dirs, params, g):
+def mlp_relu_masks_f16(enc, dirs, params):
+    """ReLU masks of a forward that rounds every layer's input to f16 (encodings,
+    h1, [o, SH], h2, h3), as the f16-weight kernels do, products in float64.  Where
+    a pre-activation lies within f16 rounding of 0 these differ from the float64
+    forward's masks, and the whole unit's gradient with them."""
+    W1, W2, W3, W4, W5 = unpack_mlp(params)
+    r16 = lambda a: np.asarray(a, np.float64).astype(np.float16).astype(np.float64)
+    h1 = np.maximum(r16(enc) @ W1, 0.0)
+    o = r16(h1) @ W2
+    h2 = np.maximum(r16(np.concatenate([o, sh4_64(dirs)], axis=1)) @ W3, 0.0)
+    h3 = np.maximum(r16(h2) @ W4, 0.0)
+    return dict(h1=h1 > 0, h2=h2 > 0, h3=h3 > 0)
+
+
+def mlp_backward64(enc, dirs, params, g, masks=None):
     """(dL/denc [S,32], dL/dparams [9408]) in float64, with trunc_exp's VJP
-    exp(clip(x, -15, 15)) (nerfs.py:230-233)."""
+    exp(clip(x, -15, 15)) (nerfs.py:230-233).  `masks` replaces the ReLU masks of
+    the float64 forward (see mlp_relu_masks_f16)."""
     W1, W2, W3, W4, W5 = unpack_mlp(params)
     _, a = mlp_forward64(enc, dirs, params)
+    m = masks if masks is not None else dict(h1=a["h1"] > 0, h2=a["h2"] > 0, h3=a["h3"] > 0)
     g = np.asarray(g, np.float64)
     dr = g[:, 1:] * a["rgb"] * (1.0 - a["rgb"])
-    dh3 = (dr @ W5.T) * (a["h3"] > 0)
-    dh2 = (dh3 @ W4.T) * (a["h2"] > 0)
+    dh3 = (dr @ W5.T) * m["h3"]
+    dh2 = (dh3 @ W4.T) * m["h2"]
     dcin = dh2 @ W3.T
     do = dcin[:, :16].copy()
     do[:, 0] += g[:, 0] * np.exp(np.clip(a["o"][:, 0], -15.0, 15.0))
-    dh1 = (do @ W2.T) * (a["h1"] > 0)
+    dh1 = (do @ W2.T) * m["h1"]
     dX = dh1 @ W1.T
     grads = [a["X"].T @ dh1, a["h1"].T @ do, a["cin"].T @ dh2, a["h2"].T @ dh3, a["h3"].T @ dr]
     return dX, np.concatenate([gw.reshape(-1) for gw in grads])

@@ -47,10 +47,12 @@ def test_descriptor_sizes_match_c_structs():
     assert len(_lib.morton3d_descriptor(5)) == 4
     assert len(_lib.packbits_descriptor(5)) == 4
     assert len(_lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])) == 9 * 4 + 32 * 4 * 2 + 33 * 4
-    assert len(_lib.nerf_mlp_descriptor(5, 0)) == 12
+    assert len(_lib.nerf_mlp_descriptor(5, 0)) == 16
     assert len(_lib.train_rays_descriptor(1, 2, 3, 4, 1.0, 1.0, 1.0, 1.0, 1.0, 5, 6, 3, (1.0, 1.0, 1.0))) == 64
-    assert len(_lib.loss_descriptor(4, 0.1)) == 8
-    assert len(_lib.adam_descriptor(10, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0)) == 48
+    assert len(_lib.loss_descriptor(4, 0.1)) == 12
+    assert len(_lib.adam_descriptor(10, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0)) == 56
+    assert len(_lib.loss_scale_check_descriptor(10)) == 8
+    assert len(_lib.loss_scale_update_descriptor()) == 12
     # field order: n_rays first, bound at byte 20
     d = _lib.marching_descriptor(7, 9, 1024, 1, 128, 1.5, 0.25)
     assert struct.unpack_from("<I", d, 0)[0] == 7 and struct.unpack_from("<f", d, 20)[0] == 1.5
@@ -68,10 +70,13 @@ _Static_assert(sizeof(IntegratingInferenceDescriptor) == 12, "");
 _Static_assert(sizeof(PackbitsDescriptor) == 4, "");
 _Static_assert(sizeof(Morton3DDescriptor) == 4, "");
 _Static_assert(sizeof(HashGridDescriptor) == 9 * 4 + 32 * 8 + 33 * 4, "");
-_Static_assert(sizeof(NerfMlpDescriptor) == 12, "");
+_Static_assert(sizeof(NerfMlpDescriptor) == 16, "");
 _Static_assert(sizeof(NgpTrainRaysDescriptor) == 64, "");
-_Static_assert(sizeof(NgpLossDescriptor) == 8, "");
-_Static_assert(sizeof(NgpAdamDescriptor) == 48, "");
+_Static_assert(sizeof(NgpLossDescriptor) == 12, "");
+_Static_assert(sizeof(NgpAdamDescriptor) == 56, "");
+_Static_assert(sizeof(NgpLossScaleState) == 16, "");
+_Static_assert(sizeof(NgpLossScaleCheckDescriptor) == 8, "");
+_Static_assert(sizeof(NgpLossScaleUpdateDescriptor) == 12, "");
 int main(void) { return 0; }
 '''
     out = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{REPO / 'include'}", "-x", "c", "-", "-o",

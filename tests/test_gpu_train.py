@@ -110,3 +110,80 @@ def test_training_reduces_loss(small_data):
     # the occupancy grid was thresholded: some cells empty, some occupied
     occ = tr.grid.occ_mask.float().mean().item()
     assert 0.0 < occ < 1.0
+
+
+def _state(dev, scale):
+    return _lib.loss_scale_state(dev, scale)
+
+
+def _scale_of(state):
+    return float(state[:4].view(torch.float32)[0].item()), int(state[4:8].view(torch.int32)[0].item()), \
+        int(state[8:12].view(torch.int32)[0].item())
+
+
+def test_loss_scaling_targets(small_data):
+    """Scaled loss gradient = scale x unscaled; the check flags a non-finite gradient;
+    Adam with the state undoes the scale, or changes nothing after an overflow; the
+    update halves after an overflow and doubles after growth_interval finite steps."""
+    dev = small_data.rgba.device
+    rng = np.random.default_rng(1)
+    n = 777
+    rgbd = torch.from_numpy(rng.uniform(0, 1, (n, 4)).astype(np.float32)).to(dev)
+    gt = torch.from_numpy(rng.uniform(0, 1, (n, 3)).astype(np.float32)).to(dev)
+    valid = torch.from_numpy(rng.uniform(size=n) < 0.8).to(dev)
+    nv = torch.empty(1, dtype=torch.int32, device=dev).view(torch.uint32)
+    loss = torch.empty(1, device=dev)
+    g1, g2 = torch.empty(n, 4, device=dev), torch.empty(n, 4, device=dev)
+    st = _state(dev, 1024.0)
+    _lib.call("train_loss_grad", [rgbd, gt, valid, nv, g1, loss], _lib.loss_descriptor(n, 0.1))
+    l1 = loss.clone()
+    _lib.call("train_loss_grad", [rgbd, gt, valid, nv, g2, loss, st], _lib.loss_descriptor(n, 0.1, _lib.LOSS_SCALED))
+    torch.testing.assert_close(g2, g1 * 1024.0, rtol=0, atol=0)
+    torch.testing.assert_close(loss, l1, rtol=1e-6, atol=0)  # summed with atomics in any order
+    # Adam: scaled gradient + state == unscaled gradient without it
+    N = 3000
+    p0 = torch.from_numpy(rng.normal(size=N).astype(np.float32)).to(dev)
+    gr = torch.from_numpy(rng.normal(size=N).astype(np.float32)).to(dev)
+    none = torch.empty(0, device=dev)
+    pa, ma, va = p0.clone(), torch.zeros(N, device=dev), torch.zeros(N, device=dev)
+    pb, mb, vb = p0.clone(), torch.zeros(N, device=dev), torch.zeros(N, device=dev)
+    desc = lambda f: _lib.adam_descriptor(N, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0, flags=f)
+    _lib.call("adam_update", [pa, gr, ma, va], desc(0))
+    _lib.call("adam_update", [pb, gr * 1024.0, mb, vb, none, st], desc(_lib.LOSS_SCALED))
+    torch.testing.assert_close(pb, pa, rtol=0, atol=0)
+    # check: finite gradients leave the flag clear, one inf sets it
+    _lib.call("loss_scale_check", [gr, st], _lib.loss_scale_check_descriptor(N))
+    assert _scale_of(st)[1] == 0
+    bad = gr.clone()
+    bad[N - 5] = float("inf")
+    _lib.call("loss_scale_check", [bad, st], _lib.loss_scale_check_descriptor(N))
+    assert _scale_of(st)[1] == 1
+    before = pb.clone()
+    _lib.call("adam_update", [pb, bad, mb, vb, none, st], desc(_lib.LOSS_SCALED))
+    torch.testing.assert_close(pb, before, rtol=0, atol=0)  # skipped
+    _lib.call("loss_scale_update", [st], _lib.loss_scale_update_descriptor(3))
+    assert _scale_of(st) == (512.0, 0, 0)
+    for k in range(3):
+        _lib.call("loss_scale_update", [st], _lib.loss_scale_update_descriptor(3))
+    assert _scale_of(st) == (1024.0, 0, 0)
+
+
+def test_fp16_training_path(small_data):
+    """f16 table + f16 MLP + dynamic loss scaling trains the scene like f32: loss
+    falls by half, and the final loss is within 1.5x of the f32 run's."""
+    finals = {}
+    for prec in ("fp32", "fp16"):
+        cfg = TrainConfig(total_samples=1 << 16, precision=prec)
+        tr = Trainer(small_data, cfg)
+        first = None
+        for i in range(300):
+            tr.step()
+            if i == 0:
+                first = tr.metrics()["loss"]
+        m = tr.metrics()
+        assert math.isfinite(m["loss"]) and m["loss"] < 0.5 * first, (prec, first, m)
+        finals[prec] = m["loss"]
+        if prec == "fp16":
+            assert tr.model.position_encoder.table.dtype == torch.float16
+            assert 1.0 <= m["loss_scale"] <= 2.0 ** 24
+    assert finals["fp16"] < 1.5 * finals["fp32"], finals
