@@ -129,11 +129,14 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
     return dict(trainer=tr, events=events, elapsed=elapsed, samples=n_samples, valid=n_valid, ms=ms)
 
 
+POOL = 1 << 18  # ray slots of the inference renderer (the reference uses 8192; same image)
+
+
 def bench_inference(tr, data, n_frames: int, dev) -> dict:
     """ms per 800x800 frame of the trained field from held-out poses, with the
-    reference's ray-pool renderer settings (8192 slots, march_steps_cap 8, white
-    background), median over the timed frames after two warm-up renders; PSNR of
-    the u8 render against the float64 ground truth blended on white."""
+    reference's ray-pool renderer (march_steps_cap 8, white background) on a pool of
+    2^18 slots, median over the timed frames after two warm-up renders; PSNR of the
+    u8 render against the float64 ground truth blended on white."""
     import torch
     from jaxngp_amd import scene as sc
     from jaxngp_amd.renderer import render_image_inference
@@ -142,7 +145,7 @@ def bench_inference(tr, data, n_frames: int, dev) -> dict:
     gt = sc.make_views(data.scene, data.camera, poses, dev)
     c = tr.cfg
     kw = dict(K=tr.K, G=c.grid_res, diagonal_n_steps=c.diagonal_n_steps, stepsize_portion=tr.portion,
-              march_steps_cap=8, pool=8192)
+              march_steps_cap=8, pool=POOL)
     pose_t = [torch.as_tensor(p, dtype=torch.float32, device=dev) for p in poses]
     for i in range(2):
         render_image_inference(tr.model, data.camera, pose_t[i % n_frames], tr.grid.bitfield, **kw)
@@ -157,7 +160,7 @@ def bench_inference(tr, data, n_frames: int, dev) -> dict:
     times.sort()
     log(f"inference: {times[len(times) // 2]:.2f} ms/frame (median of {n_frames}), PSNR {np.mean(psnrs):.2f} dB")
     return {"ms_per_frame": times[len(times) // 2], "frames": n_frames, "psnr_db": float(np.mean(psnrs)),
-            "resolution": "800x800", "pool": 8192, "march_steps_cap": 8, "background": "white",
+            "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
             "train_steps": tr.step_count}
 
 
@@ -225,15 +228,20 @@ def main() -> None:
     }
     dom = max((k for k in per if k in algo), key=lambda k: per[k])
     bound, work = algo[dom]
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload (copy-calibrated), committed beside the kernel-time summary
+    from jaxngp_amd.profiling import read_traffic
+    traffic = read_traffic(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                        "training_step_fp32_hbm_traffic.txt")).get(dom)
     if bound == "hbm":
         achieved = work / (per[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                 "algorithmic_bytes_per_launch": work, "ms_per_launch": per[dom]}
     else:
         achieved = work / (per[dom] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": None, "kernel": dom,
+                "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": traffic, "kernel": dom,
                 "algorithmic_flops_per_launch": work, "ms_per_launch": per[dom]}
 
     out = {

@@ -1,0 +1,108 @@
+"""Summaries of rocprofv3 output for the kernels of this library.
+
+    python -m jaxngp_amd.profiling stats  KERNEL_STATS_CSV [TOP]
+    python -m jaxngp_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES
+
+`stats` prints the kernels of a `--kernel-trace --stats` run by total time.
+`traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of a workload, and
+the same two counters read on a plain device copy of COPY_BYTES bytes, into HBM
+bytes per launch of each target: every counter is scaled so that the copy reads and
+writes exactly COPY_BYTES (the factor is printed), summed over the target's kernels
+and divided by the target's launches.
+"""
+from __future__ import annotations
+
+import collections
+import csv
+import glob
+import sys
+
+# the kernels each target launches, and one that runs exactly once per launch
+TARGET_KERNELS = {
+    "march_rays": (["march_count_kernel", "march_accumulate_kernel", "march_scan_kernel", "march_emit_kernel",
+                    "march_finish_kernel", "march_empty_kernel"], "march_scan_kernel"),
+    "hashgrid_encode": (["hashgrid_fwd_lm_kernel", "hashgrid_fwd_kernel"], None),
+    "hashgrid_encode_backward": (["bwd_count_kernel", "scan_sums_kernel", "scan_block_sums_kernel",
+                                  "scan_apply_kernel", "plan_parts_kernel", "bwd_scatter_kernel", "bwd_reduce_kernel",
+                                  "bwd_combine_kernel", "zero_table_kernel"], "plan_parts_kernel"),
+    "nerf_mlp_forward": (["mlp_forward_kernel", "mlp_forward_h_kernel"], None),
+    "nerf_mlp_backward": (["mlp_backward_kernel", "mlp_backward_h_kernel", "sum_rows_kernel"], "sum_rows_kernel"),
+    "integrate_rays": (["integrate_kernel"], "integrate_kernel"),
+    "integrate_rays_backward": (["integrate_backward_kernel"], "integrate_backward_kernel"),
+}
+
+
+def _short(name: str) -> str:
+    n = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("ngp::", "")
+    return n if n.startswith("at::") else n.split("(")[0]
+
+
+def _base(name: str) -> str:
+    return _short(name).split("<")[0]
+
+
+def stats(path: str, top: int = 40) -> str:
+    rows = list(csv.DictReader(open(path)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = [f"{'kernel':64s} {'calls':>7s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'%time':>6s}"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
+        out.append(f"{_short(r['Name'])[:64]:64s} {r['Calls']:>7s} {float(r['AverageNs']) / 1e3:9.1f} "
+                   f"{float(r['MinNs']) / 1e3:9.1f} {float(r['MaxNs']) / 1e3:9.1f} "
+                   f"{float(r['TotalDurationNs']) / tot * 100:6.1f}")
+    return "\n".join(out)
+
+
+def _counter(d: str) -> dict:
+    """kernel base name -> [dispatches, summed counter value (KiB)]"""
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for f in glob.glob(d + "/**/*counter_collection*.csv", recursive=True):
+        for r in csv.DictReader(open(f)):
+            a = agg[_base(r["Kernel_Name"])]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"] or 0)
+    return agg
+
+
+def traffic(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes) -> str:
+    f, w = _counter(fetch_dir), _counter(write_dir)
+    cf, cw = _counter(copy_fetch_dir), _counter(copy_write_dir)
+    pick = lambda c: max(c.items(), key=lambda kv: kv[1][1])  # the copy kernel: the largest counter total
+    (cname, (cn, cfv)), (_, (cwn, cwv)) = pick(cf), pick(cw)
+    kf = copy_bytes / (cfv / cn * 1024.0)
+    kw = copy_bytes / (cwv / cwn * 1024.0)
+    out = [f"copy calibration: kernel {cname}, {copy_bytes} bytes; FETCH_SIZE reads {cfv / cn:.1f} KiB -> scale "
+           f"{kf:.4f}; WRITE_SIZE reads {cwv / cwn:.1f} KiB -> scale {kw:.4f}", ""]
+    out.append(f"{'target':26s} {'launches':>8s} {'fetch_MB':>10s} {'write_MB':>10s} {'traffic_MB':>11s}  kernels")
+    for tgt, (kernels, once) in TARGET_KERNELS.items():
+        present = [k for k in kernels if k in f and k in w]
+        if not present:
+            continue
+        launches = f[once][0] if once in f else max(f[k][0] for k in present)
+        fb = sum(f[k][1] for k in present) * 1024.0 * kf / launches
+        wb = sum(w[k][1] for k in present) * 1024.0 * kw / launches
+        out.append(f"{tgt:26s} {launches:8d} {fb / 1e6:10.1f} {wb / 1e6:10.1f} {(fb + wb) / 1e6:11.1f}  "
+                   f"{','.join(present)}")
+        out.append(f"traffic_bytes_per_launch {tgt} {fb + wb:.0f}")
+    return "\n".join(out)
+
+
+def read_traffic(path: str) -> dict:
+    """{target: HBM bytes per launch} from a summary written by `traffic`."""
+    res = {}
+    try:
+        for line in open(path):
+            p = line.split()
+            if len(p) == 3 and p[0] == "traffic_bytes_per_launch":
+                res[p[1]] = float(p[2])
+    except OSError:
+        pass
+    return res
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        print(stats(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 40))
+    elif sys.argv[1] == "traffic":
+        print(traffic(*sys.argv[2:6], float(sys.argv[6])))
+    else:
+        raise SystemExit(__doc__)

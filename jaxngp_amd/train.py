@@ -216,9 +216,9 @@ class Trainer:
     def threshold_ogrid(self):
         """types.py:1227-1239."""
         g = self.grid
-        thr = torch.minimum(
-            torch.tensor(density_threshold_from_min_step_size(self.cfg.diagonal_n_steps, self.bound),
-                         device=self.dev), g.mean_density_up_to_cascade(1))
+        # clamp against a Python scalar: no host-to-device copy, so no host sync
+        thr = torch.clamp(g.mean_density_up_to_cascade(1),
+                          max=density_threshold_from_min_step_size(self.cfg.diagonal_n_steps, self.bound))
         from .volrend import packbits
         g.occ_mask, g.bitfield = packbits(thr, g.density)
 
