@@ -154,7 +154,8 @@ void march_rays(ngp_stream_t stream, void **buffers, const char *opaque, size_t 
  *      terminated bool[n], indices_in u32[n]
  * out: next_ray_index u32[1], indices_out u32[n], n_samples u32[n],
  *      t_starts_out f32[n], xyzs f32[n,cap,3], dss f32[n,cap], z_vals f32[n,cap]
- * Terminated slots receive new rays in slot order. */
+ * Terminated slots receive new rays in slot order.  indices_out must not alias
+ * indices_in (next_ray_index may alias next_ray_index_in). */
 void march_rays_inference(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* marching.cu:606-636.  opaque: Morton3DDescriptor.  in: xyzs u32[len,3]; out: idcs u32[len]. */
@@ -229,6 +230,39 @@ void nerf_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, s
  * out: dL_dencodings (same layout as encodings), dL_dparams f32[NGP_MLP_N_PARAMS],
  *      workspace f32[n_workspace_blocks, NGP_MLP_N_PARAMS] */
 void nerf_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct NerfFieldChunksDescriptor {
+    HashGridDescriptor grid;  /* the NeRF position encoder (L = 16, F = 2); n_points and
+                                 enc_layout are not used */
+    uint32_t n_slots;         /* ray-pool slots */
+    uint32_t cap;             /* march_steps_cap: samples per slot */
+    uint32_t n_total_rays;    /* rows of rays_d */
+    uint32_t param_dtype;     /* MLP weights: NGP_DTYPE_F32 or NGP_DTYPE_F16 */
+} NerfFieldChunksDescriptor;
+
+/* The field (hash encoding + NeRF MLP, models/nerfs.py:66-86) on one round of the
+ * inference ray pool, fused: slot j holds n_samples[j] samples of ray indices[j]
+ * (the outputs of march_rays_inference); padding entries get drgbs = 0.  Equal to
+ * hashgrid_encode followed by nerf_mlp_forward on the marched samples.  No
+ * counterpart in the reference, which evaluates its field on every [slot, step]
+ * entry, padding included (models/renderers/cuda.py:213-223).
+ * in : xyzs f32[n_slots, cap, 3], n_samples u32[n_slots], indices u32[n_slots],
+ *      rays_d f32[n_total_rays, 3], table f32|f16[offsets[16], 2],
+ *      params f32|f16[NGP_MLP_N_PARAMS]
+ * out: drgbs f32[n_slots, cap, 4] */
+void nerf_field_chunks(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct NgpPoolScatterDescriptor {
+    uint32_t n;       /* pool slots */
+    uint32_t n_total; /* rows of dst */
+    uint32_t width;   /* floats per row */
+} NgpPoolScatterDescriptor;
+
+/* dst[idx[i]] = src[i] for every slot whose idx[i] < n_total (the wrappers'
+ * `.at[indices].set(...)` with out-of-range indices dropped,
+ * volrendjax/marching/__init__.py:139-157, integrating/__init__.py:97-110).
+ * in : idx u32[n], src f32[n, width]   inout: dst f32[n_total, width] */
+void pool_scatter(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* ---------------------------------------------------------------------------
  * Training-step harness (the device-side counterpart of app/nerf/_utils.py:80-170

@@ -39,6 +39,8 @@ TARGETS = (
     "adam_update",
     "loss_scale_check",
     "loss_scale_update",
+    "nerf_field_chunks",
+    "pool_scatter",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -246,6 +248,15 @@ def adam_descriptor(n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_dec
                     mirror_dtype=DTYPE_F32, flags=0) -> bytes:
     return struct.pack("<Q9f3I", n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_decay, grad_scale,
                        mirror_dtype, flags, 0)
+
+
+def pool_scatter_descriptor(n, n_total, width) -> bytes:
+    return struct.pack("<3I", n, n_total, width)
+
+
+def field_chunks_descriptor(grid_desc: bytes, n_slots, cap, n_total_rays, param_dtype=DTYPE_F32) -> bytes:
+    """NerfFieldChunksDescriptor: a HashGridDescriptor followed by four u32."""
+    return grid_desc + struct.pack("<4I", n_slots, cap, n_total_rays, param_dtype)
 
 
 def loss_scale_check_descriptor(n) -> bytes:

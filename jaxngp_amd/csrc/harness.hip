@@ -196,6 +196,16 @@ __global__ void loss_scale_update_kernel(NgpLossScaleUpdateDescriptor d, NgpLoss
     *state = s;
 }
 
+__global__ void __launch_bounds__(kBlock)
+    pool_scatter_kernel(uint32_t n, uint32_t n_total, uint32_t width, const uint32_t *__restrict__ idx,
+                        const float *__restrict__ src, float *__restrict__ dst) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (uint64_t)n * width) return;
+    const uint32_t i = (uint32_t)(t / width), c = (uint32_t)(t % width);
+    const uint32_t r = idx[i];
+    if (r < n_total) dst[(size_t)r * width + c] = src[t];
+}
+
 }  // namespace
 }  // namespace ngp
 
@@ -296,5 +306,22 @@ extern "C" void loss_scale_update(ngp_stream_t stream, void **buffers, const cha
     if (!check_buffers(buffers, 1, target)) return;
     hipLaunchKernelGGL(loss_scale_update_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, d,
                        (NgpLossScaleState *)buffers[0]);
+    check_launch(target);
+}
+
+extern "C" void pool_scatter(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "pool_scatter";
+    NgpPoolScatterDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.width == 0) {
+        set_error(kBadDescriptor, "%s: width must be positive", target);
+        return;
+    }
+    if (d.n == 0) return;
+    if (!check_buffers(buffers, 3, target)) return;
+    const uint64_t total = (uint64_t)d.n * d.width;
+    hipLaunchKernelGGL(pool_scatter_kernel, dim3((uint32_t)((total + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, d.n, d.n_total, d.width, (const uint32_t *)buffers[0],
+                       (const float *)buffers[1], (float *)buffers[2]);
     check_launch(target);
 }

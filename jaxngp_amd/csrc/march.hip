@@ -523,22 +523,60 @@ __global__ void march_empty_kernel(uint32_t total_samples, uint32_t *next_loc, u
 // inference
 // ---------------------------------------------------------------------------
 
-// Single workgroup: terminated slots take new ray indices in slot order
-// (the reference hands them out with atomicAdd, marching.cu:300).
-__global__ void __launch_bounds__(kScanThreads)
-    assign_rays_kernel(uint32_t n_rays, const uint32_t *__restrict__ next_in, const bool *__restrict__ terminated,
-                       const uint32_t *__restrict__ indices_in, uint32_t *__restrict__ next_out,
-                       uint32_t *__restrict__ indices_out) {
-    uint32_t base = *next_in;
-    for (uint32_t c = 0; c < n_rays; c += kScanThreads) {
-        const uint32_t i = c + threadIdx.x;
-        const uint32_t term = (i < n_rays && terminated[i]) ? 1u : 0u;
-        uint32_t total;
-        const uint32_t pre = block_exclusive_scan<kScanThreads>(term, &total);
-        if (i < n_rays) indices_out[i] = term ? base + pre : indices_in[i];
-        base += total;
+// Terminated slots take new ray indices in slot order (the reference hands them
+// out with atomicAdd, marching.cu:300), in three launches without scratch memory:
+// per-block counts of terminated slots are parked in the first slot of each
+// block's range of indices_out, one workgroup turns them into exclusive prefixes
+// (and writes next_out), and every block then assigns its range.
+constexpr int kAssignBlock = 1024;
+
+__global__ void __launch_bounds__(kAssignBlock)
+    assign_count_kernel(uint32_t n_rays, const bool *__restrict__ terminated, uint32_t *__restrict__ indices_out) {
+    __shared__ uint32_t wave_cnt[kAssignBlock / kWave];
+    const uint32_t j = blockIdx.x * kAssignBlock + threadIdx.x;
+    const bool t = j < n_rays && terminated[j];
+    const uint32_t c = (uint32_t)__popcll(__ballot(t));
+    if (lane_id() == 0) wave_cnt[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < kAssignBlock / kWave; ++w) s += wave_cnt[w];
+        indices_out[(size_t)blockIdx.x * kAssignBlock] = s;
     }
-    if (threadIdx.x == 0) *next_out = base;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+    assign_scan_kernel(uint32_t n_blocks, const uint32_t *__restrict__ next_in, uint32_t *__restrict__ next_out,
+                       uint32_t *__restrict__ indices_out) {
+    const uint32_t first = *next_in;  // read before anything is written (next_out may alias it)
+    uint32_t carry = first;
+    for (uint32_t c0 = 0; c0 < n_blocks; c0 += kScanThreads) {
+        const uint32_t b = c0 + threadIdx.x;
+        const uint32_t v = b < n_blocks ? indices_out[(size_t)b * kAssignBlock] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kScanThreads>(v, &tot);
+        if (b < n_blocks) indices_out[(size_t)b * kAssignBlock] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *next_out = carry;
+}
+
+__global__ void __launch_bounds__(kAssignBlock)
+    assign_write_kernel(uint32_t n_rays, const bool *__restrict__ terminated, const uint32_t *__restrict__ indices_in,
+                        uint32_t *__restrict__ indices_out) {
+    __shared__ uint32_t base;
+    __shared__ uint32_t wave_cnt[kAssignBlock / kWave];
+    const uint32_t j = blockIdx.x * kAssignBlock + threadIdx.x;
+    if (threadIdx.x == 0) base = indices_out[(size_t)blockIdx.x * kAssignBlock];
+    const bool t = j < n_rays && terminated[j];
+    const uint64_t m = __ballot(t);
+    if (lane_id() == 0) wave_cnt[threadIdx.x / kWave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = base;
+    const int wave = (int)threadIdx.x / kWave;
+    for (int w = 0; w < wave; ++w) before += wave_cnt[w];
+    const uint64_t below = (1ull << lane_id()) - 1ull;
+    if (j < n_rays) indices_out[j] = t ? before + (uint32_t)__popcll(m & below) : indices_in[j];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -737,8 +775,13 @@ extern "C" void march_rays_inference(ngp_stream_t stream_, void **buffers, const
     float *dss = (float *)buffers[13];
     float *zs = (float *)buffers[14];
     const MarchParams p = make_march_params(d.diagonal_n_steps, d.K, d.G, d.bound, d.stepsize_portion);
-    hipLaunchKernelGGL(assign_rays_kernel, dim3(1), dim3(kScanThreads), 0, stream, d.n_rays, next_in, terminated,
-                       indices_in, next_out, indices_out);
+    const uint32_t n_assign = div_up(d.n_rays, kAssignBlock);
+    hipLaunchKernelGGL(assign_count_kernel, dim3(n_assign), dim3(kAssignBlock), 0, stream, d.n_rays, terminated,
+                       indices_out);
+    hipLaunchKernelGGL(assign_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, n_assign, next_in, next_out,
+                       indices_out);
+    hipLaunchKernelGGL(assign_write_kernel, dim3(n_assign), dim3(kAssignBlock), 0, stream, d.n_rays, terminated,
+                       indices_in, indices_out);
     if (!check_launch(target)) return;
     hipLaunchKernelGGL(march_inference_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, p,
                        bitfield_words_ok(bits, d.K, d.G), d.n_total_rays, d.n_rays, d.march_steps_cap, rays_o, rays_d, t_starts, t_ends, bits,

@@ -25,6 +25,7 @@
 // workgroup writes one row of the workspace, and a second kernel sums the rows
 // in a fixed order.
 #include "ngp_common.h"
+#include "hashgrid_device.h"
 
 namespace ngp {
 namespace {
@@ -169,6 +170,19 @@ __device__ __forceinline__ void store_enc4(float *__restrict__ enc, bool level_m
 }
 
 // Forward of one 16-sample tile starting at sample `s0` (samples >= n are zero).
+// Forward of a tile whose encodings f.x are loaded, for direction (dx, dy, dz).
+__device__ __forceinline__ void forward_from_x(const float *lds, float dx, float dy, float dz, int i, int g, Fwd &f) {
+    layer_fwd<32, 64, S_W1>(lds + L_W1, f.x, f.h1, i, g);
+    relu<4>(f.h1);
+    layer_fwd<64, 16, S_W2>(lds + L_W2, f.h1, &f.c[0], i, g);
+    f.c[1] = sh4_part(dx, dy, dz, g);
+    layer_fwd<32, 64, S_W3>(lds + L_W3, f.c, f.h2, i, g);
+    relu<4>(f.h2);
+    layer_fwd<64, 64, S_W4>(lds + L_W4, f.h2, f.h3, i, g);
+    relu<4>(f.h3);
+    layer_fwd<64, 16, S_W5>(lds + L_W5, f.h3, &f.rr, i, g);
+}
+
 __device__ __forceinline__ void forward_tile(const float *lds, const float *__restrict__ enc, bool level_major,
                                              const float *__restrict__ dirs, uint32_t n, size_t s0, int i, int g,
                                              Fwd &f) {
@@ -523,6 +537,24 @@ struct FwdH {
     v4f x[2], h1[4], c[2], h2[4], h3[4], rr;  // f32 values; c[0] = o, c[1] = SH
 };
 
+__device__ __forceinline__ void forward_from_x_h(const _Float16 *lds, float dx, float dy, float dz, int i, int g,
+                                                 FwdH &f) {
+    const h8 xin = pack_h8(f.x[0], f.x[1]);
+    layer_fwd_h<1, 4, FT1>(lds + H_F1, &xin, f.h1, i, g);
+    relu<4>(f.h1);
+    const h8 h1in[2] = {pack_h8(f.h1[0], f.h1[1]), pack_h8(f.h1[2], f.h1[3])};
+    layer_fwd_h<2, 1, FT2>(lds + H_F2, h1in, &f.c[0], i, g);
+    f.c[1] = sh4_part(dx, dy, dz, g);
+    const h8 cin = pack_h8(f.c[0], f.c[1]);
+    layer_fwd_h<1, 4, FT3>(lds + H_F3, &cin, f.h2, i, g);
+    relu<4>(f.h2);
+    const h8 h2in[2] = {pack_h8(f.h2[0], f.h2[1]), pack_h8(f.h2[2], f.h2[3])};
+    layer_fwd_h<2, 4, FT4>(lds + H_F4, h2in, f.h3, i, g);
+    relu<4>(f.h3);
+    const h8 h3in[2] = {pack_h8(f.h3[0], f.h3[1]), pack_h8(f.h3[2], f.h3[3])};
+    layer_fwd_h<2, 1, FT5>(lds + H_F5, h3in, &f.rr, i, g);
+}
+
 __device__ __forceinline__ void forward_tile_h(const _Float16 *lds, const float *__restrict__ enc, bool level_major,
                                                const float *__restrict__ dirs, uint32_t n, size_t s0, int i, int g,
                                                FwdH &f) {
@@ -711,6 +743,117 @@ __global__ void __launch_bounds__(kThreads)
 
 static_assert(H_END * 2 >= P_END * 4, "the weight area holds the gradient reduction");
 
+// ---------------------------------------------------------------------------
+// Fused field query of the inference renderer: hash encoding (L = 16, F = 2) and
+// MLP in one pass over the [slots, cap] samples of one ray-pool round, of which
+// slot j holds n_samples[j]; padding entries get drgbs = 0.  Lane (i, g) of a tile
+// computes exactly the four levels 2g, 2g+1, 8+2g, 9+2g of sample i that its MFMA
+// B operand needs (features 4g..4g+3 and 16+4g..16+4g+3), with the encoder's
+// arithmetic, so the result equals hashgrid_encode followed by nerf_mlp_forward.
+// ---------------------------------------------------------------------------
+
+template <typename TT>
+__device__ __forceinline__ float2 encode_level(const HashGridDescriptor &d, const LevelTables &lt,
+                                               const TT *__restrict__ table, float px, float py, float pz,
+                                               uint32_t l) {
+    uint32_t idx[8];
+    float w[8];
+    level_vertices(px, py, pz, lt.scale[l], lt.res[l], l < d.first_hash_level, d.T, (d.T & (d.T - 1u)) == 0,
+                   lt.offset[l], d.n_rows - 1u, idx, w);
+    float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const TT *row = table + (size_t)idx[c] * 2;
+        a0 = a0 + to_f32<TT>(row[0]) * w[c];
+        a1 = a1 + to_f32<TT>(row[1]) * w[c];
+    }
+    return make_float2(a0, a1);
+}
+
+template <typename TT, bool kHalf>
+__global__ void __launch_bounds__(kThreads)
+    field_chunks_kernel(HashGridDescriptor d, uint32_t n_slots, uint32_t cap, uint32_t n_total,
+                        const float *__restrict__ xyzs, const uint32_t *__restrict__ n_samples,
+                        const uint32_t *__restrict__ indices, const float *__restrict__ rays_d,
+                        const TT *__restrict__ table, const void *__restrict__ params, float *__restrict__ drgbs) {
+    __shared__ __align__(16) float lds_f[kHalf ? 1 : L_END];
+    __shared__ __align__(16) _Float16 lds_h[kHalf ? H_FEND : 1];
+    __shared__ LevelTables lt;
+    if (kHalf) stage_weights_h<false>(lds_h, static_cast<const _Float16 *>(params));
+    else stage_weights(lds_f, static_cast<const float *>(params));
+    if (threadIdx.x < 16) {
+        lt.scale[threadIdx.x] = d.scales[threadIdx.x];
+        lt.res[threadIdx.x] = d.resolutions[threadIdx.x];
+        lt.offset[threadIdx.x] = d.offsets[threadIdx.x];
+    }
+    __syncthreads();
+    const int lane = lane_id();
+    const int i = lane & 15, g = lane >> 4;
+    const uint64_t n_entries = (uint64_t)n_slots * cap;
+    const uint64_t n_tiles = (n_entries + kTile - 1) / kTile;
+    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / kWave;
+    const uint32_t n_waves = gridDim.x * kWaves;
+    const float two_b = 2.0f * d.bound;
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint64_t t = wave; t < n_tiles; t += n_waves) {
+        const uint64_t e = t * kTile + i;
+        bool live = false;
+        uint32_t ray = 0;
+        if (e < n_entries) {
+            const uint32_t slot = (uint32_t)(e / cap), k = (uint32_t)(e % cap);
+            live = k < n_samples[slot];
+            if (live) {
+                ray = indices[slot];
+                live = ray < n_total;
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (g == 0 && e < n_entries) *reinterpret_cast<v4f *>(drgbs + 4 * e) = zero;
+            continue;
+        }
+        float px = 0.0f, py = 0.0f, pz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
+        if (live) {
+            px = (xyzs[3 * e + 0] + d.bound) / two_b;
+            py = (xyzs[3 * e + 1] + d.bound) / two_b;
+            pz = (xyzs[3 * e + 2] + d.bound) / two_b;
+            dx = rays_d[3 * (size_t)ray + 0];
+            dy = rays_d[3 * (size_t)ray + 1];
+            dz = rays_d[3 * (size_t)ray + 2];
+        }
+        v4f x[2];
+        if (live) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const float2 lo = encode_level(d, lt, table, px, py, pz, 8 * b + 2 * g);
+                const float2 hi = encode_level(d, lt, table, px, py, pz, 8 * b + 2 * g + 1);
+                x[b] = v4f{lo.x, lo.y, hi.x, hi.y};
+            }
+        } else {
+            x[0] = x[1] = zero;
+        }
+        v4f out = zero;
+        float o0;
+        v4f rr;
+        if (kHalf) {
+            FwdH f;
+            f.x[0] = x[0];
+            f.x[1] = x[1];
+            forward_from_x_h(lds_h, dx, dy, dz, i, g, f);
+            o0 = f.c[0][0];
+            rr = f.rr;
+        } else {
+            Fwd f;
+            f.x[0] = x[0];
+            f.x[1] = x[1];
+            forward_from_x(lds_f, dx, dy, dz, i, g, f);
+            o0 = f.c[0][0];
+            rr = f.rr;
+        }
+        if (live) out = v4f{det_expf(o0), sigmoidf(rr[0]), sigmoidf(rr[1]), sigmoidf(rr[2])};
+        if (g == 0 && e < n_entries) *reinterpret_cast<v4f *>(drgbs + 4 * e) = out;
+    }
+}
+
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 uint32_t forward_grid(uint32_t n) {
@@ -781,5 +924,54 @@ extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const ch
     hipLaunchKernelGGL(sum_rows_kernel, dim3(div_up(P_END, kWave)), dim3(kSumWaves * kWave), 0, stream,
                        d.n_workspace_blocks,
                        (const float *)buffers[6], (float *)buffers[5]);
+    check_launch(target);
+}
+
+extern "C" void nerf_field_chunks(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "nerf_field_chunks";
+    NerfFieldChunksDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    const HashGridDescriptor &g = d.grid;
+    if (g.L != 16 || g.F != 2 || g.T == 0 || g.n_rows == 0 || g.first_hash_level > g.L || g.offsets[0] != 0 ||
+        g.offsets[g.L] != g.n_rows || !(g.bound > 0.0f) || g.table_dtype > NGP_DTYPE_F16 ||
+        d.param_dtype > NGP_DTYPE_F16 || d.cap == 0) {
+        set_error(kBadDescriptor, "%s: needs the NeRF grid (L=16, F=2) with a consistent level table, cap > 0 and "
+                  "f32/f16 dtypes", target);
+        return;
+    }
+    for (uint32_t l = 0; l < g.L; ++l) {
+        if (g.offsets[l + 1] < g.offsets[l]) {
+            set_error(kBadDescriptor, "%s: offsets must be non-decreasing", target);
+            return;
+        }
+    }
+    if (d.n_slots == 0) return;
+    if (!check_buffers(buffers, 7, target)) return;
+    if (!aligned16(buffers[6])) {
+        set_error(kBadArgument, "%s: drgbs must be 16-byte aligned", target);
+        return;
+    }
+    const uint64_t tiles = ((uint64_t)d.n_slots * d.cap + kTile - 1) / kTile;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((tiles + kWaves - 1) / kWaves, 2048));
+    hipStream_t s = (hipStream_t)stream;
+    const float *xyzs = (const float *)buffers[0];
+    const uint32_t *ns = (const uint32_t *)buffers[1];
+    const uint32_t *idx = (const uint32_t *)buffers[2];
+    const float *rd = (const float *)buffers[3];
+    const void *params = buffers[5];
+    float *out = (float *)buffers[6];
+    const bool t16 = g.table_dtype == NGP_DTYPE_F16, w16 = d.param_dtype == NGP_DTYPE_F16;
+    if (t16 && w16)
+        hipLaunchKernelGGL((field_chunks_kernel<_Float16, true>), dim3(grid), dim3(kThreads), 0, s, g, d.n_slots, d.cap,
+                           d.n_total_rays, xyzs, ns, idx, rd, (const _Float16 *)buffers[4], params, out);
+    else if (t16)
+        hipLaunchKernelGGL((field_chunks_kernel<_Float16, false>), dim3(grid), dim3(kThreads), 0, s, g, d.n_slots,
+                           d.cap, d.n_total_rays, xyzs, ns, idx, rd, (const _Float16 *)buffers[4], params, out);
+    else if (w16)
+        hipLaunchKernelGGL((field_chunks_kernel<float, true>), dim3(grid), dim3(kThreads), 0, s, g, d.n_slots, d.cap,
+                           d.n_total_rays, xyzs, ns, idx, rd, (const float *)buffers[4], params, out);
+    else
+        hipLaunchKernelGGL((field_chunks_kernel<float, false>), dim3(grid), dim3(kThreads), 0, s, g, d.n_slots, d.cap,
+                           d.n_total_rays, xyzs, ns, idx, rd, (const float *)buffers[4], params, out);
     check_launch(target);
 }

@@ -18,7 +18,7 @@ import math
 
 import torch
 
-from . import volrend
+from . import _lib, volrend
 from .scene import Camera, f32_to_u8
 
 
@@ -77,11 +77,21 @@ def march_and_integrate_inference(next_ray_index, terminated, indices, rays_rgbd
     next_ray_index, indices, n_samples, t_starts, xyzs, dss, z_vals = volrend.march_rays_inference(
         diagonal_n_steps, K, G, march_steps_cap, model.bound, stepsize_portion, rays_o, rays_d, t_starts, t_ends,
         occupancy_bitfield, next_ray_index, terminated, indices)
-    idx = indices.view(torch.int32).long().clamp(max=rays_d.shape[0] - 1)
-    dirs = rays_d[idx][:, None, :].expand(xyzs.shape).contiguous()
-    drgbs, _ = model(xyzs.reshape(-1, 3), dirs.reshape(-1, 3), None)
+    # The reference evaluates the field on every [slot, step] entry, zero padding
+    # included (cuda.py:213-223); here only the marched samples go through the
+    # encoder and MLP, and the padding entries get drgbs = 0 (integrate_rays_inference
+    # reads only the first n_samples of each slot).
+    n_slots, cap = dss.shape
+    live = (torch.arange(cap, device=dss.device)[None, :] < n_samples.view(torch.int32)[:, None]).reshape(-1)
+    pos = live.nonzero().squeeze(1)  # one host sync per round
+    drgbs = torch.zeros(n_slots * cap, 4, dtype=torch.float32, device=dss.device)
+    if pos.numel():
+        slot = torch.div(pos, cap, rounding_mode="floor")
+        ray = indices.view(torch.int32)[slot].long()
+        out, _ = model(xyzs.reshape(-1, 3)[pos], rays_d[ray], None)
+        drgbs[pos] = out
     terminate_cnt, terminated, rays_rgbd, rays_T = volrend.integrate_rays_inference(
-        rays_bg, rays_rgbd, rays_T, n_samples, indices, dss, z_vals, drgbs.reshape(*dss.shape, 4))
+        rays_bg, rays_rgbd, rays_T, n_samples, indices, dss, z_vals, drgbs.reshape(n_slots, cap, 4))
     return terminate_cnt, next_ray_index, terminated, indices, rays_rgbd, rays_T, t_starts
 
 
@@ -90,10 +100,17 @@ def render_image_inference(model, camera: Camera, pose: torch.Tensor, occupancy_
                            K=1, G=128, diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8,
                            pool: int = 1 << 18):
     """Renders one frame (cuda.py:244-373).  Returns (bg [H,W,3] f32, rgb u8 [H,W,3],
-    depth u8 [H,W]).  The reference uses a pool of 8192 ray slots; the image does
-    not depend on the pool size (every ray is marched in the same chunks of
-    march_steps_cap steps whichever slot holds it), so the default here is 2^18
-    slots, which fills the GPU with fewer rounds."""
+    depth u8 [H,W]).
+
+    Same ray-pool rounds as the reference: march_rays_inference, the field on the
+    marched samples, integrate_rays_inference, scatter of the per-slot results,
+    and one host sync per outer round to count finished rays.  The image does not
+    depend on the pool size (every ray is marched in the same chunks of
+    march_steps_cap steps whichever slot holds it); the reference uses 8192
+    slots, the default here is 2^18, which fills the GPU in fewer rounds.  The
+    buffers are allocated once per frame and every launch is pre-bound; the field
+    runs fused (nerf_field_chunks) on the marched samples only.
+    """
     dev = pose.device
     o, d = make_rays_worldspace(camera, pose)
     t_starts, t_ends = make_near_far_from_bound(model.bound, o, d)
@@ -101,23 +118,55 @@ def render_image_inference(model, camera: Camera, pose: torch.Tensor, occupancy_
     rays_rgbd = torch.zeros(n_px, 4, device=dev)
     rays_T = torch.ones(n_px, device=dev)
     rays_bg = torch.as_tensor(bg, dtype=torch.float32, device=dev).expand(n_px, 3).contiguous()
-    n_rays = min(pool, n_px)
-    next_ray_index = torch.zeros(1, dtype=torch.int32, device=dev).view(torch.uint32)
-    terminated = torch.ones(n_rays, dtype=torch.bool, device=dev)
-    indices = torch.zeros(n_rays, dtype=torch.int32, device=dev).view(torch.uint32)
-    n_rendered = 0
+    n = min(pool, n_px)
+    cap = int(march_steps_cap)
+    u32 = torch.uint32
+    nxt = torch.zeros(2, dtype=torch.int32, device=dev).view(u32)
+    terminated = torch.ones(n, dtype=torch.bool, device=dev)
+    idx = torch.zeros(2, n, dtype=torch.int32, device=dev).view(u32)
+    n_samples = torch.empty(n, dtype=u32, device=dev)
+    t_out = torch.empty(n, device=dev)
+    xyzs = torch.empty(n, cap, 3, device=dev)
+    dss = torch.empty(n, cap, device=dev)
+    zs = torch.empty(n, cap, device=dev)
+    drgbs = torch.empty(n, cap, 4, device=dev)
+    cnt = torch.empty(1, dtype=u32, device=dev)
+    rgbd_out = torch.empty(n, 4, device=dev)
+    T_out = torch.empty(n, device=dev)
+    total = torch.zeros((), dtype=torch.int64, device=dev)
+    pe = model.position_encoder
+    half = getattr(model, "mlp_half", None) is not None
+    params = model.mlp_half if half else model.mlp_params.detach()
+    fdesc = _lib.field_chunks_descriptor(pe.descriptor(0, model.bound), n, cap, n_px,
+                                         _lib.DTYPE_F16 if half else _lib.DTYPE_F32)
+    mdesc = _lib.marching_inference_descriptor(n_px, n, diagonal_n_steps, K, G, cap, model.bound, stepsize_portion)
+    P = _lib.Prepared
+    rounds = []
+    for a, b in ((0, 1), (1, 0)):  # next index / slot indices ping-pong between rounds
+        ib = idx[b]
+        rounds.append((
+            P("march_rays_inference", [o, d, t_starts, t_ends, occupancy_bitfield, nxt[a:a + 1], terminated, idx[a],
+                                       nxt[b:b + 1], ib, n_samples, t_out, xyzs, dss, zs], mdesc),
+            P("pool_scatter", [ib, t_out, t_starts], _lib.pool_scatter_descriptor(n, n_px, 1)),
+            P("nerf_field_chunks", [xyzs, n_samples, ib, d, pe.table.detach(), params, drgbs], fdesc),
+            P("integrate_rays_inference", [rays_bg, rays_rgbd, rays_T, n_samples, ib, dss, zs, drgbs, cnt,
+                                           terminated, rgbd_out, T_out],
+              _lib.integrating_inference_descriptor(n_px, n, cap)),
+            P("pool_scatter", [ib, rgbd_out, rays_rgbd], _lib.pool_scatter_descriptor(n, n_px, 4)),
+            P("pool_scatter", [ib, T_out, rays_T], _lib.pool_scatter_descriptor(n, n_px, 1)),
+        ))
+    cnt32 = cnt.view(torch.int32)
+    n_rendered, r = 0, 0
     while n_rendered < n_px:
-        iters = max(1, (n_px - n_rendered) // n_rays)
+        iters = max(1, (n_px - n_rendered) // n)
         iters = 2 ** int(math.log2(iters) + 1)
-        cnt_total = torch.zeros((), dtype=torch.int64, device=dev)
         for _ in range(iters):
-            cnt, next_ray_index, terminated, indices, rays_rgbd, rays_T, t_starts = march_and_integrate_inference(
-                next_ray_index, terminated, indices, rays_rgbd, rays_T, t_starts, model=model, rays_o=o, rays_d=d,
-                t_ends=t_ends, rays_bg=rays_bg, occupancy_bitfield=occupancy_bitfield,
-                march_steps_cap=march_steps_cap, K=K, G=G, diagonal_n_steps=diagonal_n_steps,
-                stepsize_portion=stepsize_portion)
-            cnt_total += cnt.view(torch.int32).long()
-        n_rendered += int(cnt_total.item())
+            for call in rounds[r & 1]:
+                call()
+            total += cnt32[0]
+            r += 1
+        _lib.check()
+        n_rendered = int(total.item())
     rgb = f32_to_u8(rays_rgbd[:, :3]).reshape(camera.height, camera.width, 3)
     depth = rays_rgbd[:, 3]
     depth_u8 = f32_to_u8((depth - depth.min()) / (depth.max() - depth.min() + 1e-15)).reshape(camera.height,

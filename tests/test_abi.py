@@ -53,6 +53,9 @@ def test_descriptor_sizes_match_c_structs():
     assert len(_lib.adam_descriptor(10, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0)) == 56
     assert len(_lib.loss_scale_check_descriptor(10)) == 8
     assert len(_lib.loss_scale_update_descriptor()) == 12
+    assert len(_lib.pool_scatter_descriptor(1, 2, 3)) == 12
+    hg = _lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])
+    assert len(_lib.field_chunks_descriptor(hg, 1, 8, 3)) == len(hg) + 16
     # field order: n_rays first, bound at byte 20
     d = _lib.marching_descriptor(7, 9, 1024, 1, 128, 1.5, 0.25)
     assert struct.unpack_from("<I", d, 0)[0] == 7 and struct.unpack_from("<f", d, 20)[0] == 1.5
@@ -77,6 +80,8 @@ _Static_assert(sizeof(NgpAdamDescriptor) == 56, "");
 _Static_assert(sizeof(NgpLossScaleState) == 16, "");
 _Static_assert(sizeof(NgpLossScaleCheckDescriptor) == 8, "");
 _Static_assert(sizeof(NgpLossScaleUpdateDescriptor) == 12, "");
+_Static_assert(sizeof(NgpPoolScatterDescriptor) == 12, "");
+_Static_assert(sizeof(NerfFieldChunksDescriptor) == sizeof(HashGridDescriptor) + 16, "");
 int main(void) { return 0; }
 '''
     out = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{REPO / 'include'}", "-x", "c", "-", "-o",
