@@ -265,6 +265,56 @@ typedef struct NgpPoolScatterDescriptor {
 void pool_scatter(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* ---------------------------------------------------------------------------
+ * Learned proposal (no counterpart in the reference): a hash grid (L = 5, F = 2)
+ * and a 10 -> 16 -> 1 MLP predicting density at 64 equal bins of each ray's
+ * segment [t_start, t_end), trained against the NeRF's sample weights.
+ * ------------------------------------------------------------------------- */
+
+#define NGP_PROPOSAL_BINS 64u
+#define NGP_PROPOSAL_N_PARAMS 176u /* W1 [10][16], W2 [16][1], no biases */
+
+typedef struct ProposalRaysDescriptor {
+    uint32_t n_rays;
+} ProposalRaysDescriptor;
+
+/* Bin centres o + (t0 + (b + 1/2) (t1 - t0) / 64) d.  opaque: ProposalRaysDescriptor.
+ * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n]
+ * out: points f32[n, 64, 3] */
+void proposal_points(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct ProposalMlpDescriptor {
+    uint32_t n_points;
+    uint32_t n_workspace_blocks; /* rows of the backward workspace (0 for forward) */
+    uint32_t enc_layout;         /* encodings [N, 10] or level-major [5, N, 2] */
+    uint32_t param_dtype;        /* NGP_DTYPE_F32 or NGP_DTYPE_F16 weights */
+} ProposalMlpDescriptor;
+
+/* sigma = exp(relu(x W1) W2).  opaque: ProposalMlpDescriptor.
+ * in : encodings f32, params f32|f16[NGP_PROPOSAL_N_PARAMS]   out: sigma f32[N] */
+void proposal_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* Backward of proposal_mlp_forward (exp's VJP clipped at |logit| <= 15).
+ * in : encodings, params, dL_dsigma f32[N]
+ * out: dL_dencodings (layout of encodings), dL_dparams f32[NGP_PROPOSAL_N_PARAMS],
+ *      workspace f32[n_workspace_blocks, NGP_PROPOSAL_N_PARAMS] */
+void proposal_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct ProposalLossDescriptor {
+    uint32_t n_rays;
+    float eps; /* 1e-7 */
+} ProposalLossDescriptor;
+
+/* Histogram-bound loss.  Per valid ray the NeRF's sample weights w_i = T_i alpha_i
+ * (0 once T falls to 1e-4) are summed into 64 equal bins of [t0, t1) (W_b); the
+ * proposal's bin weights are w_b = T_b (1 - exp(-sigma_b D)), D = (t1 - t0) / 64;
+ * loss = sum over valid rays and bins of max(0, W_b - w_b)^2 / (W_b + eps) / n_valid.
+ * in : rays_sample_startidx u32[n], rays_n_samples u32[n], dss f32[S], z_vals f32[S],
+ *      drgbs f32[S,4] (the NeRF's), t_starts f32[n], t_ends f32[n],
+ *      sigma_p f32[n, 64], ray_is_valid bool[n], n_valid_rays u32[1]
+ * out: dL_dsigma_p f32[n, 64], loss f32[1] */
+void proposal_loss(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* ---------------------------------------------------------------------------
  * Training-step harness (the device-side counterpart of app/nerf/_utils.py:80-170
  * and app/nerf/train.py:47-48).  Same calling convention.
  * ------------------------------------------------------------------------- */

@@ -41,6 +41,10 @@ TARGETS = (
     "loss_scale_update",
     "nerf_field_chunks",
     "pool_scatter",
+    "proposal_points",
+    "proposal_mlp_forward",
+    "proposal_mlp_backward",
+    "proposal_loss",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -248,6 +252,22 @@ def adam_descriptor(n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_dec
                     mirror_dtype=DTYPE_F32, flags=0) -> bytes:
     return struct.pack("<Q9f3I", n, lr, b1, b2, eps, eps_root, b1_pow_t, b2_pow_t, weight_decay, grad_scale,
                        mirror_dtype, flags, 0)
+
+
+PROPOSAL_BINS = 64
+PROPOSAL_N_PARAMS = 176
+
+
+def proposal_rays_descriptor(n_rays) -> bytes:
+    return struct.pack("<I", n_rays)
+
+
+def proposal_mlp_descriptor(n_points, n_workspace_blocks=0, enc_layout=ENC_POINT_MAJOR, param_dtype=DTYPE_F32) -> bytes:
+    return struct.pack("<4I", n_points, n_workspace_blocks, enc_layout, param_dtype)
+
+
+def proposal_loss_descriptor(n_rays, eps=1e-7) -> bytes:
+    return struct.pack("<If", n_rays, eps)
 
 
 def pool_scatter_descriptor(n, n_total, width) -> bytes:

@@ -48,6 +48,7 @@ SOURCES = {
     "hashgrid.hip": ["-ffp-contract=off"],
     "nerf_mlp.hip": ["-ffp-contract=off"],
     "harness.hip": ["-ffp-contract=off"],
+    "proposal.hip": ["-ffp-contract=off"],
 }
 
 

@@ -134,6 +134,13 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Float sum over the wavefront (butterfly: every lane gets the same value).
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
 __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);

@@ -1,0 +1,322 @@
+// proposal.hip — the learned proposal network (no counterpart in the reference):
+// a small hash grid (L = 5) and a 10 -> 16 -> 1 MLP predict density at 64 equal
+// bins of each ray's segment [t0, t1); trained with a histogram-bound loss against
+// the NeRF's own sample weights w_i = T_i alpha_i (integrating.cu:58-66), it tells
+// where each ray's mass lies.
+//
+//   proposal_points:       bin centres t0 + (b + 1/2) D, D = (t1 - t0) / 64
+//   proposal_mlp_forward:  sigma = exp(relu(x W1) W2) per point, f32 math, weights
+//                          f32 or f16 (the MLP has 176 weights and K = 10: plain
+//                          FMAs in registers, MFMA tiles would be mostly padding)
+//   proposal_mlp_backward: dL/dx and dL/dW (fixed-order reductions)
+//   proposal_loss:         per ray, one wavefront = 64 lanes = 64 bins
+#include <hip/hip_fp16.h>
+
+#include "ngp_common.h"
+
+namespace ngp {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kBins = NGP_PROPOSAL_BINS;
+constexpr int kIn = 10, kHid = 16;
+constexpr int kParams = NGP_PROPOSAL_N_PARAMS;  // W1 [10][16], W2 [16][1]
+static_assert(kBins == kWave, "one lane per bin");
+static_assert(kParams == kIn * kHid + kHid, "parameter count");
+
+__global__ void __launch_bounds__(kBlock)
+    proposal_points_kernel(uint32_t n_rays, const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+                           const float *__restrict__ t0s, const float *__restrict__ t1s, float *__restrict__ pts) {
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (uint64_t)n_rays * kBins) return;
+    const uint32_t r = (uint32_t)(t / kBins), b = (uint32_t)(t % kBins);
+    const float t0 = t0s[r], t1 = t1s[r];
+    const float D = (t1 - t0) / (float)kBins;
+    const float tc = t0 + ((float)b + 0.5f) * D;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pts[3 * t + c] = rays_o[3 * (size_t)r + c] + tc * rays_d[3 * (size_t)r + c];
+}
+
+template <typename PT>
+__device__ __forceinline__ void stage_params(float *w, const PT *__restrict__ params) {
+    for (int k = threadIdx.x; k < kParams; k += kBlock) w[k] = (float)params[k];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void load_x(const float *__restrict__ enc, bool level_major, uint32_t n, uint32_t i,
+                                       float x[kIn]) {
+#pragma unroll
+    for (int l = 0; l < kIn / 2; ++l) {
+        const size_t at = level_major ? ((size_t)l * n + i) * 2 : (size_t)i * kIn + 2 * l;
+        x[2 * l] = enc[at];
+        x[2 * l + 1] = enc[at + 1];
+    }
+}
+
+__device__ __forceinline__ float hidden_and_logit(const float *w, const float x[kIn], float h[kHid]) {
+    float o = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kHid; ++j) {
+        float a = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) a = a + x[k] * w[k * kHid + j];
+        h[j] = fmaxf(a, 0.0f);
+        o = o + h[j] * w[kIn * kHid + j];
+    }
+    return o;
+}
+
+template <typename PT>
+__global__ void __launch_bounds__(kBlock)
+    proposal_fwd_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const PT *__restrict__ params,
+                        float *__restrict__ sigma) {
+    __shared__ float w[kParams];
+    stage_params(w, params);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        float x[kIn], h[kHid];
+        load_x(enc, level_major, n, i, x);
+        sigma[i] = det_expf(hidden_and_logit(w, x, h));
+    }
+}
+
+// One workspace row of kParams per workgroup; waves add in a fixed order.
+template <typename PT>
+__global__ void __launch_bounds__(kBlock)
+    proposal_bwd_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const PT *__restrict__ params,
+                        const float *__restrict__ dL_dsigma, float *__restrict__ dL_denc,
+                        float *__restrict__ workspace) {
+    __shared__ float w[kParams];
+    __shared__ float part[kBlock / kWave][kParams];
+    stage_params(w, params);
+    float gW[kParams];
+#pragma unroll
+    for (int k = 0; k < kParams; ++k) gW[k] = 0.0f;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        float x[kIn], h[kHid];
+        load_x(enc, level_major, n, i, x);
+        const float o = hidden_and_logit(w, x, h);
+        const float dz = dL_dsigma[i] * det_expf(fminf(fmaxf(o, -15.0f), 15.0f));
+        float dx[kIn];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) dx[k] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < kHid; ++j) {
+            gW[kIn * kHid + j] += h[j] * dz;
+            const float dh = h[j] > 0.0f ? dz * w[kIn * kHid + j] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < kIn; ++k) {
+                gW[k * kHid + j] += x[k] * dh;
+                dx[k] = dx[k] + dh * w[k * kHid + j];
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < kIn / 2; ++l) {
+            const size_t at = level_major ? ((size_t)l * n + i) * 2 : (size_t)i * kIn + 2 * l;
+            dL_denc[at] = dx[2 * l];
+            dL_denc[at + 1] = dx[2 * l + 1];
+        }
+    }
+    const int wave = (int)threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < kParams; ++k) {
+        const float s = wave_sumf(gW[k]);
+        if (lane_id() == 0) part[wave][k] = s;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kParams; k += kBlock) {
+        float s = 0.0f;
+        for (int v = 0; v < kBlock / kWave; ++v) s += part[v][k];
+        workspace[(size_t)blockIdx.x * kParams + k] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    proposal_sum_rows_kernel(uint32_t rows, const float *__restrict__ ws, float *__restrict__ out) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= (uint32_t)kParams) return;
+    float s = 0.0f;
+    for (uint32_t r = 0; r < rows; ++r) s += ws[(size_t)r * kParams + k];
+    out[k] = s;
+}
+
+// Inclusive product scan over the wavefront.
+__device__ __forceinline__ float wave_prod_scan(float v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const float o = __shfl_up(v, d, kWave);
+        if (lane >= d) v *= o;
+    }
+    return v;
+}
+
+// Inclusive suffix sum over the wavefront (lane b gets sum over b' >= b).
+__device__ __forceinline__ float wave_suffix_sum(float v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const float o = __shfl_down(v, d, kWave);
+        if (lane + d < kWave) v += o;
+    }
+    return v;
+}
+
+constexpr float kTThreshold = 1e-4f;  // integrating.cu:12
+
+__global__ void __launch_bounds__(kBlock)
+    proposal_loss_kernel(uint32_t n_rays, float eps, const uint32_t *__restrict__ startidx,
+                         const uint32_t *__restrict__ nsamp, const float *__restrict__ dss,
+                         const float *__restrict__ z_vals, const float *__restrict__ drgbs,
+                         const float *__restrict__ t0s, const float *__restrict__ t1s,
+                         const float *__restrict__ sigma_p, const bool *__restrict__ valid,
+                         const uint32_t *__restrict__ n_valid, float *__restrict__ dL_dsigma_p,
+                         float *__restrict__ loss) {
+    __shared__ float bins[kBlock / kWave][kBins];
+    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
+    if (r >= n_rays) return;
+    float *out = dL_dsigma_p + (size_t)r * kBins;
+    if (!valid[r]) {
+        out[lane] = 0.0f;
+        return;
+    }
+    const float nv = (float)max(*n_valid, 1u);
+    const float t0 = t0s[r], t1 = t1s[r];
+    const float inv = (float)kBins / (t1 - t0);
+    const float D = (t1 - t0) / (float)kBins;
+    bins[wave][lane] = 0.0f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t s0 = startidx[r], m = nsamp[r];
+    float T = 1.0f;
+    for (uint32_t c = 0; c < m; c += kWave) {
+        const uint32_t k = c + lane;
+        const bool live = k < m;
+        const float a = live ? 1.0f - det_expf(-drgbs[4 * (size_t)(s0 + k)] * dss[s0 + k]) : 0.0f;
+        const float incl = wave_prod_scan(1.0f - a);
+        float excl = __shfl_up(incl, 1, kWave);
+        if (lane == 0) excl = 1.0f;
+        const float Tk = T * excl;
+        if (live && Tk > kTThreshold) {
+            const float x = (z_vals[s0 + k] - t0) * inv;
+            const int b = (int)fminf(fmaxf(floorf(x), 0.0f), (float)(kBins - 1));
+            atomicAdd(&bins[wave][b], Tk * a);
+        }
+        T = T * __shfl(incl, kWave - 1, kWave);
+        if (T <= kTThreshold) break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float W = bins[wave][lane];
+    const float a = 1.0f - det_expf(-sigma_p[(size_t)r * kBins + lane] * D);
+    const float incl = wave_prod_scan(1.0f - a);
+    float Tb = __shfl_up(incl, 1, kWave);
+    if (lane == 0) Tb = 1.0f;
+    const float wp = Tb * a;
+    const float gap = fmaxf(W - wp, 0.0f);
+    const float l = gap * gap / (W + eps) / nv;
+    const float g = -2.0f * gap / (W + eps) / nv;
+    const float suffix = wave_suffix_sum(g * wp) - g * wp;
+    out[lane] = D * (g * (Tb * (1.0f - a)) - suffix);
+    const float ls = wave_sumf(l);
+    if (lane == 0 && ls != 0.0f) atomicAdd(loss, ls);
+}
+
+__global__ void zero_f32_kernel(float *p) {
+    if (threadIdx.x == 0) *p = 0.0f;
+}
+
+}  // namespace
+}  // namespace ngp
+
+using namespace ngp;
+
+extern "C" void proposal_points(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_points";
+    ProposalRaysDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.n_rays == 0) return;
+    if (!check_buffers(buffers, 5, target)) return;
+    const uint64_t n = (uint64_t)d.n_rays * kBins;
+    hipLaunchKernelGGL(proposal_points_kernel, dim3((uint32_t)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       (hipStream_t)stream, d.n_rays, (const float *)buffers[0], (const float *)buffers[1],
+                       (const float *)buffers[2], (const float *)buffers[3], (float *)buffers[4]);
+    check_launch(target);
+}
+
+static bool valid_mlp_desc(const ProposalMlpDescriptor &d, const char *target) {
+    if (d.enc_layout > NGP_ENC_LEVEL_MAJOR || d.param_dtype > NGP_DTYPE_F16) {
+        set_error(kBadDescriptor, "%s: enc_layout and param_dtype must be 0 or 1", target);
+        return false;
+    }
+    return true;
+}
+
+extern "C" void proposal_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_mlp_forward";
+    ProposalMlpDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (!valid_mlp_desc(d, target)) return;
+    if (d.n_points == 0) return;
+    if (!check_buffers(buffers, 3, target)) return;
+    const uint32_t grid = std::min<uint32_t>(div_up(d.n_points, kBlock), 2048);
+    const bool lm = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
+    if (d.param_dtype == NGP_DTYPE_F16)
+        hipLaunchKernelGGL(proposal_fwd_kernel<__half>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, d.n_points,
+                           lm, (const float *)buffers[0], (const __half *)buffers[1], (float *)buffers[2]);
+    else
+        hipLaunchKernelGGL(proposal_fwd_kernel<float>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, d.n_points,
+                           lm, (const float *)buffers[0], (const float *)buffers[1], (float *)buffers[2]);
+    check_launch(target);
+}
+
+extern "C" void proposal_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_mlp_backward";
+    ProposalMlpDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (!valid_mlp_desc(d, target)) return;
+    if (d.n_workspace_blocks == 0) {
+        set_error(kBadDescriptor, "%s: n_workspace_blocks must be positive", target);
+        return;
+    }
+    if (!check_buffer_list(buffers, {1, 4, 5}, target)) return;
+    if (d.n_points > 0 && !check_buffers(buffers, 6, target)) return;
+    hipStream_t s = (hipStream_t)stream;
+    const bool lm = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
+    if (d.param_dtype == NGP_DTYPE_F16)
+        hipLaunchKernelGGL(proposal_bwd_kernel<__half>, dim3(d.n_workspace_blocks), dim3(kBlock), 0, s, d.n_points, lm,
+                           (const float *)buffers[0], (const __half *)buffers[1], (const float *)buffers[2],
+                           (float *)buffers[3], (float *)buffers[5]);
+    else
+        hipLaunchKernelGGL(proposal_bwd_kernel<float>, dim3(d.n_workspace_blocks), dim3(kBlock), 0, s, d.n_points, lm,
+                           (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
+                           (float *)buffers[3], (float *)buffers[5]);
+    if (!check_launch(target)) return;
+    hipLaunchKernelGGL(proposal_sum_rows_kernel, dim3(div_up(kParams, kBlock)), dim3(kBlock), 0, s,
+                       d.n_workspace_blocks, (const float *)buffers[5], (float *)buffers[4]);
+    check_launch(target);
+}
+
+extern "C" void proposal_loss(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_loss";
+    ProposalLossDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (!(d.eps > 0.0f)) {
+        set_error(kBadDescriptor, "%s: eps must be positive", target);
+        return;
+    }
+    if (!check_buffer_list(buffers, {11}, target)) return;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, (float *)buffers[11]);
+    if (d.n_rays > 0) {
+        if (!check_buffers(buffers, 12, target)) return;
+        const uint32_t grid = div_up(d.n_rays, kBlock / kWave);
+        hipLaunchKernelGGL(proposal_loss_kernel, dim3(grid), dim3(kBlock), 0, s, d.n_rays, d.eps,
+                           (const uint32_t *)buffers[0], (const uint32_t *)buffers[1], (const float *)buffers[2],
+                           (const float *)buffers[3], (const float *)buffers[4], (const float *)buffers[5],
+                           (const float *)buffers[6], (const float *)buffers[7], (const bool *)buffers[8],
+                           (const uint32_t *)buffers[9], (float *)buffers[10], (float *)buffers[11]);
+    }
+    check_launch(target);
+}

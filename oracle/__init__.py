@@ -496,3 +496,90 @@ def cpu_train_step(rays_o, rays_d, t_starts, t_ends, noises, bits, bgs, gt, tabl
     lib().oracle_hashgrid_bwd(u32(S), u32(L), u32(F), u32(T), u32(lv["first_hash_level"]), u32(lv["n_rows"]),
                               f32(1.0), P(scales), P(res), P(offs), P(m["xyzs"]), P(dX), P(dt))
     return int(m["measured"])
+
+
+# ---------------------------------------------------------------------------
+# Learned proposal (no counterpart in the reference; defined by this build and
+# pinned by finite differences and closed forms in tests/test_oracle_pins.py)
+# ---------------------------------------------------------------------------
+
+PROP_BINS = 64
+PROP_N_PARAMS = 10 * 16 + 16  # Dense 10->16 (ReLU), Dense 16->1, no biases
+
+
+def proposal_mlp_forward64(enc, params):
+    """sigma = exp(relu(enc W1) W2) for enc [N, 10]; returns (sigma [N], hidden, logit)."""
+    p = np.asarray(params, np.float64)
+    W1, W2 = p[:160].reshape(10, 16), p[160:176].reshape(16, 1)
+    h = np.maximum(np.asarray(enc, np.float64) @ W1, 0.0)
+    o = (h @ W2)[:, 0]
+    return np.exp(o), h, o
+
+
+def proposal_mlp_backward64(enc, params, g_sigma):
+    """(dL/denc [N, 10], dL/dparams [176]) with trunc_exp's VJP exp(clip(o, -15, 15))."""
+    p = np.asarray(params, np.float64)
+    W1, W2 = p[:160].reshape(10, 16), p[160:176].reshape(16, 1)
+    X = np.asarray(enc, np.float64)
+    _, h, o = proposal_mlp_forward64(X, p)
+    do = np.asarray(g_sigma, np.float64) * np.exp(np.clip(o, -15.0, 15.0))
+    dW2 = h.T @ do[:, None]
+    dh = (do[:, None] @ W2.T) * (h > 0)
+    return dh @ W1.T, np.concatenate([(X.T @ dh).reshape(-1), dW2.reshape(-1)])
+
+
+def proposal_bin_index(z, t0, t1, n_bins=PROP_BINS):
+    """Bin of sample distance z on the ray segment [t0, t1), in the kernels' f32
+    arithmetic: floor((z - t0) * (n_bins / (t1 - t0))) clamped to [0, n_bins)."""
+    f = np.float32
+    inv = f(n_bins) / (f(t1) - f(t0))
+    x = (f(z) - f(t0)) * inv
+    return int(min(max(math.floor(float(x)), 0), n_bins - 1))
+
+
+def nerf_sample_weights64(startidx, nsamp, dss, drgbs):
+    """w_i = T_i alpha_i of every sample (integrating.cu:58-66), 0 after the ray's
+    transmittance falls to 1e-4."""
+    w = np.zeros(len(dss))
+    for r in range(len(startidx)):
+        s0, m = int(startidx[r]), int(nsamp[r])
+        T = 1.0
+        for k in range(m):
+            if T <= T_THRESHOLD:
+                break
+            a = 1.0 - math.exp(-float(drgbs[s0 + k, 0]) * float(dss[s0 + k]))
+            w[s0 + k] = T * a
+            T *= 1.0 - a
+    return w
+
+
+def proposal_loss64(startidx, nsamp, dss, z_vals, drgbs, t0, t1, sigma_p, valid, n_valid, eps=1e-7):
+    """Histogram-bound loss of the proposal.  Per valid ray, the NeRF's sample
+    weights are summed into 64 equal bins of [t0, t1) (W_b); the proposal's density
+    sigma_p[r, b] at the bin centres gives bin weights w_b = T_b (1 - exp(-sigma_b D)),
+    D = (t1 - t0) / 64; loss = sum_r sum_b max(0, W_b - w_b)^2 / (W_b + eps) / n_valid.
+    Returns (loss, dL/dsigma_p [n_rays, 64])."""
+    w = nerf_sample_weights64(startidx, nsamp, dss, drgbs)
+    n_rays = len(startidx)
+    sig = np.asarray(sigma_p, np.float64).reshape(n_rays, PROP_BINS)
+    grad = np.zeros_like(sig)
+    loss = 0.0
+    nv = max(int(n_valid), 1)
+    for r in range(n_rays):
+        if not valid[r]:
+            continue
+        W = np.zeros(PROP_BINS)
+        s0, m = int(startidx[r]), int(nsamp[r])
+        for k in range(m):
+            W[proposal_bin_index(z_vals[s0 + k], t0[r], t1[r])] += w[s0 + k]
+        D = (float(t1[r]) - float(t0[r])) / PROP_BINS
+        a = 1.0 - np.exp(-sig[r] * D)
+        T = np.concatenate([[1.0], np.cumprod(1.0 - a)[:-1]])
+        wp = T * a
+        gap = np.maximum(W - wp, 0.0)
+        loss += float((gap ** 2 / (W + eps)).sum()) / nv
+        g = -2.0 * gap / (W + eps) / nv  # dL/dwp
+        Tn = T * (1.0 - a)  # T_{b+1}
+        suffix = np.concatenate([np.cumsum((g * wp)[::-1])[::-1][1:], [0.0]])  # sum_{b' > b} g wp
+        grad[r] = D * (g * Tn - suffix)
+    return loss, grad

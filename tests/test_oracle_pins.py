@@ -531,3 +531,72 @@ def test_sh4_orthonormality():
     Y = oracle.sh4_64(d)
     gram = 4 * np.pi * (Y.T @ Y) / len(d)
     np.testing.assert_allclose(gram, np.eye(16), atol=0.02)
+
+
+# ---- learned proposal (defined by this build; pinned by calculus and closed forms) ----
+
+def test_proposal_mlp_backward_matches_finite_differences():
+    rng = np.random.default_rng(31)
+    N = 7
+    enc = rng.normal(0, 0.7, (N, 10))
+    p = rng.normal(0, 0.4, oracle.PROP_N_PARAMS)
+    g = rng.normal(size=N)
+    dX, dP = oracle.proposal_mlp_backward64(enc, p, g)
+    f = lambda e, q: float((oracle.proposal_mlp_forward64(e, q)[0] * g).sum())
+    h = 1e-6
+    for k in rng.choice(oracle.PROP_N_PARAMS, 24, replace=False):
+        d = np.zeros_like(p)
+        d[k] = h
+        assert abs((f(enc, p + d) - f(enc, p - d)) / (2 * h) - dP[k]) <= 1e-6 * (1 + abs(dP[k]))
+    for (i, j) in [(0, 0), (3, 9), (6, 4)]:
+        d = np.zeros_like(enc)
+        d[i, j] = h
+        assert abs((f(enc + d, p) - f(enc - d, p)) / (2 * h) - dX[i, j]) <= 1e-6 * (1 + abs(dX[i, j]))
+
+
+def _prop_rays(rng, n_rays=5):
+    ns = rng.integers(1, 120, n_rays).astype(np.uint32)
+    start = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.uint32)
+    S = int(ns.sum())
+    t0 = rng.uniform(2.0, 3.0, n_rays).astype(np.float32)
+    t1 = (t0 + rng.uniform(1.0, 2.5, n_rays)).astype(np.float32)
+    z = np.zeros(S, np.float32)
+    dss = np.zeros(S, np.float32)
+    for r in range(n_rays):
+        zz = np.sort(rng.uniform(t0[r], t1[r], ns[r]))
+        z[start[r]:start[r] + ns[r]] = zz
+        dss[start[r]:start[r] + ns[r]] = 0.0034
+    drgbs = np.concatenate([rng.uniform(0, 60, (S, 1)), rng.uniform(0, 1, (S, 3))], 1).astype(np.float32)
+    return start, ns, dss, z, drgbs, t0, t1
+
+
+def test_nerf_sample_weights_sum_to_opacity():
+    rng = np.random.default_rng(2)
+    start, ns, dss, z, drgbs, t0, t1 = _prop_rays(rng)
+    w = oracle.nerf_sample_weights64(start, ns, dss, drgbs)
+    _, op, _ = oracle.integrate_rays64(start, ns, np.zeros(3), dss, z, drgbs)
+    for r in range(len(start)):
+        assert abs(w[start[r]:start[r] + ns[r]].sum() - op[r]) <= 1e-12
+
+
+def test_proposal_loss_gradient_and_closed_forms():
+    rng = np.random.default_rng(3)
+    start, ns, dss, z, drgbs, t0, t1 = _prop_rays(rng)
+    n = len(start)
+    valid = np.array([True, True, False, True, True])
+    sig = rng.uniform(0, 30, (n, oracle.PROP_BINS))
+    loss, grad = oracle.proposal_loss64(start, ns, dss, z, drgbs, t0, t1, sig, valid, 4)
+    assert loss > 0 and (grad[2] == 0).all()
+    h = 1e-6
+    for r, b in [(0, 0), (1, 17), (3, 40), (4, 63), (0, 31)]:
+        d = np.zeros_like(sig)
+        d[r, b] = h
+        fd = (oracle.proposal_loss64(start, ns, dss, z, drgbs, t0, t1, sig + d, valid, 4)[0] -
+              oracle.proposal_loss64(start, ns, dss, z, drgbs, t0, t1, sig - d, valid, 4)[0]) / (2 * h)
+        assert abs(fd - grad[r, b]) <= 1e-6 * (1 + abs(grad[r, b])), (r, b, fd, grad[r, b])
+    # a proposal that is opaque from the first bin bounds nothing after it; one that puts
+    # at least the NeRF's weight in every bin has zero loss
+    empty = np.zeros(ns.sum(), np.float32)
+    l0, g0 = oracle.proposal_loss64(start, ns, dss, z, np.concatenate([empty[:, None], drgbs[:, 1:]], 1), t0, t1,
+                                    sig, valid, 4)
+    assert l0 == 0 and (g0 == 0).all()
