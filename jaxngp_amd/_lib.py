@@ -45,6 +45,7 @@ TARGETS = (
     "proposal_mlp_forward",
     "proposal_mlp_backward",
     "proposal_loss",
+    "proposal_sample",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -264,6 +265,10 @@ def proposal_rays_descriptor(n_rays) -> bytes:
 
 def proposal_mlp_descriptor(n_points, n_workspace_blocks=0, enc_layout=ENC_POINT_MAJOR, param_dtype=DTYPE_F32) -> bytes:
     return struct.pack("<4I", n_points, n_workspace_blocks, enc_layout, param_dtype)
+
+
+def proposal_sample_descriptor(n_rays, n_per_ray=48, mass_threshold=0.9) -> bytes:
+    return struct.pack("<2If", n_rays, n_per_ray, mass_threshold)
 
 
 def proposal_loss_descriptor(n_rays, eps=1e-7) -> bytes:

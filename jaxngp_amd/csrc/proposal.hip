@@ -223,6 +223,73 @@ __global__ void __launch_bounds__(kBlock)
     if (lane == 0 && ls != 0.0f) atomicAdd(loss, ls);
 }
 
+// Inclusive prefix sum over the wavefront.
+__device__ __forceinline__ float wave_prefix_sum(float v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const float o = __shfl_up(v, d, kWave);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// One wavefront per ray, lane b = bin b; lanes j < n_per_ray place sample j.
+__global__ void __launch_bounds__(kBlock)
+    proposal_sample_kernel(uint32_t n_rays, uint32_t n_per_ray, float mass_threshold,
+                           const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+                           const float *__restrict__ t0s, const float *__restrict__ t1s,
+                           const float *__restrict__ sigma_p, uint32_t *__restrict__ startidx,
+                           uint32_t *__restrict__ counts, bool *__restrict__ fallback, float *__restrict__ xyzs,
+                           float *__restrict__ dss, float *__restrict__ zs) {
+    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
+    if (r >= n_rays) return;
+    const float t0 = t0s[r], t1 = t1s[r];
+    const bool seg = t1 > t0;
+    const float D = (t1 - t0) / (float)kBins;
+    const float a = seg ? 1.0f - det_expf(-sigma_p[(size_t)r * kBins + lane] * D) : 0.0f;
+    const float incl = wave_prod_scan(1.0f - a);
+    float Tb = __shfl_up(incl, 1, kWave);
+    if (lane == 0) Tb = 1.0f;
+    const float w = Tb * a;
+    const float M = wave_sumf(w);
+    const bool ok = seg && M >= mass_threshold;
+    const size_t base = (size_t)r * n_per_ray;
+    if (lane == 0) {
+        startidx[r] = (uint32_t)base;
+        counts[r] = ok ? n_per_ray : 0u;
+        fallback[r] = seg && !ok;
+    }
+    float t = 0.0f, ds = 0.0f;
+    if (ok) {
+        const float p = w / M;
+        const float C = wave_prefix_sum(p) - p;  // exclusive
+        const float u = ((float)lane + 0.5f) / (float)n_per_ray;
+        // last bin with C_b <= u (C is non-decreasing, C_0 = 0)
+        int b = 0;
+#pragma unroll
+        for (int step = kBins / 2; step > 0; step >>= 1) {
+            const int cand = b + step;
+            const float cc = __shfl(C, cand < kBins ? cand : kBins - 1, kWave);
+            if (cand < kBins && cc <= u) b = cand;
+        }
+        const float pb = __shfl(p, b, kWave), cb = __shfl(C, b, kWave);
+        const float frac = pb > 0.0f ? fminf(fmaxf((u - cb) / pb, 0.0f), 1.0f) : 0.5f;
+        t = t0 + ((float)b + frac) * D;
+        const float tn = __shfl_down(t, 1, kWave);
+        ds = (uint32_t)lane + 1 < n_per_ray ? tn - t
+                                            : fminf(pb > 0.0f ? D / ((float)n_per_ray * pb) : D, t1 - t);
+    }
+    if ((uint32_t)lane < n_per_ray) {
+        const size_t j = base + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xyzs[3 * j + c] = ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
+        dss[j] = ds;
+        zs[j] = t;
+    }
+}
+
 __global__ void zero_f32_kernel(float *p) {
     if (threadIdx.x == 0) *p = 0.0f;
 }
@@ -318,5 +385,23 @@ extern "C" void proposal_loss(ngp_stream_t stream, void **buffers, const char *o
                            (const float *)buffers[6], (const float *)buffers[7], (const bool *)buffers[8],
                            (const uint32_t *)buffers[9], (float *)buffers[10], (float *)buffers[11]);
     }
+    check_launch(target);
+}
+
+extern "C" void proposal_sample(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_sample";
+    ProposalSampleDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.n_per_ray == 0 || d.n_per_ray > kBins || !(d.mass_threshold >= 0.0f)) {
+        set_error(kBadDescriptor, "%s: n_per_ray must be in [1, %d] and mass_threshold >= 0", target, kBins);
+        return;
+    }
+    if (d.n_rays == 0) return;
+    if (!check_buffers(buffers, 11, target)) return;
+    hipLaunchKernelGGL(proposal_sample_kernel, dim3(div_up(d.n_rays, kBlock / kWave)), dim3(kBlock), 0,
+                       (hipStream_t)stream, d.n_rays, d.n_per_ray, d.mass_threshold, (const float *)buffers[0],
+                       (const float *)buffers[1], (const float *)buffers[2], (const float *)buffers[3],
+                       (const float *)buffers[4], (uint32_t *)buffers[5], (uint32_t *)buffers[6], (bool *)buffers[7],
+                       (float *)buffers[8], (float *)buffers[9], (float *)buffers[10]);
     check_launch(target);
 }

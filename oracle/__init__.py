@@ -583,3 +583,45 @@ def proposal_loss64(startidx, nsamp, dss, z_vals, drgbs, t0, t1, sigma_p, valid,
         suffix = np.concatenate([np.cumsum((g * wp)[::-1])[::-1][1:], [0.0]])  # sum_{b' > b} g wp
         grad[r] = D * (g * Tn - suffix)
     return loss, grad
+
+
+def proposal_sample64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9):
+    """Samples placed by the proposal, per ray: bin weights w_b = T_b (1 - exp(-sigma_b D))
+    over 64 equal bins of [t0, t1); mass M = sum w_b; rays with t1 <= t0 get no samples,
+    rays with M < mass_threshold get none either and are marked for the occupancy-grid
+    fallback.  Otherwise sample j sits at the (j + 1/2) / n quantile of the piecewise
+    constant density w_b / (M D): t_j = t0 + (b + (u - C_b) / p_b) D with p_b = w_b / M,
+    C_b = sum_{b' < b} p_b', b the last bin with C_b <= u; ds_j = t_{j+1} - t_j and, for
+    the last sample, min(D / (n p_b), t1 - t_j).
+    Returns (counts [n_rays], fallback [n_rays], t [n_rays, n], ds [n_rays, n])."""
+    n_rays = len(t0)
+    sig = np.asarray(sigma_p, np.float64).reshape(n_rays, PROP_BINS)
+    counts = np.zeros(n_rays, np.int64)
+    fallback = np.zeros(n_rays, bool)
+    t = np.zeros((n_rays, n_samples))
+    ds = np.zeros((n_rays, n_samples))
+    for r in range(n_rays):
+        a0, a1 = float(t0[r]), float(t1[r])
+        if not a1 > a0:
+            continue
+        D = (a1 - a0) / PROP_BINS
+        a = 1.0 - np.exp(-sig[r] * D)
+        T = np.concatenate([[1.0], np.cumprod(1.0 - a)[:-1]])
+        w = T * a
+        M = w.sum()
+        if not M >= mass_threshold:
+            fallback[r] = True
+            continue
+        p = w / M
+        C = np.concatenate([[0.0], np.cumsum(p)[:-1]])
+        for j in range(n_samples):
+            u = (j + 0.5) / n_samples
+            b = int(np.nonzero(C <= u)[0].max())
+            frac = min(max((u - C[b]) / p[b], 0.0), 1.0) if p[b] > 0 else 0.5
+            t[r, j] = a0 + (b + frac) * D
+        for j in range(n_samples - 1):
+            ds[r, j] = t[r, j + 1] - t[r, j]
+        bl = int(np.nonzero(C <= (n_samples - 0.5) / n_samples)[0].max())
+        ds[r, -1] = min(D / (n_samples * p[bl]) if p[bl] > 0 else D, a1 - t[r, -1])
+        counts[r] = n_samples
+    return counts, fallback, t, ds

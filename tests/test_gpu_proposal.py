@@ -7,6 +7,10 @@ Tolerances (f32 arithmetic against float64):
   MLP sigma:         |a - b| <= 1e-5 * |b|
   MLP gradients:     |a - b| <= 1e-5 * max|b| + 1e-6 * |b|  (sums over up to 2^17 points)
   loss:              relative 1e-4;  dL/dsigma_p: |a - b| <= 1e-4 * max|b|
+  sampling:          counts and fallback flags exact; sample distances within
+                     2e-5 of the segment length; ds = forward differences exactly
+                     (within 1e-6 of the segment); the last sample's ds within 1e-3
+                     relative for 99% of rays (its bin can differ at a CDF boundary)
 """
 import numpy as np
 import pytest
@@ -86,3 +90,46 @@ def test_proposal_loss(dev, seed, n_rays):
     assert abs(got_l - want_l) <= 1e-4 * want_l, (got_l, want_l)
     assert (np.abs(got_g - want_g) <= 1e-4 * np.abs(want_g).max()).all(), np.abs(got_g - want_g).max()
     assert (got_g[~valid] == 0).all()
+
+
+def test_proposal_sample(dev):
+    rng = np.random.default_rng(11)
+    n, N = 2000, 48
+    o = rng.normal(size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    t0 = rng.uniform(0, 2, n).astype(np.float32)
+    t1 = (t0 + rng.uniform(0.5, 3, n)).astype(np.float32)
+    t1[::97] = t0[::97]  # empty segments
+    # densities: a few bumps per ray, some rays nearly transparent (fallback)
+    bins = np.arange(64)[None, :]
+    c = rng.uniform(0, 64, (n, 1))
+    sig = (rng.uniform(5, 400, (n, 1)) * np.exp(-0.5 * ((bins - c) / rng.uniform(1, 8, (n, 1))) ** 2)).astype(np.float32)
+    sig[::7] *= 0.01
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    start = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    cnt = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    fb = torch.empty(n, dtype=torch.bool, device=dev)
+    xyz = torch.empty(n, N, 3, device=dev)
+    ds = torch.empty(n, N, device=dev)
+    z = torch.empty(n, N, device=dev)
+    _lib.call("proposal_sample", [T(o), T(d), T(t0), T(t1), T(sig), start, cnt, fb, xyz, ds, z],
+              _lib.proposal_sample_descriptor(n, N, 0.9))
+    want_c, want_fb, want_t, want_ds = oracle.proposal_sample64(t0, t1, sig, N, 0.9)
+    # masses within 1e-6 of the threshold could round either way: none here
+    got_c = cnt.view(torch.int32).cpu().numpy()
+    np.testing.assert_array_equal(got_c, want_c)
+    np.testing.assert_array_equal(fb.cpu().numpy(), want_fb)
+    np.testing.assert_array_equal(start.view(torch.int32).cpu().numpy(), np.arange(n) * N)
+    ok = want_c > 0
+    assert ok.sum() > n // 2 and want_fb.sum() > 100
+    zt = z.cpu().numpy()
+    span = (t1 - t0)[:, None]
+    assert (np.abs(zt[ok] - want_t[ok]) <= 2e-5 * span[ok] + 1e-6).all(), np.abs(zt[ok] - want_t[ok]).max()
+    assert (zt[~ok] == 0).all() and (ds.cpu().numpy()[~ok] == 0).all()
+    np.testing.assert_allclose(xyz.cpu().numpy()[ok], (o[:, None, :] + zt[:, :, None] * d[:, None, :])[ok],
+                               rtol=0, atol=1e-5)
+    # ds: forward differences of the sample positions (and the last sample's bin spacing)
+    dsg = ds.cpu().numpy()
+    assert (np.abs(dsg[ok][:, :-1] - np.diff(zt[ok], axis=1)) <= 1e-6 * span[ok]).all()
+    assert (np.abs(dsg[ok][:, -1] - want_ds[ok][:, -1]) <= 1e-3 * np.abs(want_ds[ok][:, -1]) + 1e-6).mean() > 0.99

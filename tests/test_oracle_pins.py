@@ -600,3 +600,29 @@ def test_proposal_loss_gradient_and_closed_forms():
     l0, g0 = oracle.proposal_loss64(start, ns, dss, z, np.concatenate([empty[:, None], drgbs[:, 1:]], 1), t0, t1,
                                     sig, valid, 4)
     assert l0 == 0 and (g0 == 0).all()
+
+
+def test_proposal_sample_quantiles():
+    """Closed forms of the quantile placement: a constant density (sigma tiny, so the
+    bin weights are nearly equal) puts the samples at evenly spaced quantiles of the
+    segment; mass below the threshold marks the fallback; the samples are sorted and
+    the first and last lie inside the bins holding the 1/(2n) and 1 - 1/(2n) quantiles."""
+    n = 48
+    t0, t1 = np.array([2.0, 2.0, 1.0]), np.array([4.0, 2.0, 1.5])
+    sig = np.zeros((3, 64))
+    sig[0] = 400.0  # opaque: mass ~ 1, concentrated in the first bins
+    sig[2] = 0.5  # nearly transparent: mass << 0.9
+    counts, fb, t, ds = oracle.proposal_sample64(t0, t1, sig, n)
+    assert list(counts) == [n, 0, 0] and list(fb) == [False, False, True]
+    assert (np.diff(t[0]) >= 0).all() and t[0, 0] >= 2.0 and t[0, -1] <= 4.0
+    # uniform weights: sigma such that every bin holds the same weight is not constant;
+    # instead use the exact inverse: w_b = 1/64 needs T_b a_b = 1/64
+    w = np.full(64, 1.0 / 64)
+    T = 1.0 - np.concatenate([[0.0], np.cumsum(w)[:-1]])
+    a = np.minimum(w / T, 1.0)
+    D = 2.0 / 64
+    sig_u = -np.log1p(-np.clip(a, 0, 1 - 1e-15)) / D
+    counts, fb, t, ds = oracle.proposal_sample64(np.array([2.0]), np.array([4.0]), sig_u[None], n)
+    want = 2.0 + (np.arange(n) + 0.5) / n * 2.0
+    assert np.abs(t[0] - want).max() <= 1e-9
+    assert np.abs(ds[0, :-1] - 2.0 / n).max() <= 1e-9
