@@ -111,10 +111,29 @@ def render_image_inference(model, camera: Camera, pose: torch.Tensor, occupancy_
     buffers are allocated once per frame and every launch is pre-bound; the field
     runs fused (nerf_field_chunks) on the marched samples only.
     """
-    dev = pose.device
     o, d = make_rays_worldspace(camera, pose)
+    rays_rgbd, rays_bg = render_rays_inference(model, o, d, occupancy_bitfield, bg, K=K, G=G,
+                                               diagonal_n_steps=diagonal_n_steps, stepsize_portion=stepsize_portion,
+                                               march_steps_cap=march_steps_cap, pool=pool)
+    return _image(camera, rays_rgbd, rays_bg)
+
+
+def _image(camera: Camera, rays_rgbd: torch.Tensor, rays_bg: torch.Tensor):
+    rgb = f32_to_u8(rays_rgbd[:, :3]).reshape(camera.height, camera.width, 3)
+    depth = rays_rgbd[:, 3]
+    depth_u8 = f32_to_u8((depth - depth.min()) / (depth.max() - depth.min() + 1e-15)).reshape(camera.height,
+                                                                                              camera.width)
+    return rays_bg.reshape(camera.height, camera.width, 3), rgb, depth_u8
+
+
+@torch.no_grad()
+def render_rays_inference(model, o: torch.Tensor, d: torch.Tensor, occupancy_bitfield, bg=(1.0, 1.0, 1.0), *, K=1,
+                          G=128, diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8, pool: int = 1 << 18):
+    """The ray-pool loop of render_image_inference on explicit rays o, d [n, 3].
+    Returns (rays_rgbd [n, 4], rays_bg [n, 3])."""
+    dev = o.device
     t_starts, t_ends = make_near_far_from_bound(model.bound, o, d)
-    n_px = camera.n_pixels
+    n_px = o.shape[0]
     rays_rgbd = torch.zeros(n_px, 4, device=dev)
     rays_T = torch.ones(n_px, device=dev)
     rays_bg = torch.as_tensor(bg, dtype=torch.float32, device=dev).expand(n_px, 3).contiguous()
@@ -167,8 +186,56 @@ def render_image_inference(model, camera: Camera, pose: torch.Tensor, occupancy_
             r += 1
         _lib.check()
         n_rendered = int(total.item())
-    rgb = f32_to_u8(rays_rgbd[:, :3]).reshape(camera.height, camera.width, 3)
-    depth = rays_rgbd[:, 3]
-    depth_u8 = f32_to_u8((depth - depth.min()) / (depth.max() - depth.min() + 1e-15)).reshape(camera.height,
-                                                                                              camera.width)
-    return rays_bg.reshape(camera.height, camera.width, 3), rgb, depth_u8
+    return rays_rgbd, rays_bg
+
+
+@torch.no_grad()
+def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, occupancy_bitfield,
+                          bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.9, K=1, G=128,
+                          diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8, pool: int = 1 << 18,
+                          stats: dict | None = None):
+    """Renders one frame with samples placed by the learned proposal: its density at
+    64 bins of every ray gives 48 samples at the quantiles of the ray's mass
+    (proposal_sample); the field runs on them (nerf_field_chunks) and integrate_rays
+    composites them.  Rays whose predicted mass is below `mass_threshold` are rendered
+    by the occupancy-grid ray pool (render_rays_inference) instead.  Same return as
+    render_image_inference; `stats` (if given) receives the fallback count."""
+    dev = pose.device
+    o, d = make_rays_worldspace(camera, pose)
+    t0, t1 = make_near_far_from_bound(model.bound, o, d)
+    n = o.shape[0]
+    sig = proposal.bin_density(o, d, t0, t1)
+    u32 = torch.uint32
+    start = torch.empty(n, dtype=u32, device=dev)
+    cnt = torch.empty(n, dtype=u32, device=dev)
+    fb = torch.empty(n, dtype=torch.bool, device=dev)
+    xyzs = torch.empty(n, n_per_ray, 3, device=dev)
+    dss = torch.empty(n, n_per_ray, device=dev)
+    zs = torch.empty(n, n_per_ray, device=dev)
+    _lib.call("proposal_sample", [o, d, t0, t1, sig, start, cnt, fb, xyzs, dss, zs],
+              _lib.proposal_sample_descriptor(n, n_per_ray, mass_threshold))
+    drgbs = torch.empty(n, n_per_ray, 4, device=dev)
+    pe = model.position_encoder
+    half = getattr(model, "mlp_half", None) is not None
+    params = model.mlp_half if half else model.mlp_params.detach()
+    ident = torch.arange(n, dtype=torch.int32, device=dev).view(u32)
+    _lib.call("nerf_field_chunks", [xyzs, cnt, ident, d, pe.table.detach(), params, drgbs],
+              _lib.field_chunks_descriptor(pe.descriptor(0, model.bound), n, n_per_ray, n,
+                                           _lib.DTYPE_F16 if half else _lib.DTYPE_F32))
+    bgs = torch.as_tensor(bg, dtype=torch.float32, device=dev).expand(n, 3).contiguous()
+    rays_rgbd = torch.empty(n, 4, device=dev)
+    op = torch.empty(n, device=dev)
+    measured = torch.empty(1, dtype=u32, device=dev)
+    _lib.call("integrate_rays", [start, cnt, bgs, dss.reshape(-1), zs.reshape(-1), drgbs.reshape(-1, 4), measured,
+                                 rays_rgbd, op], _lib.integrating_descriptor(n, n * n_per_ray))
+    idx = fb.nonzero().squeeze(1)
+    if stats is not None:
+        stats["fallback_rays"] = int(idx.numel())
+        stats["proposal_rays"] = int((cnt.view(torch.int32) > 0).sum().item())
+    if idx.numel():
+        rgbd_fb, _ = render_rays_inference(model, o[idx].contiguous(), d[idx].contiguous(), occupancy_bitfield, bg,
+                                           K=K, G=G, diagonal_n_steps=diagonal_n_steps,
+                                           stepsize_portion=stepsize_portion, march_steps_cap=march_steps_cap,
+                                           pool=pool)
+        rays_rgbd[idx] = rgbd_fb
+    return _image(camera, rays_rgbd, bgs)

@@ -48,6 +48,10 @@ class TrainConfig:
     precision: str = "fp32"
     loss_scale_init: float = 2.0 ** 15
     loss_scale_growth_interval: int = 2000
+    # learned proposal trained online on the marcher's sample weights of the first
+    # `proposal_rays` rays of every batch (the valid rays of a batch come first)
+    proposal: bool = False
+    proposal_rays: int = 2048
 
 
 def lr_at(step: int, lr: float) -> float:
@@ -130,6 +134,25 @@ class Trainer:
         if self.fp16:
             self.model.mlp_half = self.model.mlp_params.detach().half()
             self.loss_state = _lib.loss_scale_state(self.dev, cfg.loss_scale_init)
+        self.prop = None
+        if cfg.proposal:
+            from .proposal import BINS, ProposalNet
+            self.prop = ProposalNet(self.bound, self.dev, seed=cfg.seed % 2 ** 31 + 1, table_dtype=table_dtype)
+            R = min(cfg.proposal_rays, n)
+            P = R * BINS
+            pt = self.prop.position_encoder.table
+            self.prop_master = pt.detach().float().clone()
+            self.bp = dict(pts=z(P, 3), enc=z(5, P, 2), sig=z(P), dsig=z(P), denc=z(5, P, 2),
+                           dparams=z(_lib.PROPOSAL_N_PARAMS), ws=z(max(1, min(512, (P + 255) // 256)),
+                                                                   _lib.PROPOSAL_N_PARAMS),
+                           dtable=z(*pt.shape), loss=z(1),
+                           tm=torch.zeros_like(self.prop_master), tv=torch.zeros_like(self.prop_master),
+                           mm=z(_lib.PROPOSAL_N_PARAMS), mv=z(_lib.PROPOSAL_N_PARAMS))
+            self.bp["hg_ws"] = torch.empty(_lib.hashgrid_backward_workspace_bytes(
+                self.prop.position_encoder.descriptor(P, self.bound, _lib.ENC_LEVEL_MAJOR)), dtype=torch.uint8,
+                device=self.dev)
+            if self.fp16:
+                self.prop.mlp_half = self.prop.mlp_params.detach().half()
         self._calls = None
         self.mark_untrained_density_grid()
 
@@ -283,6 +306,31 @@ class Trainer:
             mirror=mirror is not None,
             flags=flags,
         )
+        if self.prop is not None:
+            bp, pr = self.bp, self.prop
+            R = min(c.proposal_rays, n)
+            Pn = R * _lib.PROPOSAL_BINS
+            ppe = pr.position_encoder
+            ptab = ppe.table.detach()
+            pdesc = ppe.descriptor(Pn, self.bound, LM)
+            pw, ppd = pr.weights()
+            pmirror = ptab if ptab.dtype == torch.float16 else None
+            calls.update(
+                p_points=P("proposal_points", [b["o"][:R], b["d"][:R], b["t0"][:R], b["t1"][:R], bp["pts"]],
+                           _lib.proposal_rays_descriptor(R)),
+                p_encode=P("hashgrid_encode", [bp["pts"], ptab, bp["enc"]], pdesc),
+                p_mlp=P("proposal_mlp_forward", [bp["enc"], pw, bp["sig"]], _lib.proposal_mlp_descriptor(Pn, 0, LM, ppd)),
+                p_loss=P("proposal_loss", [b["start"][:R], b["ns"][:R], b["dss"], b["zs"], b["drgbs"], b["t0"][:R],
+                                           b["t1"][:R], bp["sig"], b["valid"][:R], b["n_valid"], bp["dsig"],
+                                           bp["loss"]], _lib.proposal_loss_descriptor(R)),
+                p_mlp_bwd=P("proposal_mlp_backward", [bp["enc"], pw, bp["dsig"], bp["denc"], bp["dparams"], bp["ws"]],
+                            _lib.proposal_mlp_descriptor(Pn, bp["ws"].shape[0], LM, ppd)),
+                p_encode_bwd=P("hashgrid_encode_backward", [bp["pts"], bp["denc"], bp["dtable"], bp["hg_ws"]], pdesc),
+                p_adam_table=P("adam_update", [ptab if pmirror is None else self.prop_master, bp["dtable"], bp["tm"],
+                                               bp["tv"], pmirror if pmirror is not None else none], b""),
+                p_adam_mlp=P("adam_update", [pr.mlp_params.detach(), bp["dparams"], bp["mm"], bp["mv"],
+                                             pr.mlp_half if pr.mlp_half is not None else none], b""),
+            )
         if self.fp16:
             calls["check_table"] = P("loss_scale_check", [b["dtable"], self.loss_state],
                                      _lib.loss_scale_check_descriptor(b["dtable"].numel()))
@@ -325,6 +373,15 @@ class Trainer:
                                            flags=k["flags"]))
         if self.fp16:
             k["scale_update"]()
+        if self.prop is not None:
+            for name in ("p_points", "p_encode", "p_mlp", "p_loss", "p_mlp_bwd", "p_encode_bwd"):
+                k[name]()
+            pm16 = self.prop.position_encoder.table.dtype == torch.float16
+            k["p_adam_table"](_lib.adam_descriptor(k["p_adam_table"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, bp1, bp2,
+                                                   0.0, mirror_dtype=_lib.DTYPE_F16 if pm16 else _lib.DTYPE_F32))
+            k["p_adam_mlp"](_lib.adam_descriptor(_lib.PROPOSAL_N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
+                                                 mirror_dtype=_lib.DTYPE_F16 if self.prop.mlp_half is not None
+                                                 else _lib.DTYPE_F32))
         _lib.check()
         self.maybe_update_grid()
 
@@ -338,4 +395,6 @@ class Trainer:
                    measured_batch_size=u(b["measured"]))
         if self.fp16:
             out["loss_scale"] = float(self.loss_state[:4].view(torch.float32)[0].item())
+        if self.prop is not None:
+            out["proposal_loss"] = float(self.bp["loss"][0].item())
         return out
