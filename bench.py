@@ -132,14 +132,17 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
 POOL = 1 << 18  # ray slots of the inference renderer (the reference uses 8192; same image)
 
 
-def bench_inference(tr, data, n_frames: int, dev) -> dict:
+def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dict:
     """ms per 800x800 frame of the trained field from held-out poses, with the
     reference's ray-pool renderer (march_steps_cap 8, white background) on a pool of
     2^18 slots, median over the timed frames after two warm-up renders; PSNR of the
-    u8 render against the float64 ground truth blended on white."""
+    u8 render against the float64 ground truth blended on white.  With `proposal`,
+    the frames are rendered with 48 proposal-placed samples per ray (occupancy-grid
+    fallback below mass 0.9) and the same field is also rendered the plain way, for
+    the PSNR difference."""
     import torch
     from jaxngp_amd import scene as sc
-    from jaxngp_amd.renderer import render_image_inference
+    from jaxngp_amd.renderer import render_image_inference, render_image_proposal
 
     poses = sc.look_at_poses(n_frames, seed=2, golden=False)
     gt = sc.make_views(data.scene, data.camera, poses, dev)
@@ -147,21 +150,39 @@ def bench_inference(tr, data, n_frames: int, dev) -> dict:
     kw = dict(K=tr.K, G=c.grid_res, diagonal_n_steps=c.diagonal_n_steps, stepsize_portion=tr.portion,
               march_steps_cap=8, pool=POOL)
     pose_t = [torch.as_tensor(p, dtype=torch.float32, device=dev) for p in poses]
+    if proposal:
+        render = lambda pose, st=None: render_image_proposal(tr.model, tr.prop, data.camera, pose, tr.grid.bitfield,
+                                                             stats=st, **kw)
+    else:
+        render = lambda pose, st=None: render_image_inference(tr.model, data.camera, pose, tr.grid.bitfield, **kw)
     for i in range(2):
-        render_image_inference(tr.model, data.camera, pose_t[i % n_frames], tr.grid.bitfield, **kw)
-    times, psnrs = [], []
+        render(pose_t[i % n_frames])
+    times, psnrs, plain_psnrs, fallback = [], [], [], []
     for i in range(n_frames):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        _, rgb, _ = render_image_inference(tr.model, data.camera, pose_t[i], tr.grid.bitfield, **kw)
+        st = {}
+        _, rgb, _ = render(pose_t[i], st)
         torch.cuda.synchronize()
         times.append((time.perf_counter() - t) * 1e3)
-        psnrs.append(sc.psnr_u8(rgb, sc.blend_u8(gt.rgba[i], (1.0, 1.0, 1.0))))
+        ref = sc.blend_u8(gt.rgba[i], (1.0, 1.0, 1.0))
+        psnrs.append(sc.psnr_u8(rgb, ref))
+        if proposal:
+            fallback.append(st["fallback_rays"] / (st["fallback_rays"] + st["proposal_rays"]))
+            _, rgb_plain, _ = render_image_inference(tr.model, data.camera, pose_t[i], tr.grid.bitfield, **kw)
+            plain_psnrs.append(sc.psnr_u8(rgb_plain, ref))
     times.sort()
-    log(f"inference: {times[len(times) // 2]:.2f} ms/frame (median of {n_frames}), PSNR {np.mean(psnrs):.2f} dB")
-    return {"ms_per_frame": times[len(times) // 2], "frames": n_frames, "psnr_db": float(np.mean(psnrs)),
-            "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
-            "train_steps": tr.step_count}
+    res = {"ms_per_frame": times[len(times) // 2], "frames": n_frames, "psnr_db": float(np.mean(psnrs)),
+           "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
+           "train_steps": tr.step_count}
+    if proposal:
+        res.update(samples_per_ray=48, mass_threshold=0.9, fallback_ray_fraction=float(np.mean(fallback)),
+                   psnr_plain_render_db=float(np.mean(plain_psnrs)),
+                   delta_psnr_db=float(np.mean(psnrs) - np.mean(plain_psnrs)))
+    log(f"inference{' (proposal)' if proposal else ''}: {res['ms_per_frame']:.2f} ms/frame (median of {n_frames}), "
+        f"PSNR {res['psnr_db']:.2f} dB" + (f", plain render of the same field {res['psnr_plain_render_db']:.2f} dB, "
+                                           f"fallback rays {res['fallback_ray_fraction']:.1%}" if proposal else ""))
+    return res
 
 
 def main() -> None:
@@ -174,6 +195,7 @@ def main() -> None:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--frames", type=int, default=5, help="timed 800x800 inference frames (held-out poses)")
     ap.add_argument("--no-fp16", action="store_true", help="skip the f16-weight precision path")
+    ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 5 times (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -268,6 +290,16 @@ def main() -> None:
     }
     if args.frames > 0:
         out["inference"] = bench_inference(tr, data, args.frames, dev)
+    if not args.no_proposal:
+        # the learned-proposal path on the precision path: trained online with the field,
+        # rendering 48 proposal-placed samples per ray (occupancy-grid fallback)
+        rp = run_training(data, TrainConfig(precision="fp16", proposal=True), args.steps, args.warmup, dev)
+        tp = rp["trainer"]
+        out["proposal_fp16"] = {"value": rp["samples"] / rp["elapsed"], "unit": "samples/s",
+                                "ms_per_step": rp["ms"], "loss": tp.metrics()["loss"],
+                                "proposal_loss": tp.metrics()["proposal_loss"]}
+        if args.frames > 0:
+            out["proposal_fp16"]["inference"] = bench_inference(tp, data, args.frames, dev, proposal=True)
     if not args.no_fp16:
         # the precision path on the same scene: f16 table and MLP weights, loss scaling
         r16 = run_training(data, TrainConfig(precision="fp16"), args.steps, args.warmup, dev)

@@ -314,19 +314,32 @@ typedef struct ProposalLossDescriptor {
  * out: dL_dsigma_p f32[n, 64], loss f32[1] */
 void proposal_loss(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
+#define NGP_PROPOSAL_QUANTILES 0u /* samples at the quantiles of the bin weights */
+#define NGP_PROPOSAL_SUPPORT 1u   /* samples evenly over the bins that hold the mass */
+
 typedef struct ProposalSampleDescriptor {
     uint32_t n_rays;
     uint32_t n_per_ray;   /* samples per ray (48), at most 64 */
     float mass_threshold; /* below it the ray gets no samples and is marked (0.9) */
+    uint32_t mode;        /* NGP_PROPOSAL_QUANTILES or NGP_PROPOSAL_SUPPORT */
+    float eps;            /* SUPPORT: mass share left out at either end (1e-3) */
+    /* marching statics of the occupancy grid (as MarchingDescriptor) */
+    uint32_t diagonal_n_steps, K, G;
+    float bound, stepsize_portion;
 } ProposalSampleDescriptor;
 
 /* Samples placed by the proposal: per ray, the bin weights w_b of sigma_p give the
  * mass M = sum w_b; a ray with t_end > t_start and M >= mass_threshold gets n_per_ray
  * samples at the (j + 1/2) / n_per_ray quantiles of the piecewise constant density
- * w_b / (M D), ds_j = t_{j+1} - t_j (the last: min(D / (n p_b), t_end - t_j)); a ray
+ * w_b / (M D), ds_j = t_{j+1} - t_j (the last: min(D / (n p_b), t_end - t_j)), or
+ * (SUPPORT) evenly over the bins from the first whose cumulative share exceeds eps
+ * to the first where it reaches 1 - eps, ds = the spacing.  A sample whose cell the
+ * occupancy grid marks empty gets ds = 0, so the field there adds nothing, as
+ * march_rays never samples such cells.  A ray
  * below the threshold gets none and fallback = true (the occupancy-grid marcher
  * renders it).  Sample arrays use [n_rays, n_per_ray] slots, startidx = r n_per_ray.
- * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n], sigma_p f32[n,64]
+ * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n], sigma_p f32[n,64],
+ *      occupancy_bitfield u8[K*G^3/8]
  * out: rays_sample_startidx u32[n], rays_n_samples u32[n], fallback bool[n],
  *      xyzs f32[n,N,3], dss f32[n,N], z_vals f32[n,N] */
 void proposal_sample(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);

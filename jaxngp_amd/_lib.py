@@ -267,8 +267,14 @@ def proposal_mlp_descriptor(n_points, n_workspace_blocks=0, enc_layout=ENC_POINT
     return struct.pack("<4I", n_points, n_workspace_blocks, enc_layout, param_dtype)
 
 
-def proposal_sample_descriptor(n_rays, n_per_ray=48, mass_threshold=0.9) -> bytes:
-    return struct.pack("<2If", n_rays, n_per_ray, mass_threshold)
+PROPOSAL_QUANTILES = 0
+PROPOSAL_SUPPORT = 1
+
+
+def proposal_sample_descriptor(n_rays, n_per_ray=48, mass_threshold=0.9, mode=PROPOSAL_QUANTILES, eps=1e-3,
+                               diagonal_n_steps=1024, K=1, G=128, bound=1.0, stepsize_portion=0.0) -> bytes:
+    return struct.pack("<2IfIf3I2f", n_rays, n_per_ray, mass_threshold, mode, eps, diagonal_n_steps, K, G, bound,
+                       stepsize_portion)
 
 
 def proposal_loss_descriptor(n_rays, eps=1e-7) -> bytes:

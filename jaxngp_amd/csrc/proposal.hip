@@ -13,6 +13,7 @@
 #include <hip/hip_fp16.h>
 
 #include "ngp_common.h"
+#include "ngp_device.h"
 
 namespace ngp {
 namespace {
@@ -236,10 +237,11 @@ __device__ __forceinline__ float wave_prefix_sum(float v) {
 
 // One wavefront per ray, lane b = bin b; lanes j < n_per_ray place sample j.
 __global__ void __launch_bounds__(kBlock)
-    proposal_sample_kernel(uint32_t n_rays, uint32_t n_per_ray, float mass_threshold,
-                           const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+    proposal_sample_kernel(uint32_t n_rays, uint32_t n_per_ray, float mass_threshold, uint32_t mode, float eps,
+                           MarchParams mp, const float *__restrict__ rays_o, const float *__restrict__ rays_d,
                            const float *__restrict__ t0s, const float *__restrict__ t1s,
-                           const float *__restrict__ sigma_p, uint32_t *__restrict__ startidx,
+                           const float *__restrict__ sigma_p, const uint8_t *__restrict__ bits,
+                           uint32_t *__restrict__ startidx,
                            uint32_t *__restrict__ counts, bool *__restrict__ fallback, float *__restrict__ xyzs,
                            float *__restrict__ dss, float *__restrict__ zs) {
     const int wave = (int)threadIdx.x / kWave, lane = lane_id();
@@ -262,7 +264,17 @@ __global__ void __launch_bounds__(kBlock)
         fallback[r] = seg && !ok;
     }
     float t = 0.0f, ds = 0.0f;
-    if (ok) {
+    if (ok && mode == NGP_PROPOSAL_SUPPORT) {
+        // bins [lo, hi]: first with cumulative share > eps, first with >= 1 - eps
+        const float inc = wave_prefix_sum(w / M);
+        const uint64_t above = __ballot(inc > eps), full = __ballot(inc >= 1.0f - eps);
+        const int lo = above ? __builtin_ctzll(above) : 0;
+        int hi = full ? __builtin_ctzll(full) : kBins - 1;
+        hi = hi > lo ? hi : lo;
+        const float t_lo = t0 + (float)lo * D, t_hi = t0 + (float)(hi + 1) * D;
+        ds = (t_hi - t_lo) / (float)n_per_ray;
+        t = t_lo + ((float)lane + 0.5f) * ds;
+    } else if (ok) {
         const float p = w / M;
         const float C = wave_prefix_sum(p) - p;  // exclusive
         const float u = ((float)lane + 0.5f) / (float)n_per_ray;
@@ -283,8 +295,18 @@ __global__ void __launch_bounds__(kBlock)
     }
     if ((uint32_t)lane < n_per_ray) {
         const size_t j = base + lane;
+        float p[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xyzs[3 * j + c] = ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
+        for (int c = 0; c < 3; ++c) p[c] = ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
+        if (ok) {
+            // the occupancy grid judges the field as in march_rays: a sample in a cell
+            // the grid marks empty contributes nothing (ds = 0)
+            BitCache cache = make_bit_cache(bits, false);
+            float gx, gy, gz, mb;
+            if (!lookup_cell(mp, bits, cache, p[0], p[1], p[2], step_size(mp, t), gx, gy, gz, mb)) ds = 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xyzs[3 * j + c] = p[c];
         dss[j] = ds;
         zs[j] = t;
     }
@@ -392,16 +414,27 @@ extern "C" void proposal_sample(ngp_stream_t stream, void **buffers, const char 
     const char *target = "proposal_sample";
     ProposalSampleDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
-    if (d.n_per_ray == 0 || d.n_per_ray > kBins || !(d.mass_threshold >= 0.0f)) {
-        set_error(kBadDescriptor, "%s: n_per_ray must be in [1, %d] and mass_threshold >= 0", target, kBins);
+    if (d.n_per_ray == 0 || d.n_per_ray > kBins || !(d.mass_threshold >= 0.0f) ||
+        d.mode > NGP_PROPOSAL_SUPPORT || !(d.eps >= 0.0f && d.eps < 0.5f)) {
+        set_error(kBadDescriptor, "%s: n_per_ray must be in [1, %d], mass_threshold >= 0, mode 0 or 1, eps in "
+                  "[0, 0.5)", target, kBins);
+        return;
+    }
+    const uint32_t G = d.G;
+    const bool g_ok = G != 0 && G <= 1024 && (G & (G - 1)) == 0 && (uint64_t)d.K * G * G * G <= 0xffffffffull;
+    if (d.diagonal_n_steps == 0 || d.K == 0 || !g_ok || !(d.bound > 0.0f) || !(d.stepsize_portion >= 0.0f)) {
+        set_error(kBadDescriptor, "%s: invalid marching statics (need diagonal_n_steps, K > 0, G a power of two "
+                  "<= 1024, bound > 0, portion >= 0)", target);
         return;
     }
     if (d.n_rays == 0) return;
-    if (!check_buffers(buffers, 11, target)) return;
+    if (!check_buffers(buffers, 12, target)) return;
+    const MarchParams mp = make_march_params(d.diagonal_n_steps, d.K, d.G, d.bound, d.stepsize_portion);
     hipLaunchKernelGGL(proposal_sample_kernel, dim3(div_up(d.n_rays, kBlock / kWave)), dim3(kBlock), 0,
-                       (hipStream_t)stream, d.n_rays, d.n_per_ray, d.mass_threshold, (const float *)buffers[0],
-                       (const float *)buffers[1], (const float *)buffers[2], (const float *)buffers[3],
-                       (const float *)buffers[4], (uint32_t *)buffers[5], (uint32_t *)buffers[6], (bool *)buffers[7],
-                       (float *)buffers[8], (float *)buffers[9], (float *)buffers[10]);
+                       (hipStream_t)stream, d.n_rays, d.n_per_ray, d.mass_threshold, d.mode, d.eps, mp,
+                       (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
+                       (const float *)buffers[3], (const float *)buffers[4], (const uint8_t *)buffers[5],
+                       (uint32_t *)buffers[6], (uint32_t *)buffers[7], (bool *)buffers[8], (float *)buffers[9],
+                       (float *)buffers[10], (float *)buffers[11]);
     check_launch(target);
 }

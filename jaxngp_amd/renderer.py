@@ -191,13 +191,15 @@ def render_rays_inference(model, o: torch.Tensor, d: torch.Tensor, occupancy_bit
 
 @torch.no_grad()
 def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, occupancy_bitfield,
-                          bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.9, K=1, G=128,
+                          bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.9,
+                          placement: int = _lib.PROPOSAL_SUPPORT, K=1, G=128,
                           diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8, pool: int = 1 << 18,
                           stats: dict | None = None):
     """Renders one frame with samples placed by the learned proposal: its density at
-    64 bins of every ray gives 48 samples at the quantiles of the ray's mass
-    (proposal_sample); the field runs on them (nerf_field_chunks) and integrate_rays
-    composites them.  Rays whose predicted mass is below `mass_threshold` are rendered
+    64 bins of every ray gives 48 samples, by default evenly over the bins that hold
+    the ray's mass (placement PROPOSAL_SUPPORT; PROPOSAL_QUANTILES puts them at the
+    quantiles of the mass instead) (proposal_sample); the field runs on them
+    (nerf_field_chunks) and integrate_rays composites them.  Rays whose predicted mass is below `mass_threshold` are rendered
     by the occupancy-grid ray pool (render_rays_inference) instead.  Same return as
     render_image_inference; `stats` (if given) receives the fallback count."""
     dev = pose.device
@@ -212,8 +214,9 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     xyzs = torch.empty(n, n_per_ray, 3, device=dev)
     dss = torch.empty(n, n_per_ray, device=dev)
     zs = torch.empty(n, n_per_ray, device=dev)
-    _lib.call("proposal_sample", [o, d, t0, t1, sig, start, cnt, fb, xyzs, dss, zs],
-              _lib.proposal_sample_descriptor(n, n_per_ray, mass_threshold))
+    _lib.call("proposal_sample", [o, d, t0, t1, sig, occupancy_bitfield, start, cnt, fb, xyzs, dss, zs],
+              _lib.proposal_sample_descriptor(n, n_per_ray, mass_threshold, placement, 1e-3, diagonal_n_steps, K, G,
+                                              model.bound, stepsize_portion))
     drgbs = torch.empty(n, n_per_ray, 4, device=dev)
     pe = model.position_encoder
     half = getattr(model, "mlp_half", None) is not None

@@ -51,7 +51,8 @@ class TrainConfig:
     # learned proposal trained online on the marcher's sample weights of the first
     # `proposal_rays` rays of every batch (the valid rays of a batch come first)
     proposal: bool = False
-    proposal_rays: int = 2048
+    proposal_rays: int = 1024
+    proposal_every: int = 4  # proposal update on every n-th step
 
 
 def lr_at(step: int, lr: float) -> float:
@@ -135,6 +136,7 @@ class Trainer:
             self.model.mlp_half = self.model.mlp_params.detach().half()
             self.loss_state = _lib.loss_scale_state(self.dev, cfg.loss_scale_init)
         self.prop = None
+        self.prop_t = 0
         if cfg.proposal:
             from .proposal import BINS, ProposalNet
             self.prop = ProposalNet(self.bound, self.dev, seed=cfg.seed % 2 ** 31 + 1, table_dtype=table_dtype)
@@ -373,13 +375,15 @@ class Trainer:
                                            flags=k["flags"]))
         if self.fp16:
             k["scale_update"]()
-        if self.prop is not None:
+        if self.prop is not None and (t - 1) % c.proposal_every == 0:
             for name in ("p_points", "p_encode", "p_mlp", "p_loss", "p_mlp_bwd", "p_encode_bwd"):
                 k[name]()
             pm16 = self.prop.position_encoder.table.dtype == torch.float16
-            k["p_adam_table"](_lib.adam_descriptor(k["p_adam_table"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, bp1, bp2,
+            self.prop_t += 1  # the proposal's own update count for Adam's bias corrections
+            q1, q2 = float(np.float32(b1) ** self.prop_t), float(np.float32(b2) ** self.prop_t)
+            k["p_adam_table"](_lib.adam_descriptor(k["p_adam_table"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, q1, q2,
                                                    0.0, mirror_dtype=_lib.DTYPE_F16 if pm16 else _lib.DTYPE_F32))
-            k["p_adam_mlp"](_lib.adam_descriptor(_lib.PROPOSAL_N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
+            k["p_adam_mlp"](_lib.adam_descriptor(_lib.PROPOSAL_N_PARAMS, lr, b1, b2, 1e-15, 1e-15, q1, q2, 1e-6,
                                                  mirror_dtype=_lib.DTYPE_F16 if self.prop.mlp_half is not None
                                                  else _lib.DTYPE_F32))
         _lib.check()

@@ -585,6 +585,42 @@ def proposal_loss64(startidx, nsamp, dss, z_vals, drgbs, t0, t1, sigma_p, valid,
     return loss, grad
 
 
+def proposal_sample_support64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9, eps=1e-3):
+    """Support placement: the bins from the first whose cumulative share C_b + p_b
+    exceeds eps to the first where it reaches 1 - eps hold the ray's mass; n samples
+    sit evenly in that range [t_lo, t_hi), t_j = t_lo + (j + 1/2) (t_hi - t_lo) / n,
+    ds_j = (t_hi - t_lo) / n.  Fallback and empty segments as proposal_sample64.
+    Returns (counts, fallback, t, ds)."""
+    n_rays = len(t0)
+    sig = np.asarray(sigma_p, np.float64).reshape(n_rays, PROP_BINS)
+    counts = np.zeros(n_rays, np.int64)
+    fallback = np.zeros(n_rays, bool)
+    t = np.zeros((n_rays, n_samples))
+    ds = np.zeros((n_rays, n_samples))
+    for r in range(n_rays):
+        a0, a1 = float(t0[r]), float(t1[r])
+        if not a1 > a0:
+            continue
+        D = (a1 - a0) / PROP_BINS
+        a = 1.0 - np.exp(-sig[r] * D)
+        T = np.concatenate([[1.0], np.cumprod(1.0 - a)[:-1]])
+        w = T * a
+        M = w.sum()
+        if not M >= mass_threshold:
+            fallback[r] = True
+            continue
+        inc = np.cumsum(w / M)
+        lo = int(np.argmax(inc > eps))
+        hi = int(np.argmax(inc >= 1.0 - eps)) if (inc >= 1.0 - eps).any() else PROP_BINS - 1
+        hi = max(hi, lo)
+        t_lo, t_hi = a0 + lo * D, a0 + (hi + 1) * D
+        step = (t_hi - t_lo) / n_samples
+        t[r] = t_lo + (np.arange(n_samples) + 0.5) * step
+        ds[r] = step
+        counts[r] = n_samples
+    return counts, fallback, t, ds
+
+
 def proposal_sample64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9):
     """Samples placed by the proposal, per ray: bin weights w_b = T_b (1 - exp(-sigma_b D))
     over 64 equal bins of [t0, t1); mass M = sum w_b; rays with t1 <= t0 get no samples,

@@ -113,7 +113,8 @@ def test_proposal_sample(dev):
     xyz = torch.empty(n, N, 3, device=dev)
     ds = torch.empty(n, N, device=dev)
     z = torch.empty(n, N, device=dev)
-    _lib.call("proposal_sample", [T(o), T(d), T(t0), T(t1), T(sig), start, cnt, fb, xyz, ds, z],
+    full = torch.full((128 ** 3 // 8,), 255, dtype=torch.uint8, device=dev)
+    _lib.call("proposal_sample", [T(o), T(d), T(t0), T(t1), T(sig), full, start, cnt, fb, xyz, ds, z],
               _lib.proposal_sample_descriptor(n, N, 0.9))
     want_c, want_fb, want_t, want_ds = oracle.proposal_sample64(t0, t1, sig, N, 0.9)
     # masses within 1e-6 of the threshold could round either way: none here
@@ -133,3 +134,55 @@ def test_proposal_sample(dev):
     dsg = ds.cpu().numpy()
     assert (np.abs(dsg[ok][:, :-1] - np.diff(zt[ok], axis=1)) <= 1e-6 * span[ok]).all()
     assert (np.abs(dsg[ok][:, -1] - want_ds[ok][:, -1]) <= 1e-3 * np.abs(want_ds[ok][:, -1]) + 1e-6).mean() > 0.99
+
+
+def test_proposal_sample_support_mode(dev):
+    rng = np.random.default_rng(12)
+    n, N = 1500, 48
+    o = rng.normal(size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    t0 = rng.uniform(0, 2, n).astype(np.float32)
+    t1 = (t0 + rng.uniform(0.5, 3, n)).astype(np.float32)
+    bins = np.arange(64)[None, :]
+    c = rng.uniform(0, 64, (n, 1))
+    sig = (rng.uniform(5, 400, (n, 1)) * np.exp(-0.5 * ((bins - c) / rng.uniform(1, 8, (n, 1))) ** 2)).astype(np.float32)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    start = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    cnt = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    fb = torch.empty(n, dtype=torch.bool, device=dev)
+    xyz = torch.empty(n, N, 3, device=dev)
+    ds = torch.empty(n, N, device=dev)
+    z = torch.empty(n, N, device=dev)
+    full = torch.full((128 ** 3 // 8,), 255, dtype=torch.uint8, device=dev)
+    _lib.call("proposal_sample", [T(o), T(d), T(t0), T(t1), T(sig), full, start, cnt, fb, xyz, ds, z],
+              _lib.proposal_sample_descriptor(n, N, 0.9, _lib.PROPOSAL_SUPPORT, 1e-3))
+    want_c, want_fb, want_t, want_ds = oracle.proposal_sample_support64(t0, t1, sig, N, 0.9, 1e-3)
+    np.testing.assert_array_equal(cnt.view(torch.int32).cpu().numpy(), want_c)
+    np.testing.assert_array_equal(fb.cpu().numpy(), want_fb)
+    ok = want_c > 0
+    span = (t1 - t0)[:, None]
+    zt, dsg = z.cpu().numpy(), ds.cpu().numpy()
+    # the support's end bins can differ where a cumulative share sits within f32
+    # rounding of eps or 1 - eps: allow that for 1% of the rays
+    close = (np.abs(zt - want_t) <= 2e-5 * span + 1e-6).all(axis=1) & \
+            (np.abs(dsg - want_ds) <= 2e-5 * span + 1e-7).all(axis=1)
+    assert close[ok].mean() > 0.99, close[ok].mean()
+
+    # occupancy: with a grid whose cells are empty for x < 0, exactly the samples in
+    # those cells lose their ds (grid coordinate floor((x + 1) / 2 * 128) < 64)
+    from tests.helpers import blob_bitfield
+    bits, _ = blob_bitfield(rng, fill=False)
+    xs, ys, zs_ = np.meshgrid(np.arange(128), np.arange(128), np.arange(128), indexing="ij")
+    m = oracle.morton3d(np.stack([xs.ravel(), ys.ravel(), zs_.ravel()], 1).astype(np.uint32))
+    flat = np.zeros(128 ** 3, bool)
+    flat[m] = (xs >= 64).ravel()
+    half = T(np.packbits(flat.reshape(-1, 8), axis=-1, bitorder="little").reshape(-1))
+    ds2 = torch.empty_like(ds)
+    _lib.call("proposal_sample", [T(o), T(d), T(t0), T(t1), T(sig), half, start, cnt, fb, xyz, ds2, z],
+              _lib.proposal_sample_descriptor(n, N, 0.9, _lib.PROPOSAL_SUPPORT, 1e-3))
+    x = xyz.cpu().numpy()[..., 0]
+    ix = np.floor(((x / np.float32(1.0) + np.float32(1.0)) * np.float32(0.5)) * np.float32(128))
+    empty = (ix < 64) & ok[:, None]
+    d2 = ds2.cpu().numpy()
+    assert (d2[empty] == 0).all() and (d2[~empty & ok[:, None]] == dsg[~empty & ok[:, None]]).all()

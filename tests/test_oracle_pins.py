@@ -626,3 +626,22 @@ def test_proposal_sample_quantiles():
     want = 2.0 + (np.arange(n) + 0.5) / n * 2.0
     assert np.abs(t[0] - want).max() <= 1e-9
     assert np.abs(ds[0, :-1] - 2.0 / n).max() <= 1e-9
+
+
+def test_proposal_sample_support():
+    """Support placement: equal bin weights cover the whole segment, so the samples are
+    evenly spaced over [t0, t1); an opaque first bin confines them to that bin."""
+    n = 48
+    w = np.full(64, 1.0 / 64)
+    T = 1.0 - np.concatenate([[0.0], np.cumsum(w)[:-1]])
+    a = np.minimum(w / T, 1.0)
+    D = 2.0 / 64
+    sig_u = -np.log1p(-np.clip(a, 0, 1 - 1e-15)) / D
+    sig_o = np.zeros(64)
+    sig_o[0] = 1e4
+    counts, fb, t, ds = oracle.proposal_sample_support64(np.array([2.0, 2.0]), np.array([4.0, 4.0]),
+                                                         np.stack([sig_u, sig_o]), n)
+    assert list(counts) == [n, n] and not fb.any()
+    assert np.abs(t[0] - (2.0 + (np.arange(n) + 0.5) / n * 2.0)).max() <= 1e-12
+    assert np.abs(ds[0] - 2.0 / n).max() <= 1e-12
+    assert t[1, 0] > 2.0 and t[1, -1] < 2.0 + D and np.abs(ds[1] - D / n).max() <= 1e-12
