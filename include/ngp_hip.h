@@ -333,9 +333,10 @@ typedef struct ProposalSampleDescriptor {
  * samples at the (j + 1/2) / n_per_ray quantiles of the piecewise constant density
  * w_b / (M D), ds_j = t_{j+1} - t_j (the last: min(D / (n p_b), t_end - t_j)), or
  * (SUPPORT) evenly over the bins from the first whose cumulative share exceeds eps
- * to the first where it reaches 1 - eps, ds = the spacing.  A sample whose cell the
- * occupancy grid marks empty gets ds = 0, so the field there adds nothing, as
- * march_rays never samples such cells.  A ray
+ * to the first where it reaches 1 - eps, ds = the spacing.  Samples whose cell the
+ * occupancy grid marks empty are dropped (march_rays never samples such cells): the
+ * others are packed in order at the front of the ray's slots, rays_n_samples of
+ * them, the remaining slots are zero.  A ray
  * below the threshold gets none and fallback = true (the occupancy-grid marcher
  * renders it).  Sample arrays use [n_rays, n_per_ray] slots, startidx = r n_per_ray.
  * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n], sigma_p f32[n,64],
@@ -343,6 +344,21 @@ typedef struct ProposalSampleDescriptor {
  * out: rays_sample_startidx u32[n], rays_n_samples u32[n], fallback bool[n],
  *      xyzs f32[n,N,3], dss f32[n,N], z_vals f32[n,N] */
 void proposal_sample(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct ProposalRenderDescriptor {
+    HashGridDescriptor grid;          /* the proposal's encoder (L = 5, F = 2) */
+    ProposalSampleDescriptor sample;  /* placement, thresholds, marching statics */
+    uint32_t param_dtype;             /* proposal MLP weights: f32 or f16 */
+} ProposalRenderDescriptor;
+
+/* proposal_points + hashgrid_encode + proposal_mlp_forward + proposal_sample fused
+ * for rendering: one wavefront per ray, lane b evaluates the proposal at bin b's
+ * centre, then the samples are placed; nothing in between goes through memory.
+ * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n],
+ *      table f32|f16[offsets[5], 2], params f32|f16[NGP_PROPOSAL_N_PARAMS],
+ *      occupancy_bitfield u8[K*G^3/8]
+ * out: as proposal_sample */
+void proposal_render_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* ---------------------------------------------------------------------------
  * Training-step harness (the device-side counterpart of app/nerf/_utils.py:80-170

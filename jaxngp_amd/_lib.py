@@ -46,6 +46,7 @@ TARGETS = (
     "proposal_mlp_backward",
     "proposal_loss",
     "proposal_sample",
+    "proposal_render_samples",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -275,6 +276,11 @@ def proposal_sample_descriptor(n_rays, n_per_ray=48, mass_threshold=0.9, mode=PR
                                diagonal_n_steps=1024, K=1, G=128, bound=1.0, stepsize_portion=0.0) -> bytes:
     return struct.pack("<2IfIf3I2f", n_rays, n_per_ray, mass_threshold, mode, eps, diagonal_n_steps, K, G, bound,
                        stepsize_portion)
+
+
+def proposal_render_descriptor(grid_desc: bytes, sample_desc: bytes, param_dtype=DTYPE_F32) -> bytes:
+    """ProposalRenderDescriptor: HashGridDescriptor, ProposalSampleDescriptor, u32."""
+    return grid_desc + sample_desc + struct.pack("<I", param_dtype)
 
 
 def proposal_loss_descriptor(n_rays, eps=1e-7) -> bytes:

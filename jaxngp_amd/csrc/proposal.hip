@@ -14,6 +14,7 @@
 
 #include "ngp_common.h"
 #include "ngp_device.h"
+#include "hashgrid_device.h"
 
 namespace ngp {
 namespace {
@@ -235,49 +236,49 @@ __device__ __forceinline__ float wave_prefix_sum(float v) {
     return v;
 }
 
-// One wavefront per ray, lane b = bin b; lanes j < n_per_ray place sample j.
-__global__ void __launch_bounds__(kBlock)
-    proposal_sample_kernel(uint32_t n_rays, uint32_t n_per_ray, float mass_threshold, uint32_t mode, float eps,
-                           MarchParams mp, const float *__restrict__ rays_o, const float *__restrict__ rays_d,
-                           const float *__restrict__ t0s, const float *__restrict__ t1s,
-                           const float *__restrict__ sigma_p, const uint8_t *__restrict__ bits,
-                           uint32_t *__restrict__ startidx,
-                           uint32_t *__restrict__ counts, bool *__restrict__ fallback, float *__restrict__ xyzs,
-                           float *__restrict__ dss, float *__restrict__ zs) {
-    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
-    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
-    if (r >= n_rays) return;
-    const float t0 = t0s[r], t1 = t1s[r];
+struct SampleArgs {
+    uint32_t n_per_ray, mode;
+    float mass_threshold, eps;
+    MarchParams mp;
+};
+
+// Places ray r's samples from lane b's bin density sigma_b (one wavefront per ray,
+// lane b = bin b; lanes j < n_per_ray place sample j).  Samples whose cell the
+// occupancy grid marks empty are dropped (they would add nothing: march_rays never
+// samples such cells); the rest are packed in order at the front of the ray's slots,
+// counts[r] of them, and the remaining slots are zero.
+__device__ __forceinline__ void place_samples(const SampleArgs &sa, uint32_t r, float sigma_b, float t0, float t1,
+                                              const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+                                              const uint8_t *__restrict__ bits, uint32_t *__restrict__ startidx,
+                                              uint32_t *__restrict__ counts, bool *__restrict__ fallback,
+                                              float *__restrict__ xyzs, float *__restrict__ dss,
+                                              float *__restrict__ zs) {
+    const int lane = lane_id();
+    const uint32_t N = sa.n_per_ray;
     const bool seg = t1 > t0;
     const float D = (t1 - t0) / (float)kBins;
-    const float a = seg ? 1.0f - det_expf(-sigma_p[(size_t)r * kBins + lane] * D) : 0.0f;
+    const float a = seg ? 1.0f - det_expf(-sigma_b * D) : 0.0f;
     const float incl = wave_prod_scan(1.0f - a);
     float Tb = __shfl_up(incl, 1, kWave);
     if (lane == 0) Tb = 1.0f;
     const float w = Tb * a;
     const float M = wave_sumf(w);
-    const bool ok = seg && M >= mass_threshold;
-    const size_t base = (size_t)r * n_per_ray;
-    if (lane == 0) {
-        startidx[r] = (uint32_t)base;
-        counts[r] = ok ? n_per_ray : 0u;
-        fallback[r] = seg && !ok;
-    }
+    const bool ok = seg && M >= sa.mass_threshold;
     float t = 0.0f, ds = 0.0f;
-    if (ok && mode == NGP_PROPOSAL_SUPPORT) {
+    if (ok && sa.mode == NGP_PROPOSAL_SUPPORT) {
         // bins [lo, hi]: first with cumulative share > eps, first with >= 1 - eps
         const float inc = wave_prefix_sum(w / M);
-        const uint64_t above = __ballot(inc > eps), full = __ballot(inc >= 1.0f - eps);
+        const uint64_t above = __ballot(inc > sa.eps), full = __ballot(inc >= 1.0f - sa.eps);
         const int lo = above ? __builtin_ctzll(above) : 0;
         int hi = full ? __builtin_ctzll(full) : kBins - 1;
         hi = hi > lo ? hi : lo;
         const float t_lo = t0 + (float)lo * D, t_hi = t0 + (float)(hi + 1) * D;
-        ds = (t_hi - t_lo) / (float)n_per_ray;
+        ds = (t_hi - t_lo) / (float)N;
         t = t_lo + ((float)lane + 0.5f) * ds;
     } else if (ok) {
         const float p = w / M;
         const float C = wave_prefix_sum(p) - p;  // exclusive
-        const float u = ((float)lane + 0.5f) / (float)n_per_ray;
+        const float u = ((float)lane + 0.5f) / (float)N;
         // last bin with C_b <= u (C is non-decreasing, C_0 = 0)
         int b = 0;
 #pragma unroll
@@ -290,26 +291,105 @@ __global__ void __launch_bounds__(kBlock)
         const float frac = pb > 0.0f ? fminf(fmaxf((u - cb) / pb, 0.0f), 1.0f) : 0.5f;
         t = t0 + ((float)b + frac) * D;
         const float tn = __shfl_down(t, 1, kWave);
-        ds = (uint32_t)lane + 1 < n_per_ray ? tn - t
-                                            : fminf(pb > 0.0f ? D / ((float)n_per_ray * pb) : D, t1 - t);
+        ds = (uint32_t)lane + 1 < N ? tn - t : fminf(pb > 0.0f ? D / ((float)N * pb) : D, t1 - t);
     }
-    if ((uint32_t)lane < n_per_ray) {
-        const size_t j = base + lane;
-        float p[3];
+    float p[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) p[c] = ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
-        if (ok) {
-            // the occupancy grid judges the field as in march_rays: a sample in a cell
-            // the grid marks empty contributes nothing (ds = 0)
-            BitCache cache = make_bit_cache(bits, false);
-            float gx, gy, gz, mb;
-            if (!lookup_cell(mp, bits, cache, p[0], p[1], p[2], step_size(mp, t), gx, gy, gz, mb)) ds = 0.0f;
-        }
+    for (int c = 0; c < 3; ++c) p[c] = ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
+    bool keep = false;
+    if (ok && (uint32_t)lane < N) {
+        BitCache cache = make_bit_cache(bits, false);
+        float gx, gy, gz, mb;
+        keep = lookup_cell(sa.mp, bits, cache, p[0], p[1], p[2], step_size(sa.mp, t), gx, gy, gz, mb);
+    }
+    const uint64_t kept = __ballot(keep);
+    const uint32_t n_kept = (uint32_t)__popcll(kept);
+    const size_t base = (size_t)r * N;
+    if (lane == 0) {
+        startidx[r] = (uint32_t)base;
+        counts[r] = n_kept;
+        fallback[r] = seg && !ok;
+    }
+    if (keep) {
+        const size_t j = base + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull));
 #pragma unroll
         for (int c = 0; c < 3; ++c) xyzs[3 * j + c] = p[c];
         dss[j] = ds;
         zs[j] = t;
     }
+    if ((uint32_t)lane >= n_kept && (uint32_t)lane < N) {
+        const size_t j = base + lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xyzs[3 * j + c] = 0.0f;
+        dss[j] = 0.0f;
+        zs[j] = 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    proposal_sample_kernel(uint32_t n_rays, SampleArgs sa, const float *__restrict__ rays_o,
+                           const float *__restrict__ rays_d, const float *__restrict__ t0s,
+                           const float *__restrict__ t1s, const float *__restrict__ sigma_p,
+                           const uint8_t *__restrict__ bits, uint32_t *__restrict__ startidx,
+                           uint32_t *__restrict__ counts, bool *__restrict__ fallback, float *__restrict__ xyzs,
+                           float *__restrict__ dss, float *__restrict__ zs) {
+    const int wave = (int)threadIdx.x / kWave;
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
+    if (r >= n_rays) return;
+    place_samples(sa, r, sigma_p[(size_t)r * kBins + lane_id()], t0s[r], t1s[r], rays_o, rays_d, bits, startidx,
+                  counts, fallback, xyzs, dss, zs);
+}
+
+// Fused: lane b evaluates the proposal (encoder + MLP) at its bin centre, then the
+// ray's samples are placed as in proposal_sample.  No bin points, encodings or
+// densities go through memory.
+template <typename TT, typename PT>
+__global__ void __launch_bounds__(kBlock)
+    proposal_render_kernel(uint32_t n_rays, HashGridDescriptor g, SampleArgs sa, const float *__restrict__ rays_o,
+                           const float *__restrict__ rays_d, const float *__restrict__ t0s,
+                           const float *__restrict__ t1s, const TT *__restrict__ table, const PT *__restrict__ params,
+                           const uint8_t *__restrict__ bits, uint32_t *__restrict__ startidx,
+                           uint32_t *__restrict__ counts, bool *__restrict__ fallback, float *__restrict__ xyzs,
+                           float *__restrict__ dss, float *__restrict__ zs) {
+    __shared__ float w[kParams];
+    __shared__ LevelTables lt;
+    if (threadIdx.x < kIn / 2) {
+        lt.scale[threadIdx.x] = g.scales[threadIdx.x];
+        lt.res[threadIdx.x] = g.resolutions[threadIdx.x];
+        lt.offset[threadIdx.x] = g.offsets[threadIdx.x];
+    }
+    stage_params(w, params);  // ends with __syncthreads
+    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
+    if (r >= n_rays) return;
+    const float t0 = t0s[r], t1 = t1s[r];
+    const float D = (t1 - t0) / (float)kBins;
+    const float tc = t0 + ((float)lane + 0.5f) * D;
+    const float two_b = 2.0f * g.bound;
+    const float px = (rays_o[3 * (size_t)r + 0] + tc * rays_d[3 * (size_t)r + 0] + g.bound) / two_b;
+    const float py = (rays_o[3 * (size_t)r + 1] + tc * rays_d[3 * (size_t)r + 1] + g.bound) / two_b;
+    const float pz = (rays_o[3 * (size_t)r + 2] + tc * rays_d[3 * (size_t)r + 2] + g.bound) / two_b;
+    float x[kIn];
+    const bool pow2 = (g.T & (g.T - 1u)) == 0;
+#pragma unroll
+    for (int l = 0; l < kIn / 2; ++l) {
+        uint32_t idx[8];
+        float wt[8];
+        level_vertices(px, py, pz, lt.scale[l], lt.res[l], (uint32_t)l < g.first_hash_level, g.T, pow2, lt.offset[l],
+                       g.n_rows - 1u, idx, wt);
+        float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const TT *row = table + (size_t)idx[c] * 2;
+            a0 = a0 + to_f32<TT>(row[0]) * wt[c];
+            a1 = a1 + to_f32<TT>(row[1]) * wt[c];
+        }
+        x[2 * l] = a0;
+        x[2 * l + 1] = a1;
+    }
+    float h[kHid];
+    const float sigma_b = t1 > t0 ? det_expf(hidden_and_logit(w, x, h)) : 0.0f;
+    place_samples(sa, r, sigma_b, t0, t1, rays_o, rays_d, bits, startidx, counts, fallback, xyzs, dss, zs);
 }
 
 __global__ void zero_f32_kernel(float *p) {
@@ -429,12 +509,78 @@ extern "C" void proposal_sample(ngp_stream_t stream, void **buffers, const char 
     }
     if (d.n_rays == 0) return;
     if (!check_buffers(buffers, 12, target)) return;
-    const MarchParams mp = make_march_params(d.diagonal_n_steps, d.K, d.G, d.bound, d.stepsize_portion);
+    SampleArgs sa;
+    sa.n_per_ray = d.n_per_ray;
+    sa.mode = d.mode;
+    sa.mass_threshold = d.mass_threshold;
+    sa.eps = d.eps;
+    sa.mp = make_march_params(d.diagonal_n_steps, d.K, d.G, d.bound, d.stepsize_portion);
     hipLaunchKernelGGL(proposal_sample_kernel, dim3(div_up(d.n_rays, kBlock / kWave)), dim3(kBlock), 0,
-                       (hipStream_t)stream, d.n_rays, d.n_per_ray, d.mass_threshold, d.mode, d.eps, mp,
-                       (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
-                       (const float *)buffers[3], (const float *)buffers[4], (const uint8_t *)buffers[5],
-                       (uint32_t *)buffers[6], (uint32_t *)buffers[7], (bool *)buffers[8], (float *)buffers[9],
-                       (float *)buffers[10], (float *)buffers[11]);
+                       (hipStream_t)stream, d.n_rays, sa, (const float *)buffers[0], (const float *)buffers[1],
+                       (const float *)buffers[2], (const float *)buffers[3], (const float *)buffers[4],
+                       (const uint8_t *)buffers[5], (uint32_t *)buffers[6], (uint32_t *)buffers[7], (bool *)buffers[8],
+                       (float *)buffers[9], (float *)buffers[10], (float *)buffers[11]);
+    check_launch(target);
+}
+
+extern "C" void proposal_render_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_render_samples";
+    ProposalRenderDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    const HashGridDescriptor &g = d.grid;
+    const ProposalSampleDescriptor &q = d.sample;
+    if (g.L != (uint32_t)(kIn / 2) || g.F != 2 || g.T == 0 || g.n_rows == 0 || g.first_hash_level > g.L ||
+        g.offsets[0] != 0 || g.offsets[g.L] != g.n_rows || !(g.bound > 0.0f) || g.table_dtype > NGP_DTYPE_F16 ||
+        d.param_dtype > NGP_DTYPE_F16) {
+        set_error(kBadDescriptor, "%s: needs the proposal grid (L = 5, F = 2) with a consistent level table", target);
+        return;
+    }
+    for (uint32_t l = 0; l < g.L; ++l) {
+        if (g.offsets[l + 1] < g.offsets[l]) {
+            set_error(kBadDescriptor, "%s: offsets must be non-decreasing", target);
+            return;
+        }
+    }
+    const uint32_t G = q.G;
+    const bool g_ok = G != 0 && G <= 1024 && (G & (G - 1)) == 0 && (uint64_t)q.K * G * G * G <= 0xffffffffull;
+    if (q.n_per_ray == 0 || q.n_per_ray > kBins || !(q.mass_threshold >= 0.0f) || q.mode > NGP_PROPOSAL_SUPPORT ||
+        !(q.eps >= 0.0f && q.eps < 0.5f) || q.diagonal_n_steps == 0 || q.K == 0 || !g_ok || !(q.bound > 0.0f) ||
+        !(q.stepsize_portion >= 0.0f)) {
+        set_error(kBadDescriptor, "%s: invalid sample placement or marching statics", target);
+        return;
+    }
+    if (q.n_rays == 0) return;
+    if (!check_buffers(buffers, 13, target)) return;
+    SampleArgs sa;
+    sa.n_per_ray = q.n_per_ray;
+    sa.mode = q.mode;
+    sa.mass_threshold = q.mass_threshold;
+    sa.eps = q.eps;
+    sa.mp = make_march_params(q.diagonal_n_steps, q.K, q.G, q.bound, q.stepsize_portion);
+    const dim3 grid(div_up(q.n_rays, kBlock / kWave));
+    hipStream_t s = (hipStream_t)stream;
+    const float *ro = (const float *)buffers[0], *rd = (const float *)buffers[1];
+    const float *t0 = (const float *)buffers[2], *t1 = (const float *)buffers[3];
+    const uint8_t *bits = (const uint8_t *)buffers[6];
+    uint32_t *start = (uint32_t *)buffers[7], *cnt = (uint32_t *)buffers[8];
+    bool *fb = (bool *)buffers[9];
+    float *xyzs = (float *)buffers[10], *dss = (float *)buffers[11], *zs = (float *)buffers[12];
+    const bool t16 = g.table_dtype == NGP_DTYPE_F16, p16 = d.param_dtype == NGP_DTYPE_F16;
+    if (t16 && p16)
+        hipLaunchKernelGGL((proposal_render_kernel<__half, __half>), grid, dim3(kBlock), 0, s, q.n_rays, g, sa, ro, rd,
+                           t0, t1, (const __half *)buffers[4], (const __half *)buffers[5], bits, start, cnt, fb, xyzs,
+                           dss, zs);
+    else if (t16)
+        hipLaunchKernelGGL((proposal_render_kernel<__half, float>), grid, dim3(kBlock), 0, s, q.n_rays, g, sa, ro, rd,
+                           t0, t1, (const __half *)buffers[4], (const float *)buffers[5], bits, start, cnt, fb, xyzs,
+                           dss, zs);
+    else if (p16)
+        hipLaunchKernelGGL((proposal_render_kernel<float, __half>), grid, dim3(kBlock), 0, s, q.n_rays, g, sa, ro, rd,
+                           t0, t1, (const float *)buffers[4], (const __half *)buffers[5], bits, start, cnt, fb, xyzs,
+                           dss, zs);
+    else
+        hipLaunchKernelGGL((proposal_render_kernel<float, float>), grid, dim3(kBlock), 0, s, q.n_rays, g, sa, ro, rd,
+                           t0, t1, (const float *)buffers[4], (const float *)buffers[5], bits, start, cnt, fb, xyzs,
+                           dss, zs);
     check_launch(target);
 }

@@ -198,15 +198,15 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     """Renders one frame with samples placed by the learned proposal: its density at
     64 bins of every ray gives 48 samples, by default evenly over the bins that hold
     the ray's mass (placement PROPOSAL_SUPPORT; PROPOSAL_QUANTILES puts them at the
-    quantiles of the mass instead) (proposal_sample); the field runs on them
-    (nerf_field_chunks) and integrate_rays composites them.  Rays whose predicted mass is below `mass_threshold` are rendered
+    quantiles of the mass instead), minus those in cells the occupancy grid marks
+    empty (proposal_render_samples, which evaluates the proposal too); the field runs
+    on them (nerf_field_chunks) and integrate_rays composites them.  Rays whose predicted mass is below `mass_threshold` are rendered
     by the occupancy-grid ray pool (render_rays_inference) instead.  Same return as
     render_image_inference; `stats` (if given) receives the fallback count."""
     dev = pose.device
     o, d = make_rays_worldspace(camera, pose)
     t0, t1 = make_near_far_from_bound(model.bound, o, d)
     n = o.shape[0]
-    sig = proposal.bin_density(o, d, t0, t1)
     u32 = torch.uint32
     start = torch.empty(n, dtype=u32, device=dev)
     cnt = torch.empty(n, dtype=u32, device=dev)
@@ -214,9 +214,14 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     xyzs = torch.empty(n, n_per_ray, 3, device=dev)
     dss = torch.empty(n, n_per_ray, device=dev)
     zs = torch.empty(n, n_per_ray, device=dev)
-    _lib.call("proposal_sample", [o, d, t0, t1, sig, occupancy_bitfield, start, cnt, fb, xyzs, dss, zs],
-              _lib.proposal_sample_descriptor(n, n_per_ray, mass_threshold, placement, 1e-3, diagonal_n_steps, K, G,
-                                              model.bound, stepsize_portion))
+    ppe = proposal.position_encoder
+    pw, ppd = proposal.weights()
+    _lib.call("proposal_render_samples", [o, d, t0, t1, ppe.table.detach(), pw, occupancy_bitfield, start, cnt, fb,
+                                          xyzs, dss, zs],
+              _lib.proposal_render_descriptor(ppe.descriptor(0, proposal.bound),
+                                              _lib.proposal_sample_descriptor(n, n_per_ray, mass_threshold, placement,
+                                                                              1e-3, diagonal_n_steps, K, G, model.bound,
+                                                                              stepsize_portion), ppd))
     drgbs = torch.empty(n, n_per_ray, 4, device=dev)
     pe = model.position_encoder
     half = getattr(model, "mlp_half", None) is not None

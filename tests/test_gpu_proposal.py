@@ -18,6 +18,7 @@ import torch
 
 import oracle
 from jaxngp_amd import _lib
+from tests.helpers import blob_bitfield
 from tests.test_oracle_pins import _prop_rays
 
 pytestmark = pytest.mark.gpu
@@ -169,20 +170,66 @@ def test_proposal_sample_support_mode(dev):
             (np.abs(dsg - want_ds) <= 2e-5 * span + 1e-7).all(axis=1)
     assert close[ok].mean() > 0.99, close[ok].mean()
 
-    # occupancy: with a grid whose cells are empty for x < 0, exactly the samples in
-    # those cells lose their ds (grid coordinate floor((x + 1) / 2 * 128) < 64)
+    # occupancy: with a grid whose cells are empty for x < 0, the samples in those
+    # cells are dropped and the others kept in order (grid x = floor((x + 1) / 2 * 128))
     from tests.helpers import blob_bitfield
-    bits, _ = blob_bitfield(rng, fill=False)
     xs, ys, zs_ = np.meshgrid(np.arange(128), np.arange(128), np.arange(128), indexing="ij")
     m = oracle.morton3d(np.stack([xs.ravel(), ys.ravel(), zs_.ravel()], 1).astype(np.uint32))
     flat = np.zeros(128 ** 3, bool)
     flat[m] = (xs >= 64).ravel()
     half = T(np.packbits(flat.reshape(-1, 8), axis=-1, bitorder="little").reshape(-1))
-    ds2 = torch.empty_like(ds)
-    _lib.call("proposal_sample", [T(o), T(d), T(t0), T(t1), T(sig), half, start, cnt, fb, xyz, ds2, z],
+    cnt2 = torch.empty_like(cnt)
+    xyz2, ds2, z2 = torch.empty_like(xyz), torch.empty_like(ds), torch.empty_like(z)
+    _lib.call("proposal_sample", [T(o), T(d), T(t0), T(t1), T(sig), half, start, cnt2, fb, xyz2, ds2, z2],
               _lib.proposal_sample_descriptor(n, N, 0.9, _lib.PROPOSAL_SUPPORT, 1e-3))
     x = xyz.cpu().numpy()[..., 0]
     ix = np.floor(((x / np.float32(1.0) + np.float32(1.0)) * np.float32(0.5)) * np.float32(128))
-    empty = (ix < 64) & ok[:, None]
-    d2 = ds2.cpu().numpy()
-    assert (d2[empty] == 0).all() and (d2[~empty & ok[:, None]] == dsg[~empty & ok[:, None]]).all()
+    c2 = cnt2.view(torch.int32).cpu().numpy()
+    zz2, dd2 = z2.cpu().numpy(), ds2.cpu().numpy()
+    for r in np.nonzero(ok)[0][:300]:
+        keep = ix[r] >= 64
+        assert c2[r] == keep.sum()
+        np.testing.assert_array_equal(zz2[r, :c2[r]], zt[r][keep])
+        np.testing.assert_array_equal(dd2[r, :c2[r]], dsg[r][keep])
+        assert (zz2[r, c2[r]:] == 0).all() and (dd2[r, c2[r]:] == 0).all()
+
+
+def test_proposal_render_samples_fused(dev):
+    """The fused kernel equals proposal_points + hashgrid_encode + proposal_mlp_forward
+    + proposal_sample bit for bit (same arithmetic in the same order)."""
+    from jaxngp_amd.proposal import ProposalNet
+    rng = np.random.default_rng(13)
+    n, N = 3000, 48
+    prop = ProposalNet(1.0, dev, seed=5, table_dtype=torch.float16)
+    with torch.no_grad():
+        prop.position_encoder.table.uniform_(-1.0, 1.0)
+        prop.mlp_params.mul_(3.0)
+    prop.mlp_half = prop.mlp_params.detach().half()
+    o = (rng.normal(size=(n, 3)) * 0.3 - np.array([0, 0, 3.0])).astype(np.float32)
+    d = rng.normal(size=(n, 3)) * 0.2 + np.array([0, 0, 1.0])
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    from jaxngp_amd.renderer import make_near_far_from_bound
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    ot, dt = T(o), T(d)
+    t0, t1 = make_near_far_from_bound(1.0, ot, dt)
+    bits, _ = blob_bitfield(rng)
+    bt = T(bits)
+    outs = []
+    for fused in (False, True):
+        start = torch.empty(n, dtype=torch.int32, device=dev).view(torch.uint32)
+        cnt = torch.empty_like(start)
+        fb = torch.empty(n, dtype=torch.bool, device=dev)
+        xyz, ds, z = torch.empty(n, N, 3, device=dev), torch.empty(n, N, device=dev), torch.empty(n, N, device=dev)
+        sd = _lib.proposal_sample_descriptor(n, N, 0.5, _lib.PROPOSAL_SUPPORT, 1e-3)
+        if fused:
+            w, pd = prop.weights()
+            _lib.call("proposal_render_samples", [ot, dt, t0, t1, prop.position_encoder.table.detach(), w, bt, start,
+                                                  cnt, fb, xyz, ds, z],
+                      _lib.proposal_render_descriptor(prop.position_encoder.descriptor(0, 1.0), sd, pd))
+        else:
+            sig = prop.bin_density(ot, dt, t0, t1)
+            _lib.call("proposal_sample", [ot, dt, t0, t1, sig, bt, start, cnt, fb, xyz, ds, z], sd)
+        outs.append([a.cpu() for a in (start.view(torch.int32), cnt.view(torch.int32), fb, xyz, ds, z)])
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+    assert outs[0][1].sum() > 0 and outs[0][2].sum() > 0
