@@ -132,15 +132,28 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
 POOL = 1 << 18  # ray slots of the inference renderer (the reference uses 8192; same image)
 
 
+def train_to(tr, total_steps: int) -> None:
+    """Untimed training until the trainer has taken `total_steps` steps in all, so
+    that the frames are rendered from a field trained as long as the PSNR figures
+    are quoted for."""
+    import torch
+
+    while tr.step_count < total_steps:
+        tr.step()
+    torch.cuda.synchronize()
+
+
 def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dict:
-    """ms per 800x800 frame of the trained field from held-out poses, with the
+    """ms per 800x800 frame of the trained field from held-out seeded poses, with the
     reference's ray-pool renderer (march_steps_cap 8, white background) on a pool of
-    2^18 slots, median over the timed frames after two warm-up renders; PSNR of the
+    2^18 slots, median over the timed frames after three warm-up renders; PSNR of the
     u8 render against the float64 ground truth blended on white.  With `proposal`,
     the frames are rendered with 48 proposal-placed samples per ray (occupancy-grid
-    fallback below mass 0.9) and the same field is also rendered the plain way, for
-    the PSNR difference."""
+    fallback below predicted mass 0.97) and the same field is also rendered the plain
+    way, for the PSNR difference; the same comparison at the fallback threshold 0.9
+    is reported beside it."""
     import torch
+    from jaxngp_amd import _lib
     from jaxngp_amd import scene as sc
     from jaxngp_amd.renderer import render_image_inference, render_image_proposal
 
@@ -150,14 +163,15 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
     kw = dict(K=tr.K, G=c.grid_res, diagonal_n_steps=c.diagonal_n_steps, stepsize_portion=tr.portion,
               march_steps_cap=8, pool=POOL)
     pose_t = [torch.as_tensor(p, dtype=torch.float32, device=dev) for p in poses]
+    plain = lambda pose: render_image_inference(tr.model, data.camera, pose, tr.grid.bitfield, **kw)
     if proposal:
-        render = lambda pose, st=None: render_image_proposal(tr.model, tr.prop, data.camera, pose, tr.grid.bitfield,
-                                                             stats=st, **kw)
+        render = lambda pose, st=None, **o: render_image_proposal(tr.model, tr.prop, data.camera, pose,
+                                                                  tr.grid.bitfield, stats=st, **o, **kw)
     else:
-        render = lambda pose, st=None: render_image_inference(tr.model, data.camera, pose, tr.grid.bitfield, **kw)
-    for i in range(2):
+        render = lambda pose, st=None: plain(pose)
+    for i in range(3):
         render(pose_t[i % n_frames])
-    times, psnrs, plain_psnrs, fallback = [], [], [], []
+    times, psnrs, plain_psnrs, fallback, psnrs09, fallback09 = [], [], [], [], [], []
     for i in range(n_frames):
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -169,19 +183,29 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
         psnrs.append(sc.psnr_u8(rgb, ref))
         if proposal:
             fallback.append(st["fallback_rays"] / (st["fallback_rays"] + st["proposal_rays"]))
-            _, rgb_plain, _ = render_image_inference(tr.model, data.camera, pose_t[i], tr.grid.bitfield, **kw)
-            plain_psnrs.append(sc.psnr_u8(rgb_plain, ref))
+            plain_psnrs.append(sc.psnr_u8(plain(pose_t[i])[1], ref))
+            st = {}
+            rgb09 = render(pose_t[i], st, mass_threshold=0.9, placement=_lib.PROPOSAL_SUPPORT)[1]
+            psnrs09.append(sc.psnr_u8(rgb09, ref))
+            fallback09.append(st["fallback_rays"] / (st["fallback_rays"] + st["proposal_rays"]))
     times.sort()
     res = {"ms_per_frame": times[len(times) // 2], "frames": n_frames, "psnr_db": float(np.mean(psnrs)),
            "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
            "train_steps": tr.step_count}
     if proposal:
-        res.update(samples_per_ray=48, mass_threshold=0.9, fallback_ray_fraction=float(np.mean(fallback)),
+        res.update(samples_per_ray=48, mass_threshold=0.97, placement="quantiles",
+                   fallback_ray_fraction=float(np.mean(fallback)),
                    psnr_plain_render_db=float(np.mean(plain_psnrs)),
-                   delta_psnr_db=float(np.mean(psnrs) - np.mean(plain_psnrs)))
+                   delta_psnr_db=float(np.mean(psnrs) - np.mean(plain_psnrs)),
+                   worst_view_delta_psnr_db=float(min(a - b for a, b in zip(psnrs, plain_psnrs))),
+                   at_mass_threshold_0_9={"placement": "support",
+                                          "delta_psnr_db": float(np.mean(psnrs09) - np.mean(plain_psnrs)),
+                                          "fallback_ray_fraction": float(np.mean(fallback09))})
     log(f"inference{' (proposal)' if proposal else ''}: {res['ms_per_frame']:.2f} ms/frame (median of {n_frames}), "
-        f"PSNR {res['psnr_db']:.2f} dB" + (f", plain render of the same field {res['psnr_plain_render_db']:.2f} dB, "
-                                           f"fallback rays {res['fallback_ray_fraction']:.1%}" if proposal else ""))
+        f"PSNR {res['psnr_db']:.2f} dB after {tr.step_count} steps" +
+        (f", plain render of the same field {res['psnr_plain_render_db']:.2f} dB, fallback rays "
+         f"{res['fallback_ray_fraction']:.1%}; at mass threshold 0.9: delta "
+         f"{res['at_mass_threshold_0_9']['delta_psnr_db']:+.3f} dB" if proposal else ""))
     return res
 
 
@@ -193,7 +217,9 @@ def main() -> None:
     ap.add_argument("--views", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--frames", type=int, default=5, help="timed 800x800 inference frames (held-out poses)")
+    ap.add_argument("--frames", type=int, default=20, help="timed 800x800 inference frames (held-out poses)")
+    ap.add_argument("--render-steps", type=int, default=5000,
+                    help="train (untimed, after the timed steps) to this many steps in all before rendering frames")
     ap.add_argument("--no-fp16", action="store_true", help="skip the f16-weight precision path")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal path")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -289,6 +315,7 @@ def main() -> None:
         "roofline": roof,
     }
     if args.frames > 0:
+        train_to(tr, args.render_steps)
         out["inference"] = bench_inference(tr, data, args.frames, dev)
     if not args.no_proposal:
         # the learned-proposal path on the precision path: trained online with the field,
@@ -299,6 +326,7 @@ def main() -> None:
                                 "ms_per_step": rp["ms"], "loss": tp.metrics()["loss"],
                                 "proposal_loss": tp.metrics()["proposal_loss"]}
         if args.frames > 0:
+            train_to(tp, args.render_steps)
             out["proposal_fp16"]["inference"] = bench_inference(tp, data, args.frames, dev, proposal=True)
     if not args.no_fp16:
         # the precision path on the same scene: f16 table and MLP weights, loss scaling
@@ -309,6 +337,7 @@ def main() -> None:
                                               for k, v in r16["events"].items() if v},
                        "loss": t16.metrics()["loss"], "loss_fp32": tr.metrics()["loss"]}
         if args.frames > 0:
+            train_to(t16, args.render_steps)
             out["fp16"]["inference"] = bench_inference(t16, data, args.frames, dev)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

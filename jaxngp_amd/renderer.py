@@ -191,17 +191,25 @@ def render_rays_inference(model, o: torch.Tensor, d: torch.Tensor, occupancy_bit
 
 @torch.no_grad()
 def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, occupancy_bitfield,
-                          bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.9,
-                          placement: int = _lib.PROPOSAL_SUPPORT, K=1, G=128,
+                          bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.97,
+                          placement: int = _lib.PROPOSAL_QUANTILES, K=1, G=128,
                           diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8, pool: int = 1 << 18,
-                          stats: dict | None = None):
+                          fallback_steps_cap: int = 32, stats: dict | None = None):
     """Renders one frame with samples placed by the learned proposal: its density at
-    64 bins of every ray gives 48 samples, by default evenly over the bins that hold
-    the ray's mass (placement PROPOSAL_SUPPORT; PROPOSAL_QUANTILES puts them at the
-    quantiles of the mass instead), minus those in cells the occupancy grid marks
-    empty (proposal_render_samples, which evaluates the proposal too); the field runs
-    on them (nerf_field_chunks) and integrate_rays composites them.  Rays whose predicted mass is below `mass_threshold` are rendered
-    by the occupancy-grid ray pool (render_rays_inference) instead.  Same return as
+    64 bins of every ray gives 48 samples, by default at the quantiles of the ray's
+    predicted mass (placement PROPOSAL_QUANTILES; PROPOSAL_SUPPORT spreads them evenly
+    over the bins that hold mass instead), minus those in cells the occupancy grid
+    marks empty (proposal_render_samples, which evaluates the proposal too); the field
+    runs on them (nerf_field_chunks) and integrate_rays composites them.  Rays whose
+    predicted mass is below `mass_threshold` are rendered by the occupancy-grid ray
+    pool (render_rays_inference) instead.  The proposal is trained to bound the
+    field's weights from above, so its mass over-reads; 0.97 is the default because
+    rays predicted between 0.9 and 0.97 are where its samples miss the field's mass.
+    The fallback marches `fallback_steps_cap` samples per ray and round: a ray's
+    samples and colour do not depend on that cap (march_rays_inference resumes from
+    the t it stopped at and defers a far-plane sample to the next round), so its
+    pixels equal the plain render's, in fewer rounds.  `march_steps_cap` is accepted
+    for the same keyword set as render_image_inference.  Same return as
     render_image_inference; `stats` (if given) receives the fallback count."""
     dev = pose.device
     o, d = make_rays_worldspace(camera, pose)
@@ -243,7 +251,7 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     if idx.numel():
         rgbd_fb, _ = render_rays_inference(model, o[idx].contiguous(), d[idx].contiguous(), occupancy_bitfield, bg,
                                            K=K, G=G, diagonal_n_steps=diagonal_n_steps,
-                                           stepsize_portion=stepsize_portion, march_steps_cap=march_steps_cap,
+                                           stepsize_portion=stepsize_portion, march_steps_cap=fallback_steps_cap,
                                            pool=pool)
         rays_rgbd[idx] = rgbd_fb
     return _image(camera, rays_rgbd, bgs)

@@ -1,12 +1,13 @@
 """GPU checks of the inference renderer (models/renderers/cuda.py:244-373): the frame
 does not depend on the ray-pool size (each ray is marched in the same chunks of
-march_steps_cap steps whichever slot holds it), and a trained field renders the
-held-out view close to the float64 ground truth."""
+march_steps_cap steps whichever slot holds it) nor on march_steps_cap, and a
+trained field renders the held-out view close to the float64 ground truth."""
 import math
 
-import torch
 import pytest
+import torch
 
+from jaxngp_amd import _lib
 from jaxngp_amd import scene as sc
 from jaxngp_amd.renderer import (make_near_far_from_bound, make_rays_worldspace, march_and_integrate_inference,
                                  render_image_inference)
@@ -56,6 +57,10 @@ def test_render_pool_invariance_and_quality():
         _, a, da = render_image_inference(tr.model, cam, pose, tr.grid.bitfield, pool=300, **kw)
         _, b, db = render_image_inference(tr.model, cam, pose, tr.grid.bitfield, pool=1 << 14, **kw)
         assert torch.equal(a, b) and torch.equal(da, db)
+        # nor on the samples marched per round (the proposal renderer's fallback uses 32)
+        kw32 = dict(kw, march_steps_cap=32)
+        _, b32, db32 = render_image_inference(tr.model, cam, pose, tr.grid.bitfield, pool=1 << 12, **kw32)
+        assert torch.equal(a, b32) and torch.equal(da, db32)
         # the fused, pre-bound renderer equals the op-by-op loop with the reference's pool
         c = _render_with_ops(tr.model, cam, pose, tr.grid.bitfield, 8192, march_steps_cap=8, K=tr.K, G=128,
                              diagonal_n_steps=1024, stepsize_portion=tr.portion)
@@ -68,7 +73,8 @@ def test_render_pool_invariance_and_quality():
 def test_proposal_training_and_render():
     """The proposal trained online on the marcher's weights: its loss falls, and the
     same field rendered with 48 proposal-placed samples per ray (occupancy-grid
-    fallback below mass 0.9) stays close to the occupancy-grid render."""
+    fallback below predicted mass 0.97, and at 0.9 with support placement) stays
+    close to the occupancy-grid render."""
     from jaxngp_amd.renderer import render_image_proposal
     dev = torch.device("cuda:0")
     cam = sc.Camera(width=96, height=96, fx=133.0, fy=133.0, cx=48.0, cy=48.0)
@@ -92,6 +98,10 @@ def test_proposal_training_and_render():
         _, prop, _ = render_image_proposal(tr.model, tr.prop, cam, pose, tr.grid.bitfield, stats=st, **kw)
         ref = sc.blend_u8(gt.rgba[i], (1.0, 1.0, 1.0))
         p_plain, p_prop = sc.psnr_u8(plain, ref), sc.psnr_u8(prop, ref)
-        print(f"view {i}: plain {p_plain:.2f} dB, proposal {p_prop:.2f} dB, {st}")
-        assert st["proposal_rays"] > 0
-        assert abs(p_prop - p_plain) < 1.5
+        st09 = {}
+        _, prop09, _ = render_image_proposal(tr.model, tr.prop, cam, pose, tr.grid.bitfield, stats=st09,
+                                             mass_threshold=0.9, placement=_lib.PROPOSAL_SUPPORT, **kw)
+        p_prop09 = sc.psnr_u8(prop09, ref)
+        print(f"view {i}: plain {p_plain:.2f} dB, proposal {p_prop:.2f} dB {st}, at mass 0.9 {p_prop09:.2f} dB {st09}")
+        assert st["proposal_rays"] > 0 and st09["proposal_rays"] >= st["proposal_rays"]
+        assert abs(p_prop - p_plain) < 1.5 and abs(p_prop09 - p_plain) < 1.5

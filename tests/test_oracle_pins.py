@@ -428,6 +428,70 @@ def test_integrate_inference_chunks_equal_single_pass(rng):
     np.testing.assert_array_equal(rgbd, rgbd_train)
 
 
+def _pool_render(o, d, t0, t1, bits, cap, pool, sigma_scale):
+    """The inference ray-pool loop (models/renderers/cuda.py:319-361) over the oracle's
+    march_rays_inference / integrate_rays_inference with a closed-form field.
+    Returns (rays_rgbd, number of far-plane samples, number of full chunks that
+    deferred their far-plane sample)."""
+    n_tot = o.shape[0]
+    rgbd_all = np.zeros((n_tot, 4), np.float32)
+    T_all = np.ones(n_tot, np.float32)
+    bg = np.ones((n_tot, 3), np.float32)
+    ts = t0.copy()
+    terminated = np.ones(pool, bool)
+    indices = np.zeros(pool, np.uint32)
+    nri, done, far, deferred = np.uint32(0), 0, 0, 0
+    for _ in range(20000):
+        out = oracle.march_rays_inference(1024, 1, 128, cap, 1.0, 0.0, o, d, ts, t1, bits, nri, terminated, indices)
+        nri, indices = out["next_ray_index"], out["indices"]
+        live = indices < n_tot
+        r = indices[live]
+        ts[r] = out["t_starts_out"][live]
+        k = out["n_samples"]
+        full = k[live] == cap
+        deferred += int((full & (out["t_starts_out"][live] == t1[r]) & (out["z_vals"][live, -1] < t1[r])).sum())
+        far += int(sum((out["z_vals"][i, :k[i]] == t1[indices[i]]).sum() for i in np.nonzero(live)[0]))
+        x = out["xyzs"].astype(np.float32)
+        drgbs = np.zeros(x.shape[:2] + (4,), np.float32)
+        drgbs[..., 0] = np.float32(sigma_scale) * np.abs(np.sin(np.float32(5.0) * x[..., 0]) *
+                                                         np.cos(np.float32(3.0) * x[..., 1]))
+        drgbs[..., 1:] = np.float32(0.5) + np.float32(0.5) * np.sin(np.float32(4.0) * x)
+        cnt, terminated, rgbd, T = oracle.integrate_rays_inference(bg, rgbd_all, T_all, k, indices, out["dss"],
+                                                                   out["z_vals"], drgbs)
+        rgbd_all[r] = rgbd[live]
+        T_all[r] = T[live]
+        done += int(cnt)
+        if done >= n_tot:
+            return rgbd_all, far, deferred
+    raise AssertionError("pool loop did not finish")
+
+
+def test_inference_render_independent_of_steps_cap(rng):
+    """A ray's colour from the ray pool does not depend on march_steps_cap: the march
+    resumes from the t it stopped at, a far-plane sample that does not fit is written
+    in the next round, and the compositing carries (rgbd, T) between rounds in f32 in
+    the same order.  The proposal renderer's occupancy-grid fallback relies on this
+    to march more samples per round than the plain render and give the same pixels.
+    Two scenes: blobs of occupied cells with a dense field (early termination), and
+    every cell occupied with a faint field, where rays leave the box in an occupied
+    cell (far-plane samples, deferred when the chunk is full)."""
+    n = 96
+    o, d, t0, t1, _, bits, _ = march_inputs(rng, n)
+    full_bits = np.full_like(bits, 0xFF)
+    for b, sigma_scale, cases in ((bits, 40.0, ((1, 40), (3, 17), (32, 40), (64, 96))),
+                                  (full_bits, 0.3, ((2, 96), (5, 48), (32, 96)))):
+        ref, far, _ = _pool_render(o, d, t0, t1, b, 8, 40, sigma_scale)
+        assert (ref[:, :3] != 1.0).any()  # some rays composite samples
+        deferred = 0
+        for cap, pool in cases:
+            got, far_c, deferred_c = _pool_render(o, d, t0, t1, b, cap, pool, sigma_scale)
+            np.testing.assert_array_equal(got, ref)
+            assert far_c == far
+            deferred += deferred_c
+        if b is full_bits:
+            assert far > 0 and deferred > 0
+
+
 def _fd_loss(startidx, ns, bgs, dss, z, drgbs, g):
     rgbd, _, _ = oracle.integrate_rays64(startidx, ns, bgs, dss, z, drgbs)
     return float((rgbd * g).sum())
