@@ -6,9 +6,9 @@
 //
 // Training integration: one wavefront per ray.  The 64 lanes load a chunk of the
 // ray's samples with coalesced accesses and compute alpha = 1 - exp(-sigma ds) in
-// parallel; the serial transmittance recurrence (T *= 1 - alpha, a running f32
-// product, so the early-stop index is exactly the oracle's) then walks the chunk on
-// wavefront-uniform values.  Inference integration keeps one thread per pool slot
+// parallel; the transmittance recurrence (T *= 1 - alpha, a running f32 product,
+// so the early-stop index is exactly the oracle's) runs as a branch-free chain over
+// the chunk, and the colour and depth sums run in sample order in four lanes.  Inference integration keeps one thread per pool slot
 // (at most march_steps_cap samples each).  Counters are reduced per wavefront and
 // added with one atomic per wave.
 #include "ngp_common.h"
@@ -47,66 +47,198 @@ __device__ __forceinline__ float lane_value(float v, int q) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
 }
 
-// One wavefront per ray (rays assigned round-robin to the grid's wavefronts).
-// Each chunk of 64 samples is loaded by the 64 lanes at once and every lane
-// evaluates its sample's alpha; the compositing recurrence then walks the chunk in
-// order on wavefront-uniform values, with exactly the operation order of the
-// one-thread-per-ray formulation (so the early-stop index is the oracle's).
+// Rays of wavefront w: w, w + n_waves, w + 2 n_waves, ...  Lane i of a batch loads
+// the count and start of the batch's i-th ray (one load for 64 rays), rays
+// without samples are written by their own lanes at once, and the wavefront walks
+// only the rays that have samples.  Training batches hold their valid rays first
+// (march_rays keeps ray order), so the stride spreads them over the wavefronts.
+struct RayBatch {
+    uint64_t ray;  // this lane's ray
+    bool in;       // ray < n_rays
+    size_t s0;
+    uint32_t n;    // samples inside the arrays
+};
+
+__device__ __forceinline__ RayBatch load_ray_batch(uint64_t first, uint32_t stride, uint32_t n_rays,
+                                                   const uint32_t *__restrict__ startidx,
+                                                   const uint32_t *__restrict__ nsamp, uint32_t total_samples) {
+    RayBatch rb;
+    rb.ray = first + (uint64_t)lane_id() * stride;
+    rb.in = rb.ray < n_rays;
+    rb.s0 = rb.in ? startidx[rb.ray] : 0;
+    rb.n = rb.in ? sample_count(rb.s0, nsamp[rb.ray], total_samples) : 0u;
+    return rb;
+}
+
+// A chunk of up to 64 samples of one ray, lane q holding sample c0 + q.
+struct Chunk {
+    float4 c;  // sigma, r, g, b
+    float z, ds;
+};
+
+__device__ __forceinline__ Chunk load_chunk(const float *__restrict__ drgbs, const float *__restrict__ zs,
+                                            const float *__restrict__ dss, size_t s0, uint32_t c0, uint32_t n) {
+    Chunk k;
+    const uint32_t j = c0 + lane_id();
+    if (j < n) {
+        k.c = load_drgb(drgbs, s0 + j);
+        k.z = zs[s0 + j];
+        k.ds = dss[s0 + j];
+    } else {
+        k.c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        k.z = 0.0f;
+        k.ds = 0.0f;
+    }
+    return k;
+}
+
+// LDS rows of one wavefront: 1 - alpha, T before each sample, and the four
+// product rows (r, g, b, depth).  Rows are 68 floats apart so that the four lanes
+// of ordered_sums read different banks.
+constexpr int kRow = kWave + 4;
+constexpr int kRows = 6 * kRow;
+
+// The transmittance recurrence over one chunk, T = T * (1 - alpha) in sample order,
+// with no data-dependent branch: the 64 factors go through LDS (broadcast reads),
+// lane 0 stores T before each sample, and lane q reads back T_q.  The stop index
+// (first q < m with !(T_q > threshold), where the one-thread loop
+// `for (; T > thr && k < n; ++k)` exits) comes from one ballot.  T returns the
+// transmittance the sequential loop ends the chunk with.  Lanes >= m carry
+// 1 - alpha = 1, which leaves T unchanged.
+__device__ __forceinline__ int transmittance_chunk(float *rows, float om, int m, float &T, float &Tq) {
+    const int lane = lane_id();
+    float *om_row = rows, *t_row = rows + kRow;
+    om_row[lane] = om;
+    __builtin_amdgcn_wave_barrier();
+    float4 om4[kWave / 4];  // all factors in registers first, so the reads overlap
+#pragma unroll
+    for (int q = 0; q < kWave / 4; ++q) om4[q] = reinterpret_cast<const float4 *>(om_row)[q];
+    float t = T;
+#pragma unroll
+    for (int q = 0; q < kWave / 4; ++q) {
+        const float4 o = om4[q];
+        float4 v;
+        v.x = t;
+        t = t * o.x;
+        v.y = t;
+        t = t * o.y;
+        v.z = t;
+        t = t * o.z;
+        v.w = t;
+        t = t * o.w;
+        if (lane == 0) reinterpret_cast<float4 *>(t_row)[q] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+    Tq = t_row[lane];
+    const uint64_t dead = __ballot(lane < m && !(Tq > kTThreshold));
+    const int stop = dead ? __builtin_ctzll(dead) : m;
+    T = stop < m ? lane_value(Tq, stop) : t;
+    __builtin_amdgcn_wave_barrier();
+    return stop;
+}
+
+// Sequential sums in sample order: lane ch (0..3) adds product row ch (64 values,
+// +0 past the stop index) into acc, the running channel value, one sample at a
+// time.  With `prefix`, the running value after each sample is written back into
+// the row.  A sum that starts at +0 is never -0, so the +0 padding adds nothing.
+__device__ __forceinline__ void ordered_sums(float *prods, float &acc, bool prefix) {
+    const int lane = lane_id();
+    if (lane < 4) {
+        float4 *row = reinterpret_cast<float4 *>(prods + lane * kRow);
+        float a = acc;
+#pragma unroll 4
+        for (int q = 0; q < kWave / 4; ++q) {
+            float4 v = row[q];
+            a = a + v.x;
+            v.x = a;
+            a = a + v.y;
+            v.y = a;
+            a = a + v.z;
+            v.z = a;
+            a = a + v.w;
+            v.w = a;
+            if (prefix) row[q] = v;
+        }
+        acc = a;
+    }
+}
+
+// One wavefront per ray with samples.  Each chunk of 64 samples is loaded by the 64
+// lanes at once (the next chunk's loads are issued before this chunk is worked);
+// every lane evaluates its sample's alpha and 1 - alpha; the transmittance chain
+// and the colour/depth sums then run in sample order with the one-thread-per-ray
+// formulation's exact operations (w = T a, acc = acc + w c, T = T (1 - a)), so
+// colours, opacities and the early-stop index equal the oracle's bit for bit.
 __global__ void __launch_bounds__(kBlock)
     integrate_kernel(uint32_t n_rays, uint32_t total_samples, const uint32_t *__restrict__ startidx,
                      const uint32_t *__restrict__ nsamp, const float *__restrict__ bgs, const float *__restrict__ dss,
                      const float *__restrict__ zs, const float *__restrict__ drgbs, uint32_t *__restrict__ measured,
                      float *__restrict__ rgbd_out, float *__restrict__ opacity_out) {
+    __shared__ __align__(16) float rows_all[kBlock / kWave][kRows];
+    float *rows = rows_all[threadIdx.x / kWave];
+    float *prods = rows + 2 * kRow;
     const int lane = lane_id();
     const uint32_t n_waves = gridDim.x * (kBlock / kWave);
+    const uint32_t wave = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     uint32_t composited = 0;
-    for (uint32_t ray = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; ray < n_rays; ray += n_waves) {
-        const size_t s0 = startidx[ray];
-        const uint32_t n = sample_count(s0, nsamp[ray], total_samples);
-        float T = 1.0f, r = 0.0f, g = 0.0f, b = 0.0f, depth = 0.0f;
-        uint32_t k = 0;
-        for (uint32_t c0 = 0; c0 < n && T > kTThreshold; c0 += kWave) {
-            const uint32_t j = c0 + lane;
-            float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            float z = 0.0f, alpha = 0.0f;
-            if (j < n) {
-                c = load_drgb(drgbs, s0 + j);
-                z = zs[s0 + j];
-                alpha = 1.0f - det_expf(-c.x * dss[s0 + j]);
-            }
-            const int m = (int)min(n - c0, (uint32_t)kWave);
-            for (int q = 0; q < m && T > kTThreshold; ++q) {
-                const float a = lane_value(alpha, q);
-                const float w = T * a;
-                r = r + w * lane_value(c.y, q);
-                g = g + w * lane_value(c.z, q);
-                b = b + w * lane_value(c.w, q);
-                depth = depth + w * lane_value(z, q);
-                T = T * (1.0f - a);
-                ++k;
-            }
+    for (uint64_t first = wave; first < n_rays; first += (uint64_t)kWave * n_waves) {
+        const RayBatch rb = load_ray_batch(first, n_waves, n_rays, startidx, nsamp, total_samples);
+        if (rb.in && rb.n == 0) {  // T = 1: background, opacity 0
+            opacity_out[rb.ray] = 0.0f;
+            *reinterpret_cast<float4 *>(rgbd_out + 4 * rb.ray) =
+                make_float4(0.0f + bgs[3 * rb.ray], 0.0f + bgs[3 * rb.ray + 1], 0.0f + bgs[3 * rb.ray + 2], 0.0f);
         }
-        composited += k;
-        if (lane == 0) {
-            const float op = 1.0f - T;
-            opacity_out[ray] = op;
-            float4 out;
-            if (T <= kTThreshold) {
-                const float inv = 1.0f / op;
-                out = make_float4(r * inv, g * inv, b * inv, depth * inv);
-            } else {
-                out = make_float4(r + T * bgs[3 * (size_t)ray], g + T * bgs[3 * (size_t)ray + 1],
-                                  b + T * bgs[3 * (size_t)ray + 2], depth);
+        for (uint64_t todo = __ballot(rb.in && rb.n > 0); todo != 0; todo &= todo - 1) {
+            const int src = __builtin_ctzll(todo);
+            const uint32_t ray = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rb.ray, src);
+            const size_t s0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rb.s0, src);
+            const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)rb.n, src);
+            float T = 1.0f, acc = 0.0f;  // lane ch < 4: running r, g, b, depth
+            uint32_t k = 0;
+            Chunk next = load_chunk(drgbs, zs, dss, s0, 0, n);
+            for (uint32_t c0 = 0; c0 < n && T > kTThreshold; c0 += kWave) {
+                const Chunk ch = next;
+                if (c0 + kWave < n) next = load_chunk(drgbs, zs, dss, s0, c0 + kWave, n);
+                const int m = (int)min(n - c0, (uint32_t)kWave);
+                const float alpha = lane < m ? 1.0f - det_expf(-ch.c.x * ch.ds) : 0.0f;
+                const float om = lane < m ? 1.0f - alpha : 1.0f;
+                float Tq;
+                const int stop = transmittance_chunk(rows, om, m, T, Tq);
+                k += (uint32_t)stop;
+                const float w = Tq * alpha;
+                const bool in = lane < stop;
+                prods[lane] = in ? w * ch.c.y : 0.0f;
+                prods[kRow + lane] = in ? w * ch.c.z : 0.0f;
+                prods[2 * kRow + lane] = in ? w * ch.c.w : 0.0f;
+                prods[3 * kRow + lane] = in ? w * ch.z : 0.0f;
+                __builtin_amdgcn_wave_barrier();
+                ordered_sums(prods, acc, false);
+                __builtin_amdgcn_wave_barrier();
             }
-            *reinterpret_cast<float4 *>(rgbd_out + 4 * (size_t)ray) = out;
+            composited += k;
+            // lane ch < 4 writes channel ch
+            const float op = 1.0f - T;
+            if (lane == 0) opacity_out[ray] = op;
+            if (lane < 4) {
+                float out;
+                if (T <= kTThreshold) {
+                    const float inv = 1.0f / op;
+                    out = acc * inv;
+                } else {
+                    out = lane < 3 ? acc + T * bgs[3 * (size_t)ray + lane] : acc;
+                }
+                rgbd_out[4 * (size_t)ray + lane] = out;
+            }
         }
     }
     if (lane == 0 && composited != 0) atomicAdd(measured, composited);
 }
 
-// Backward, one wavefront per ray.  The recurrence runs as in the forward; lane q
-// keeps the state right after sample q (w_q, T_{q+1}, running colour and depth),
-// then every lane writes its own sample's gradients (integrating.cu:177-233).
+// Backward, one wavefront per ray with samples (rays batched as in the forward;
+// rays without samples give only dL/dbg = gr).  The forward's chain runs again;
+// lane q then holds w_q, T_{q+1} = T_q (1 - a_q) and the colour and depth sums up
+// to and including sample q (the ordered prefix sums), and writes its own sample's
+// gradients (integrating.cu:177-233).
 __global__ void __launch_bounds__(kBlock)
     integrate_backward_kernel(uint32_t n_rays, uint32_t total_samples, float near_distance,
                               const uint32_t *__restrict__ startidx, const uint32_t *__restrict__ nsamp,
@@ -115,68 +247,74 @@ __global__ void __launch_bounds__(kBlock)
                               const float *__restrict__ final_rgbds, const float *__restrict__ final_opacities,
                               const float *__restrict__ dL_drgbd, float *__restrict__ dL_dbgs,
                               float *__restrict__ dL_dz, float *__restrict__ dL_ddrgbs) {
+    __shared__ __align__(16) float rows_all[kBlock / kWave][kRows];
+    float *rows = rows_all[threadIdx.x / kWave];
+    float *prods = rows + 2 * kRow;
     const int lane = lane_id();
     const uint32_t n_waves = gridDim.x * (kBlock / kWave);
-    for (uint32_t ray = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; ray < n_rays; ray += n_waves) {
-        const size_t s0 = startidx[ray];
-        const uint32_t n = sample_count(s0, nsamp[ray], total_samples);
-        const float4 R = *reinterpret_cast<const float4 *>(final_rgbds + 4 * (size_t)ray);
-        const float4 gr = *reinterpret_cast<const float4 *>(dL_drgbd + 4 * (size_t)ray);
-        const float bg0 = bgs[3 * (size_t)ray], bg1 = bgs[3 * (size_t)ray + 1], bg2 = bgs[3 * (size_t)ray + 2];
-        const float op = final_opacities[ray];
-        const bool terminated = op >= 1.0f - kTThreshold;
-        const float bgk = terminated ? 0.0f : 1.0f - op;
-        float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, cd = 0.0f;
-        for (uint32_t c0 = 0; c0 < n && T > kTThreshold; c0 += kWave) {
-            const uint32_t j = c0 + lane;
-            float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            float z = 0.0f, ds = 0.0f, alpha = 0.0f;
-            if (j < n) {
-                c = load_drgb(drgbs, s0 + j);
-                z = zs[s0 + j];
-                ds = dss[s0 + j];
-                alpha = 1.0f - det_expf(-c.x * ds);
-            }
-            const int m = (int)min(n - c0, (uint32_t)kWave);
-            float my_w = 0.0f, my_T = 0.0f, my_r = 0.0f, my_g = 0.0f, my_b = 0.0f, my_d = 0.0f;
-            int done = 0;  // samples of this chunk composited
-            for (int q = 0; q < m && T > kTThreshold; ++q) {
-                const float a = lane_value(alpha, q);
-                const float w = T * a;
-                cr = cr + w * lane_value(c.y, q);
-                cg = cg + w * lane_value(c.z, q);
-                cb = cb + w * lane_value(c.w, q);
-                cd = cd + w * lane_value(z, q);
-                T = T * (1.0f - a);  // T_{i+1} enters the gradient (integrating.cu:190-192)
-                if (lane == q) {
-                    my_w = w;
-                    my_T = T;
-                    my_r = cr;
-                    my_g = cg;
-                    my_b = cb;
-                    my_d = cd;
-                }
-                done = q + 1;
-            }
-            if (lane < done) {
-                const size_t s = s0 + j;
-                dL_dz[s] = my_w * gr.w;
-                const float a0 = gr.x * ((my_T * c.y - (R.x - my_r)) - bg0 * bgk);
-                const float a1 = gr.y * ((my_T * c.z - (R.y - my_g)) - bg1 * bgk);
-                const float a2 = gr.z * ((my_T * c.w - (R.z - my_b)) - bg2 * bgk);
-                const float a3 = gr.w * (my_T * z - (R.w - my_d));
-                const float dsig = ds * (((a0 + a1) + a2) + a3);
-                const float reg = ((double)c.x > 4e-5 && z < near_distance) ? 1e-4f : 0.0f;
-                const float gs = fminf(z * z, 1.0f);
-                *reinterpret_cast<float4 *>(dL_ddrgbs + 4 * s) =
-                    make_float4(gs * dsig + reg, my_w * gr.x, my_w * gr.y, my_w * gr.z);
-            }
+    const uint32_t wave = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    for (uint64_t first = wave; first < n_rays; first += (uint64_t)kWave * n_waves) {
+        const RayBatch rb = load_ray_batch(first, n_waves, n_rays, startidx, nsamp, total_samples);
+        if (rb.in && rb.n == 0) {  // T = 1 > threshold: only the background gets a gradient
+            const float4 gr = *reinterpret_cast<const float4 *>(dL_drgbd + 4 * rb.ray);
+            dL_dbgs[3 * rb.ray + 0] = 1.0f * gr.x;
+            dL_dbgs[3 * rb.ray + 1] = 1.0f * gr.y;
+            dL_dbgs[3 * rb.ray + 2] = 1.0f * gr.z;
         }
-        if (lane == 0) {
-            const bool live = T > kTThreshold;
-            dL_dbgs[3 * (size_t)ray + 0] = live ? T * gr.x : 0.0f;
-            dL_dbgs[3 * (size_t)ray + 1] = live ? T * gr.y : 0.0f;
-            dL_dbgs[3 * (size_t)ray + 2] = live ? T * gr.z : 0.0f;
+        for (uint64_t todo = __ballot(rb.in && rb.n > 0); todo != 0; todo &= todo - 1) {
+            const int src = __builtin_ctzll(todo);
+            const uint32_t ray = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rb.ray, src);
+            const size_t s0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rb.s0, src);
+            const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)rb.n, src);
+            const float4 R = *reinterpret_cast<const float4 *>(final_rgbds + 4 * (size_t)ray);
+            const float4 gr = *reinterpret_cast<const float4 *>(dL_drgbd + 4 * (size_t)ray);
+            const float bg0 = bgs[3 * (size_t)ray], bg1 = bgs[3 * (size_t)ray + 1], bg2 = bgs[3 * (size_t)ray + 2];
+            const float op = final_opacities[ray];
+            const bool terminated = op >= 1.0f - kTThreshold;
+            const float bgk = terminated ? 0.0f : 1.0f - op;
+            float T = 1.0f, acc = 0.0f;
+            Chunk next = load_chunk(drgbs, zs, dss, s0, 0, n);
+            for (uint32_t c0 = 0; c0 < n && T > kTThreshold; c0 += kWave) {
+                const Chunk ch = next;
+                if (c0 + kWave < n) next = load_chunk(drgbs, zs, dss, s0, c0 + kWave, n);
+                const int m = (int)min(n - c0, (uint32_t)kWave);
+                const float alpha = lane < m ? 1.0f - det_expf(-ch.c.x * ch.ds) : 0.0f;
+                const float om = lane < m ? 1.0f - alpha : 1.0f;
+                float Tq;
+                const int stop = transmittance_chunk(rows, om, m, T, Tq);
+                const float w = Tq * alpha;
+                const float T1 = Tq * om;  // T_{i+1} enters the gradient (integrating.cu:190-192)
+                const bool in = lane < stop;
+                prods[lane] = in ? w * ch.c.y : 0.0f;
+                prods[kRow + lane] = in ? w * ch.c.z : 0.0f;
+                prods[2 * kRow + lane] = in ? w * ch.c.w : 0.0f;
+                prods[3 * kRow + lane] = in ? w * ch.z : 0.0f;
+                __builtin_amdgcn_wave_barrier();
+                ordered_sums(prods, acc, true);
+                __builtin_amdgcn_wave_barrier();
+                if (in) {
+                    const float my_r = prods[lane], my_g = prods[kRow + lane], my_b = prods[2 * kRow + lane],
+                                my_d = prods[3 * kRow + lane];
+                    const size_t s = s0 + c0 + lane;
+                    dL_dz[s] = w * gr.w;
+                    const float a0 = gr.x * ((T1 * ch.c.y - (R.x - my_r)) - bg0 * bgk);
+                    const float a1 = gr.y * ((T1 * ch.c.z - (R.y - my_g)) - bg1 * bgk);
+                    const float a2 = gr.z * ((T1 * ch.c.w - (R.z - my_b)) - bg2 * bgk);
+                    const float a3 = gr.w * (T1 * ch.z - (R.w - my_d));
+                    const float dsig = ch.ds * (((a0 + a1) + a2) + a3);
+                    const float reg = ((double)ch.c.x > 4e-5 && ch.z < near_distance) ? 1e-4f : 0.0f;
+                    const float gs = fminf(ch.z * ch.z, 1.0f);
+                    *reinterpret_cast<float4 *>(dL_ddrgbs + 4 * s) =
+                        make_float4(gs * dsig + reg, w * gr.x, w * gr.y, w * gr.z);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (lane == 0) {
+                const bool live = T > kTThreshold;
+                dL_dbgs[3 * (size_t)ray + 0] = live ? T * gr.x : 0.0f;
+                dL_dbgs[3 * (size_t)ray + 1] = live ? T * gr.y : 0.0f;
+                dL_dbgs[3 * (size_t)ray + 2] = live ? T * gr.z : 0.0f;
+            }
         }
     }
 }
