@@ -148,7 +148,7 @@ constexpr int kChunk = 1024;         // points per work item
 constexpr int kItemThreads = 256;    // 4 points per thread per item
 constexpr int kReduceThreads = 1024;
 constexpr uint32_t kPart = 32768;    // entries one reduce workgroup sums
-constexpr uint32_t kMaxSlices = 4096;
+constexpr uint32_t kMaxSlices = 8192;
 constexpr uint32_t kSliceBytes = 131072;
 constexpr int kScanThreads = 1024;
 constexpr int kScanPer = 4;  // elements per thread in the scan kernels
@@ -165,13 +165,13 @@ struct BwdPlan {
     uint64_t M;             // n_slices * n_items counters
     uint64_t scan_blocks;   // blocks of the counter scan
     uint32_t max_parts;     // upper bound of the reduce grid
-    uint64_t counts_off, sums_off, meta_off, parts_off, partials_off, entries_off, bytes;
+    uint64_t counts_off, sums_off, meta_off, parts_off, part_slice_off, partials_off, entries_off, bytes;
 };
 
 inline BwdPlan make_plan(const HashGridDescriptor &d) {
     auto align = [](uint64_t x) { return (x + 255) / 256 * 256; };
     BwdPlan p;
-    p.R = std::max<uint32_t>(64, kSliceBytes / (4 * d.F));
+    p.R = std::max<uint32_t>(64, kSliceBytes / (8 * d.F));  // one f64 accumulator per row and feature
     p.n_slices = div_up(d.n_rows, p.R);
     p.n_chunks = std::max<uint32_t>(1, div_up(d.n_points, kChunk));
     p.n_items = d.L * p.n_chunks;
@@ -185,7 +185,8 @@ inline BwdPlan make_plan(const HashGridDescriptor &d) {
     p.sums_off = align(p.counts_off + 4 * p.M);
     p.meta_off = align(p.sums_off + 4 * p.scan_blocks);        // [0] total entries, [1] total parts
     p.parts_off = align(p.meta_off + 8);                          // first part of each slice
-    p.partials_off = align(p.parts_off + 4 * ((uint64_t)p.n_slices + 1));
+    p.part_slice_off = align(p.parts_off + 4 * ((uint64_t)p.n_slices + 1));  // slice of each part
+    p.partials_off = align(p.part_slice_off + 4 * (uint64_t)p.max_parts);
     p.entries_off = align(p.partials_off + (uint64_t)p.max_parts * p.R * d.F * 4);
     p.bytes = p.entries_off + max_entries * 4 * ew;
     return p;
@@ -206,24 +207,34 @@ __device__ __forceinline__ void item_contributions(const HashGridDescriptor &d, 
     const float scale = d.scales[l];
     const uint32_t res = d.resolutions[l];
     const bool dense = l < d.first_hash_level;
-    for (int sub = 0; sub < kChunk / kItemThreads; ++sub) {
+    constexpr int kSubs = kChunk / kItemThreads;
+    // every input of the item is loaded before any is used, so the loads overlap
+    float xyz[kSubs][3], gin[kSubs][F];
+#pragma unroll
+    for (int sub = 0; sub < kSubs; ++sub) {
+        const uint32_t n = k * kChunk + sub * kItemThreads + threadIdx.x;
+        const uint32_t np = n < d.n_points ? n : d.n_points - 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) xyz[sub][a] = pos[3 * (size_t)np + a];
+        if (kValues) {
+            const size_t gb = level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
+#pragma unroll
+            for (int f = 0; f < F; ++f) gin[sub][f] = dL_denc[gb + f];
+        }
+    }
+#pragma unroll
+    for (int sub = 0; sub < kSubs; ++sub) {
         const uint32_t n = k * kChunk + sub * kItemThreads + threadIdx.x;
         const bool active = n < d.n_points;
-        const uint32_t np = active ? n : d.n_points - 1;
-        const float px = (pos[3 * (size_t)np + 0] + d.bound) / two_b;
-        const float py = (pos[3 * (size_t)np + 1] + d.bound) / two_b;
-        const float pz = (pos[3 * (size_t)np + 2] + d.bound) / two_b;
+        const float px = (xyz[sub][0] + d.bound) / two_b;
+        const float py = (xyz[sub][1] + d.bound) / two_b;
+        const float pz = (xyz[sub][2] + d.bound) / two_b;
         uint32_t idx[8];
         float w[8];
         level_vertices(px, py, pz, scale, res, dense, d.T, pow2, d.offsets[l], d.n_rows - 1u, idx, w);
         float g[F];
 #pragma unroll
-        for (int f = 0; f < F; ++f) g[f] = 0.0f;
-        if (kValues && active) {
-            const size_t gb = level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
-#pragma unroll
-            for (int f = 0; f < F; ++f) g[f] = dL_denc[gb + f];
-        }
+        for (int f = 0; f < F; ++f) g[f] = kValues && active ? gin[sub][f] : 0.0f;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             // inactive lanes get keys no active lane has, so they form no runs
@@ -327,10 +338,11 @@ __device__ __forceinline__ uint32_t slice_begin(const BwdPlan &p, const uint32_t
 }
 
 // 2d. one workgroup: parts per slice (kPart entries each, at least one) and their
-// prefix; parts[n_slices] and meta[1] hold the total.
+// prefix, and the slice of every part; parts[n_slices] and meta[1] hold the total.
 __global__ void __launch_bounds__(kScanThreads) plan_parts_kernel(BwdPlan p, const uint32_t *__restrict__ offsets,
                                                                   uint32_t *__restrict__ meta,
-                                                                  uint32_t *__restrict__ parts) {
+                                                                  uint32_t *__restrict__ parts,
+                                                                  uint32_t *__restrict__ part_slice) {
     uint32_t carry = 0;
     for (uint32_t c = 0; c < p.n_slices; c += kScanThreads) {
         const uint32_t s = c + threadIdx.x;
@@ -341,7 +353,10 @@ __global__ void __launch_bounds__(kScanThreads) plan_parts_kernel(BwdPlan p, con
         }
         uint32_t tot;
         const uint32_t ex = block_exclusive_scan<kScanThreads>(k, &tot);
-        if (s < p.n_slices) parts[s] = carry + ex;
+        if (s < p.n_slices) {
+            parts[s] = carry + ex;
+            for (uint32_t j = 0; j < k; ++j) part_slice[carry + ex + j] = s;
+        }
         carry += tot;
     }
     if (threadIdx.x == 0) {
@@ -399,59 +414,55 @@ __device__ __forceinline__ void load_entry(const uint32_t *__restrict__ src, uin
 
 // 4. workgroup b sums part b's entries in LDS.  A slice of one part writes its
 // rows of the gradient; a slice of several writes partial rows, summed in order
-// by combine_kernel.
+// by combine_kernel.  The LDS sums are f64 (each row rounded to f32 once at the
+// end): f64 LDS atomics run several times faster than f32 ones here, and they
+// keep full relative precision whatever the magnitudes, with inf and NaN
+// propagating as in a float sum.
 template <int F>
 __global__ void __launch_bounds__(kReduceThreads)
     bwd_reduce_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ offsets,
                       const uint32_t *__restrict__ meta, const uint32_t *__restrict__ parts,
-                      const uint32_t *__restrict__ entries, float *__restrict__ partials, float *__restrict__ dtable) {
+                      const uint32_t *__restrict__ part_slice, const uint32_t *__restrict__ entries,
+                      float *__restrict__ partials, float *__restrict__ dtable) {
     constexpr int EW = entry_words<F>();
-    extern __shared__ float acc[];
+    extern __shared__ double acc[];
     const uint32_t b = blockIdx.x;
     if (b >= meta[1]) return;
-    // slice s: the last with parts[s] <= b
-    uint32_t lo = 0, hi = p.n_slices - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        if (parts[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const uint32_t s = lo;
+    const uint32_t s = part_slice[b];
     const uint32_t part = b - parts[s];
     const uint32_t n_parts = parts[s + 1] - parts[s];
     const uint32_t s_begin = slice_begin(p, offsets, meta, s), s_end = slice_begin(p, offsets, meta, s + 1);
     const uint32_t begin = s_begin + part * kPart;
     const uint32_t end = min(s_end, begin + kPart);
     const uint32_t rows = min(d.n_rows - s * p.R, p.R);
-    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) acc[t] = 0.0f;
+    // feature f of local row r at acc[f * R + r]: one atomic instruction's rows
+    // spread over all banks
+    for (uint32_t t = threadIdx.x; t < p.R * F; t += kReduceThreads) acc[t] = 0.0;
     __syncthreads();
-    // four entries in flight per thread
-    constexpr int kU = 4;
+    // kU entries in flight per thread: the loads are unconditional (indices past the
+    // part clamp to its last entry and are dropped afterwards), so they issue back
+    // to back instead of each waiting in its own branch
+    constexpr int kU = 8;
     for (uint32_t e0 = begin; e0 < end; e0 += kU * kReduceThreads) {
         uint32_t w[kU][EW];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const uint32_t e = e0 + u * kReduceThreads + threadIdx.x;
-            if (e < end) {
-                load_entry<EW>(entries + (size_t)e * EW, w[u]);
-            } else {
-                w[u][0] = 0u;
-#pragma unroll
-                for (int f = 1; f < EW; ++f) w[u][f] = 0u;  // zero values add nothing
-            }
+            const uint32_t e = min(e0 + u * kReduceThreads + threadIdx.x, end - 1u);
+            load_entry<EW>(entries + (size_t)e * EW, w[u]);
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
+            const bool in = e0 + u * kReduceThreads + threadIdx.x < end;
 #pragma unroll
             for (int f = 0; f < F; ++f) {
                 const float v = __uint_as_float(w[u][1 + f]);
-                if (v != 0.0f) atomicAdd(&acc[w[u][0] * F + f], v);
+                if (in && v != 0.0f) atomicAdd(&acc[f * p.R + w[u][0]], (double)v);
             }
         }
     }
     __syncthreads();
     float *out = n_parts == 1 ? dtable + (size_t)s * p.R * F : partials + (size_t)b * p.R * F;
-    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) out[t] = acc[t];
+    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) out[t] = (float)acc[(t % F) * p.R + t / F];
 }
 
 // 5. slices of several parts: rows = sum of the parts' partial rows, in part
@@ -499,9 +510,10 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
     uint32_t *sums = reinterpret_cast<uint32_t *>(ws + p.sums_off);
     uint32_t *meta = reinterpret_cast<uint32_t *>(ws + p.meta_off);
     uint32_t *parts = reinterpret_cast<uint32_t *>(ws + p.parts_off);
+    uint32_t *part_slice = reinterpret_cast<uint32_t *>(ws + p.part_slice_off);
     float *partials = reinterpret_cast<float *>(ws + p.partials_off);
     uint32_t *entries = reinterpret_cast<uint32_t *>(ws + p.entries_off);
-    const size_t lds = (size_t)p.R * F * sizeof(float);
+    const size_t lds = (size_t)p.R * F * sizeof(double);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_reduce_kernel<F>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
@@ -514,11 +526,12 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
     hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kScanThreads), 0, stream, p.scan_blocks, sums, meta);
     hipLaunchKernelGGL(scan_apply_kernel, dim3((uint32_t)p.scan_blocks), dim3(kScanThreads), 0, stream, p.M, counts,
                        sums);
-    hipLaunchKernelGGL(plan_parts_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, counts, meta, parts);
+    hipLaunchKernelGGL(plan_parts_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, counts, meta, parts,
+                       part_slice);
     hipLaunchKernelGGL(bwd_scatter_kernel<F>, dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, g, counts,
                        entries);
     hipLaunchKernelGGL(bwd_reduce_kernel<F>, dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts, meta,
-                       parts, entries, partials, dtable);
+                       parts, part_slice, entries, partials, dtable);
     hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices, kCombineSplit), dim3(256), 0, stream, d, p, parts,
                        partials, dtable);
     return true;
