@@ -149,7 +149,7 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
     2^18 slots, median over the timed frames after three warm-up renders; PSNR of the
     u8 render against the float64 ground truth blended on white.  With `proposal`,
     the frames are rendered with 48 proposal-placed samples per ray (occupancy-grid
-    fallback below predicted mass 0.97) and the same field is also rendered the plain
+    fallback below predicted mass 0.98) and the same field is also rendered the plain
     way, for the PSNR difference; the same comparison at the fallback threshold 0.9
     is reported beside it."""
     import torch
@@ -193,7 +193,7 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
            "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
            "train_steps": tr.step_count}
     if proposal:
-        res.update(samples_per_ray=48, mass_threshold=0.97, placement="quantiles",
+        res.update(samples_per_ray=48, mass_threshold=0.98, placement="quantiles",
                    fallback_ray_fraction=float(np.mean(fallback)),
                    psnr_plain_render_db=float(np.mean(plain_psnrs)),
                    delta_psnr_db=float(np.mean(psnrs) - np.mean(plain_psnrs)),

@@ -192,6 +192,29 @@ inline BwdPlan make_plan(const HashGridDescriptor &d) {
     return p;
 }
 
+// Lane shifts on the vector ALU (DPP), no LDS: lanes without a source, or in rows
+// the row mask leaves out, get `fill`.
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, kCtrl, kRowMask, 0xf, false);
+}
+
+// One step of the segmented inclusive sum: combine with the pair the shift brings
+// (0 and flag 0 where there is none, which changes nothing).
+template <int kCtrl, int kRowMask, int F>
+__device__ __forceinline__ void seg_step(float *v, uint32_t &fl) {
+    const uint32_t fin = dpp_u32<kCtrl, kRowMask>(fl, 0u);
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+        const float vin = __uint_as_float(dpp_u32<kCtrl, kRowMask>(__float_as_uint(v[f]), 0u));
+        v[f] = fl ? v[f] : v[f] + vin;
+    }
+    fl |= fin;
+}
+
 // Visits the run-merged contributions of work item `item` and calls
 // emit(row, v) on each run's last lane.  The same inputs always give the same
 // multiset of emitted rows, so counting and scattering agree.
@@ -242,25 +265,24 @@ __device__ __forceinline__ void item_contributions(const HashGridDescriptor &d, 
             float v[F];
 #pragma unroll
             for (int f = 0; f < F; ++f) v[f] = w[c] * g[f];
-            const uint32_t prev = __shfl_up(key, 1, kWave);
-            const bool head = lane == 0 || prev != key;
+            const uint32_t prev = dpp_u32<kWaveShr1, 0xf>(key, ~key);  // lane 0: no predecessor
+            const bool head = prev != key;
             bool tail = true;
             if (__ballot(!head) != 0) {
                 if (kValues) {
-                    int seg = head ? lane : 0;
-#pragma unroll
-                    for (int dd = 1; dd < kWave; dd <<= 1) seg = max(seg, __shfl_up(seg, dd, kWave));
-#pragma unroll
-                    for (int dd = 1; dd < kWave; dd <<= 1) {
-#pragma unroll
-                        for (int f = 0; f < F; ++f) {
-                            const float o = __shfl_up(v[f], dd, kWave);
-                            if (lane - dd >= seg) v[f] += o;
-                        }
-                    }
+                    // segmented inclusive sums of runs: (v, flag) pairs combined by
+                    // (a, fa) + (b, fb) = (fb ? b : a + b, fa | fb), shifts within
+                    // 16-lane rows, then row 15 -> rows 1 and 3, lane 31 -> rows 2 and 3
+                    uint32_t fl = head ? 1u : 0u;
+                    seg_step<kRowShr1, 0xf, F>(v, fl);
+                    seg_step<kRowShr2, 0xf, F>(v, fl);
+                    seg_step<kRowShr4, 0xf, F>(v, fl);
+                    seg_step<kRowShr8, 0xf, F>(v, fl);
+                    seg_step<kRowBcast15, 0xa, F>(v, fl);
+                    seg_step<kRowBcast31, 0xc, F>(v, fl);
                 }
-                const int next_head = __shfl_down(head ? 1 : 0, 1, kWave);
-                tail = lane == kWave - 1 || next_head != 0;
+                const uint32_t next_head = dpp_u32<kWaveShl1, 0xf>(head ? 1u : 0u, 1u);  // lane 63: end of wave
+                tail = next_head != 0u;
             }
             if (tail && active) emit(key, v);
         }

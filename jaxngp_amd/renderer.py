@@ -189,9 +189,20 @@ def render_rays_inference(model, o: torch.Tensor, d: torch.Tensor, occupancy_bit
     return rays_rgbd, rays_bg
 
 
+_SIDE_STREAMS: dict = {}
+
+
+def _side_stream(dev) -> torch.cuda.Stream:
+    """A second stream per device (default priority) for work that overlaps the ray pool."""
+    key = torch.device(dev).index
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return _SIDE_STREAMS[key]
+
+
 @torch.no_grad()
 def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, occupancy_bitfield,
-                          bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.97,
+                          bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.98,
                           placement: int = _lib.PROPOSAL_QUANTILES, K=1, G=128,
                           diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8, pool: int = 1 << 18,
                           fallback_steps_cap: int = 32, stats: dict | None = None):
@@ -203,8 +214,8 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     runs on them (nerf_field_chunks) and integrate_rays composites them.  Rays whose
     predicted mass is below `mass_threshold` are rendered by the occupancy-grid ray
     pool (render_rays_inference) instead.  The proposal is trained to bound the
-    field's weights from above, so its mass over-reads; 0.97 is the default because
-    rays predicted between 0.9 and 0.97 are where its samples miss the field's mass.
+    field's weights from above, so its mass over-reads; 0.98 is the default because
+    rays predicted between 0.9 and 0.98 are where its samples miss the field's mass.
     The fallback marches `fallback_steps_cap` samples per ray and round: a ray's
     samples and colour do not depend on that cap (march_rays_inference resumes from
     the t it stopped at and defers a far-plane sample to the next round), so its
@@ -235,23 +246,34 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     half = getattr(model, "mlp_half", None) is not None
     params = model.mlp_half if half else model.mlp_params.detach()
     ident = torch.arange(n, dtype=torch.int32, device=dev).view(u32)
-    _lib.call("nerf_field_chunks", [xyzs, cnt, ident, d, pe.table.detach(), params, drgbs],
-              _lib.field_chunks_descriptor(pe.descriptor(0, model.bound), n, n_per_ray, n,
-                                           _lib.DTYPE_F16 if half else _lib.DTYPE_F32))
     bgs = torch.as_tensor(bg, dtype=torch.float32, device=dev).expand(n, 3).contiguous()
     rays_rgbd = torch.empty(n, 4, device=dev)
     op = torch.empty(n, device=dev)
     measured = torch.empty(1, dtype=u32, device=dev)
-    _lib.call("integrate_rays", [start, cnt, bgs, dss.reshape(-1), zs.reshape(-1), drgbs.reshape(-1, 4), measured,
-                                 rays_rgbd, op], _lib.integrating_descriptor(n, n * n_per_ray))
     idx = fb.nonzero().squeeze(1)
-    if stats is not None:
-        stats["fallback_rays"] = int(idx.numel())
-        stats["proposal_rays"] = int((cnt.view(torch.int32) > 0).sum().item())
+    # The proposal rays' field and compositing (side stream) and the fallback rays'
+    # ray pool (this stream) touch disjoint rays: they run concurrently, and the
+    # pool's host checks no longer leave the GPU idle.  Every buffer is allocated on
+    # this stream before the fork and outlives the join.
+    main = torch.cuda.current_stream(dev)
+    side = _side_stream(dev)
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        _lib.call("nerf_field_chunks", [xyzs, cnt, ident, d, pe.table.detach(), params, drgbs],
+                  _lib.field_chunks_descriptor(pe.descriptor(0, model.bound), n, n_per_ray, n,
+                                               _lib.DTYPE_F16 if half else _lib.DTYPE_F32))
+        _lib.call("integrate_rays", [start, cnt, bgs, dss.reshape(-1), zs.reshape(-1), drgbs.reshape(-1, 4),
+                                     measured, rays_rgbd, op], _lib.integrating_descriptor(n, n * n_per_ray))
+    rgbd_fb = None
     if idx.numel():
         rgbd_fb, _ = render_rays_inference(model, o[idx].contiguous(), d[idx].contiguous(), occupancy_bitfield, bg,
                                            K=K, G=G, diagonal_n_steps=diagonal_n_steps,
                                            stepsize_portion=stepsize_portion, march_steps_cap=fallback_steps_cap,
                                            pool=pool)
+    main.wait_stream(side)
+    if rgbd_fb is not None:
         rays_rgbd[idx] = rgbd_fb
+    if stats is not None:
+        stats["fallback_rays"] = int(idx.numel())
+        stats["proposal_rays"] = int((cnt.view(torch.int32) > 0).sum().item())
     return _image(camera, rays_rgbd, bgs)
