@@ -169,7 +169,41 @@ __device__ __forceinline__ void store_enc4(float *__restrict__ enc, bool level_m
     *reinterpret_cast<float2 *>(enc + ((l0 + 1) * n + s) * 2) = make_float2(v[2], v[3]);
 }
 
-// Forward of one 16-sample tile starting at sample `s0` (samples >= n are zero).
+// The inputs of one 16-sample tile: lane (i, g)'s encodings of sample s0 + i, its
+// direction and (backward) its output gradient.  Loads are unconditional (samples
+// past n read sample n - 1 and are then zeroed), so a tile's loads can be issued
+// one tile ahead of its use.
+struct TileIn {
+    v4f x[2], gin;
+    float dx, dy, dz;
+};
+
+__device__ __forceinline__ TileIn load_tile(const float *__restrict__ enc, bool level_major,
+                                            const float *__restrict__ dirs, const float *__restrict__ grads,
+                                            uint32_t n, size_t s0, int i, int g) {
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    TileIn t;
+    if (n == 0) {  // nothing to load (the backward still runs to write zero rows)
+        t.x[0] = t.x[1] = t.gin = zero;
+        t.dx = t.dy = t.dz = 0.0f;
+        return t;
+    }
+    const size_t s = s0 + i;
+    const bool live = s < n;
+    const size_t sc = live ? s : (size_t)n - 1;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) t.x[b] = load_enc4(enc, level_major, n, sc, b, g);
+    t.dx = dirs[3 * sc + 0];
+    t.dy = dirs[3 * sc + 1];
+    t.dz = dirs[3 * sc + 2];
+    t.gin = grads != nullptr ? *reinterpret_cast<const v4f *>(grads + 4 * sc) : zero;
+    if (!live) {
+        t.x[0] = t.x[1] = t.gin = zero;
+        t.dx = t.dy = t.dz = 0.0f;
+    }
+    return t;
+}
+
 // Forward of a tile whose encodings f.x are loaded, for direction (dx, dy, dz).
 __device__ __forceinline__ void forward_from_x(const float *lds, float dx, float dy, float dz, int i, int g, Fwd &f) {
     layer_fwd<32, 64, S_W1>(lds + L_W1, f.x, f.h1, i, g);
@@ -183,29 +217,6 @@ __device__ __forceinline__ void forward_from_x(const float *lds, float dx, float
     layer_fwd<64, 16, S_W5>(lds + L_W5, f.h3, &f.rr, i, g);
 }
 
-__device__ __forceinline__ void forward_tile(const float *lds, const float *__restrict__ enc, bool level_major,
-                                             const float *__restrict__ dirs, uint32_t n, size_t s0, int i, int g,
-                                             Fwd &f) {
-    const size_t s = s0 + i;
-    const bool live = s < n;
-#pragma unroll
-    for (int b = 0; b < 2; ++b) f.x[b] = live ? load_enc4(enc, level_major, n, s, b, g) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    layer_fwd<32, 64, S_W1>(lds + L_W1, f.x, f.h1, i, g);
-    relu<4>(f.h1);
-    layer_fwd<64, 16, S_W2>(lds + L_W2, f.h1, &f.c[0], i, g);
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-    if (live) {
-        dx = dirs[3 * s + 0];
-        dy = dirs[3 * s + 1];
-        dz = dirs[3 * s + 2];
-    }
-    f.c[1] = sh4_part(dx, dy, dz, g);
-    layer_fwd<32, 64, S_W3>(lds + L_W3, f.c, f.h2, i, g);
-    relu<4>(f.h2);
-    layer_fwd<64, 64, S_W4>(lds + L_W4, f.h2, f.h3, i, g);
-    relu<4>(f.h3);
-    layer_fwd<64, 16, S_W5>(lds + L_W5, f.h3, &f.rr, i, g);
-}
 
 __global__ void __launch_bounds__(kThreads)
     mlp_forward_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
@@ -217,10 +228,15 @@ __global__ void __launch_bounds__(kThreads)
     const uint32_t n_tiles = (n + kTile - 1) / kTile;
     const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / kWave;
     const uint32_t n_waves = gridDim.x * kWaves;
+    TileIn next = load_tile(enc, level_major, dirs, nullptr, n, (size_t)min(wave, n_tiles - 1) * kTile, i, g);
     for (uint32_t t = wave; t < n_tiles; t += n_waves) {
         const size_t s0 = (size_t)t * kTile;
+        const TileIn in = next;
+        next = load_tile(enc, level_major, dirs, nullptr, n, (size_t)min(t + n_waves, n_tiles - 1) * kTile, i, g);
         Fwd f;
-        forward_tile(lds, enc, level_major, dirs, n, s0, i, g, f);
+        f.x[0] = in.x[0];
+        f.x[1] = in.x[1];
+        forward_from_x(lds, in.dx, in.dy, in.dz, i, g, f);
         const size_t s = s0 + i;
         if (g == 0 && s < n) {
             const float sigma = det_expf(f.c[0][0]);
@@ -296,13 +312,18 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int k = 0; k < 16; ++k) gW4[k] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 
+    TileIn next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(wave, n_tiles - 1) * kTile, i, g);
     for (uint32_t t = wave; t < n_tiles; t += n_waves) {
         const size_t s0 = (size_t)t * kTile;
         const size_t s = s0 + i;
         const bool live = s < n;
+        const TileIn in = next;
+        next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(t + n_waves, n_tiles - 1) * kTile, i, g);
         Fwd f;
-        forward_tile(lds, enc, level_major, dirs, n, s0, i, g, f);
-        const v4f gin = live ? *reinterpret_cast<const v4f *>(dL_ddrgbs + 4 * s) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        f.x[0] = in.x[0];
+        f.x[1] = in.x[1];
+        forward_from_x(lds, in.dx, in.dy, in.dz, i, g, f);
+        const v4f gin = in.gin;
         // colour head: d(pre-sigmoid), features 0..2 live in lanes g == 0
         v4f dr = {0.0f, 0.0f, 0.0f, 0.0f};
         float dsig = 0.0f;
@@ -555,34 +576,6 @@ __device__ __forceinline__ void forward_from_x_h(const _Float16 *lds, float dx, 
     layer_fwd_h<2, 1, FT5>(lds + H_F5, h3in, &f.rr, i, g);
 }
 
-__device__ __forceinline__ void forward_tile_h(const _Float16 *lds, const float *__restrict__ enc, bool level_major,
-                                               const float *__restrict__ dirs, uint32_t n, size_t s0, int i, int g,
-                                               FwdH &f) {
-    const size_t s = s0 + i;
-    const bool live = s < n;
-#pragma unroll
-    for (int b = 0; b < 2; ++b) f.x[b] = live ? load_enc4(enc, level_major, n, s, b, g) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    const h8 xin = pack_h8(f.x[0], f.x[1]);
-    layer_fwd_h<1, 4, FT1>(lds + H_F1, &xin, f.h1, i, g);
-    relu<4>(f.h1);
-    const h8 h1in[2] = {pack_h8(f.h1[0], f.h1[1]), pack_h8(f.h1[2], f.h1[3])};
-    layer_fwd_h<2, 1, FT2>(lds + H_F2, h1in, &f.c[0], i, g);
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-    if (live) {
-        dx = dirs[3 * s + 0];
-        dy = dirs[3 * s + 1];
-        dz = dirs[3 * s + 2];
-    }
-    f.c[1] = sh4_part(dx, dy, dz, g);
-    const h8 cin = pack_h8(f.c[0], f.c[1]);
-    layer_fwd_h<1, 4, FT3>(lds + H_F3, &cin, f.h2, i, g);
-    relu<4>(f.h2);
-    const h8 h2in[2] = {pack_h8(f.h2[0], f.h2[1]), pack_h8(f.h2[2], f.h2[3])};
-    layer_fwd_h<2, 4, FT4>(lds + H_F4, h2in, f.h3, i, g);
-    relu<4>(f.h3);
-    const h8 h3in[2] = {pack_h8(f.h3[0], f.h3[1]), pack_h8(f.h3[2], f.h3[3])};
-    layer_fwd_h<2, 1, FT5>(lds + H_F5, h3in, &f.rr, i, g);
-}
 
 __global__ void __launch_bounds__(kThreads)
     mlp_forward_h_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
@@ -594,10 +587,15 @@ __global__ void __launch_bounds__(kThreads)
     const uint32_t n_tiles = (n + kTile - 1) / kTile;
     const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / kWave;
     const uint32_t n_waves = gridDim.x * kWaves;
+    TileIn next = load_tile(enc, level_major, dirs, nullptr, n, (size_t)min(wave, n_tiles - 1) * kTile, i, g);
     for (uint32_t t = wave; t < n_tiles; t += n_waves) {
         const size_t s0 = (size_t)t * kTile;
+        const TileIn in = next;
+        next = load_tile(enc, level_major, dirs, nullptr, n, (size_t)min(t + n_waves, n_tiles - 1) * kTile, i, g);
         FwdH f;
-        forward_tile_h(lds, enc, level_major, dirs, n, s0, i, g, f);
+        f.x[0] = in.x[0];
+        f.x[1] = in.x[1];
+        forward_from_x_h(lds, in.dx, in.dy, in.dz, i, g, f);
         const size_t s = s0 + i;
         if (g == 0 && s < n) {
             const float sigma = det_expf(f.c[0][0]);
@@ -651,13 +649,18 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int k = 0; k < 16; ++k) gW4[k] = zero;
 
+    TileIn next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(wave, n_tiles - 1) * kTile, i, g);
     for (uint32_t t = wave; t < n_tiles; t += n_waves) {
         const size_t s0 = (size_t)t * kTile;
         const size_t s = s0 + i;
         const bool live = s < n;
+        const TileIn in = next;
+        next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(t + n_waves, n_tiles - 1) * kTile, i, g);
         FwdH f;
-        forward_tile_h(lds, enc, level_major, dirs, n, s0, i, g, f);
-        const v4f gin = live ? *reinterpret_cast<const v4f *>(dL_ddrgbs + 4 * s) : zero;
+        f.x[0] = in.x[0];
+        f.x[1] = in.x[1];
+        forward_from_x_h(lds, in.dx, in.dy, in.dz, i, g, f);
+        const v4f gin = in.gin;
         v4f dr = zero;
         float dsig = 0.0f;
         if (g == 0) {

@@ -93,3 +93,22 @@ def test_mlp_level_major_encodings(dev):
         outs.append((out, denc, dp))
     for a, b in zip(outs[0], outs[1]):
         torch.testing.assert_close(a, b, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_mlp_no_samples(dev, half):
+    """n_samples = 0: the forward does nothing and the backward still writes an all-zero
+    weight gradient (its workspace rows are written, not left stale)."""
+    from jaxngp_amd import _lib
+    p = glorot_uniform_(torch.empty(N_PARAMS), torch.Generator().manual_seed(3)).to(dev)
+    p_t = p.half() if half else p
+    dtype = _lib.DTYPE_F16 if half else _lib.DTYPE_F32
+    e = torch.zeros(1, 32, device=dev)
+    d_t = torch.zeros(1, 3, device=dev)
+    g_t = torch.zeros(1, 4, device=dev)
+    denc = torch.zeros(1, 32, device=dev)
+    dp = torch.full((N_PARAMS,), 7.0, device=dev)
+    ws = torch.full((4, N_PARAMS), 5.0, device=dev)
+    _lib.call("nerf_mlp_backward", [e, d_t, p_t, g_t, denc, dp, ws], _lib.nerf_mlp_descriptor(0, 4, 0, dtype))
+    torch.cuda.synchronize()
+    assert (dp == 0).all() and (ws == 0).all()
