@@ -117,6 +117,32 @@ typedef struct HashGridDescriptor {
     uint32_t offsets[NGP_HASHGRID_MAX_LEVELS + 1];
 } HashGridDescriptor;
 
+/* jax-tcnn's hash-grid encoding (the reference's inference-time encoder,
+ * models/encoders.py:259-305, bound through deps/jax-tcnn/lib/ffi.cc and
+ * lib/impl/hashgrid.cu, which run tiny-cuda-nn v1.6's grid kernels).  The
+ * descriptor is jaxtcnn::HashGridDescriptor byte for byte
+ * (deps/jax-tcnn/lib/impl/tcnnutils.h:11-29).  Levels are unpadded: offsets[l+1] -
+ * offsets[l] = res_l^3 for dense levels, T for hashed ones. */
+typedef struct TcnnHashGridDescriptor {
+    uint32_t n_coords;      /* N */
+    uint32_t L;             /* levels (1..32) */
+    uint32_t F;             /* features per level: 2 or 4 */
+    uint32_t N_min;         /* base resolution */
+    float per_level_scale;  /* b */
+} TcnnHashGridDescriptor;
+
+/* Replaces jaxtcnn::hashgrid_encode (hashgrid.cu:20-96; operands and results as
+ * hashgrid.cu:35-42).
+ * in : offset_table u32[L+1] (device), coords f32[3, N], params f32[offsets[L], F]
+ * out: encodings f32[L*F, N], dy_dcoords f32[L*F, N, 3] */
+void tcnn_hashgrid_encode(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* Replaces jaxtcnn::hashgrid_encode_backward (hashgrid.cu:98-175; buffers as
+ * hashgrid.cu:111-118).
+ * in : offset_table u32[L+1], coords f32[3, N], dL_dy f32[L*F, N], dy_dcoords f32[L*F, N, 3]
+ * out: dL_dparams f32[offsets[L], F] (fully written), dL_dcoords f32[3, N] */
+void tcnn_hashgrid_encode_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
 /* Fused NeRF MLP of models/nerfs.py:39-86 with make_nerf_ngp's shapes
  * (nerfs.py:422-454): density MLP 32->64->16, SH degree 4 of the direction,
  * colour MLP (16+16)->64->64->3, no biases, ReLU hidden layers, trunc_exp density

@@ -32,6 +32,8 @@ TARGETS = (
     "integrate_rays_inference",
     "hashgrid_encode",
     "hashgrid_encode_backward",
+    "tcnn_hashgrid_encode",
+    "tcnn_hashgrid_encode_backward",
     "nerf_mlp_forward",
     "nerf_mlp_backward",
     "train_make_rays",
@@ -228,6 +230,11 @@ def hashgrid_descriptor(n_points, L, F, T, first_hash_level, table_dtype, n_rows
     of = list(offsets) + [offsets[-1]] * (M + 1 - len(offsets))
     return struct.pack(f"<7IfI{M}f{M}I{M + 1}I", n_points, L, F, T, first_hash_level, table_dtype, n_rows, bound,
                        enc_layout, *sc, *rs, *of)
+
+
+def tcnn_hashgrid_descriptor(n_coords, L, F, N_min, per_level_scale) -> bytes:
+    """jaxtcnn::HashGridDescriptor (deps/jax-tcnn/lib/impl/tcnnutils.h:11-29)."""
+    return struct.pack("<4If", n_coords, L, F, N_min, per_level_scale)
 
 
 def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0, enc_layout=ENC_POINT_MAJOR, param_dtype=DTYPE_F32) -> bytes:

@@ -49,6 +49,7 @@ SOURCES = {
     "nerf_mlp.hip": ["-ffp-contract=off"],
     "harness.hip": ["-ffp-contract=off"],
     "proposal.hip": ["-ffp-contract=off"],
+    "tcnn_hashgrid.hip": ["-ffp-contract=off"],
 }
 
 

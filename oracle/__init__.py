@@ -661,3 +661,147 @@ def proposal_sample64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9):
         ds[r, -1] = min(D / (n_samples * p[bl]) if p[bl] > 0 else D, a1 - t[r, -1])
         counts[r] = n_samples
     return counts, fallback, t, ds
+
+
+# ---------------------------------------------------------------------------
+# jax-tcnn hash grid (tiny-cuda-nn v1.6 kernel_grid as deps/jax-tcnn/lib/impl/
+# hashgrid.cu:20-96 launches it; tiny-cuda-nn is not vendored, so this restates its
+# published linear / coherent-prime grid encoding).  Test infrastructure only.
+# ---------------------------------------------------------------------------
+
+def _log2f(x: float) -> np.float32:
+    """The host's libm log2f (what the library's entry computes on the host)."""
+    import ctypes.util
+    m = ctypes.CDLL(ctypes.util.find_library("m"))
+    m.log2f.restype, m.log2f.argtypes = ctypes.c_float, [ctypes.c_float]
+    return np.float32(m.log2f(float(x)))
+
+
+def _fmaf(a, b, c):
+    """f32 fused multiply-add (the product of two f32 is exact in f64)."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
+
+
+def tcnn_levels(L, N_min, per_level_scale):
+    """f32 scales and resolutions: exp2(l log2(b)) N_min - 1, ceil + 1."""
+    l2 = _log2f(per_level_scale)
+    out = []
+    for l in range(L):
+        t = np.float32(l) * l2
+        e = np.float32(np.exp2(np.float64(t)))
+        scale = np.float32(e * np.float32(N_min)) - np.float32(1.0)
+        out.append((np.float32(scale), int(np.ceil(scale)) + 1))
+    return out
+
+
+def tcnn_index(size, res, x, y, z):
+    """Dense index while the stride stays within the level, else the coherent prime
+    hash; both modulo the level's size (u32 arithmetic)."""
+    x, y, z = (np.asarray(v, np.uint32) for v in (x, y, z))
+    size, res = np.uint32(size), np.uint32(res)
+    with np.errstate(over="ignore"):
+        stride = np.uint32(1)
+        index = np.zeros_like(x)
+        for p in (x, y, z):
+            if stride > size:
+                break
+            index = index + p * stride
+            stride = stride * res
+        if size < stride:
+            index = x ^ (y * np.uint32(2654435761)) ^ (z * np.uint32(805459861))
+    return index % size
+
+
+def tcnn_grid_forward(coords, table, offsets, L, F, N_min, per_level_scale):
+    """(encodings f32[L*F, N], dy_dcoords f32[L*F, N, 3]) in the kernels' f32 order."""
+    coords = np.asarray(coords, np.float32)
+    n = coords.shape[1]
+    enc = np.zeros((L * F, n), np.float32)
+    dy = np.zeros((L * F, n, 3), np.float32)
+    for l, (scale, res) in enumerate(tcnn_levels(L, N_min, per_level_scale)):
+        lo, hi = int(offsets[l]), int(offsets[l + 1])
+        size = hi - lo
+        grid = table[lo:hi]
+        pos = _fmaf(scale, coords, np.float32(0.5))
+        g = np.floor(pos).astype(np.float32)
+        frac = (pos - g).astype(np.float32)
+        gi = g.astype(np.int64).astype(np.uint32)
+        acc = np.zeros((n, F), np.float32)
+        for c in range(8):
+            w = np.ones(n, np.float32)
+            q = []
+            for d in range(3):
+                bit = (c >> d) & 1
+                w = (w * (frac[d] if bit else np.float32(1) - frac[d])).astype(np.float32)
+                q.append(gi[d] + np.uint32(bit))
+            rows = grid[tcnn_index(size, res, *q)]
+            acc = _fmaf(w[:, None], rows, acc)
+        enc[l * F:(l + 1) * F] = acc.T
+        for gd in range(3):
+            grad = np.zeros((n, F), np.float32)
+            others = [d for d in range(3) if d != gd]
+            for c in range(4):
+                w = np.full(n, scale, np.float32)
+                q = [None, None, None]
+                for k, d in enumerate(others):
+                    bit = (c >> k) & 1
+                    w = (w * (frac[d] if bit else np.float32(1) - frac[d])).astype(np.float32)
+                    q[d] = gi[d] + np.uint32(bit)
+                q[gd] = gi[gd]
+                left = grid[tcnn_index(size, res, *q)]
+                q[gd] = gi[gd] + np.uint32(1)
+                right = grid[tcnn_index(size, res, *q)]
+                grad = (grad + (w[:, None] * (right - left).astype(np.float32)).astype(np.float32)).astype(np.float32)
+            dy[l * F:(l + 1) * F, :, gd] = grad.T
+    return enc, dy
+
+
+def tcnn_grid_forward64(coords, table, offsets, L, F, N_min, per_level_scale):
+    """float64 encodings of the same grid (the f32 scales and resolutions), for
+    finite differences of dy_dcoords."""
+    coords = np.asarray(coords, np.float64)
+    n = coords.shape[1]
+    enc = np.zeros((L * F, n))
+    for l, (scale, res) in enumerate(tcnn_levels(L, N_min, per_level_scale)):
+        lo, hi = int(offsets[l]), int(offsets[l + 1])
+        grid = np.asarray(table[lo:hi], np.float64)
+        pos = float(scale) * coords + 0.5
+        g = np.floor(pos)
+        frac = pos - g
+        gi = g.astype(np.int64).astype(np.uint32)
+        acc = np.zeros((n, F))
+        for c in range(8):
+            w = np.ones(n)
+            q = []
+            for d in range(3):
+                bit = (c >> d) & 1
+                w = w * (frac[d] if bit else 1 - frac[d])
+                q.append(gi[d] + np.uint32(bit))
+            acc += w[:, None] * grid[tcnn_index(hi - lo, res, *q)]
+        enc[l * F:(l + 1) * F] = acc.T
+    return enc
+
+
+def tcnn_grid_backward64(coords, dL_dy, dy_dcoords, offsets, L, F, N_min, per_level_scale, n_rows):
+    """float64 (dL_dparams [n_rows, F], dL_dcoords [3, N])."""
+    coords = np.asarray(coords, np.float32)
+    n = coords.shape[1]
+    dt = np.zeros((n_rows, F))
+    for l, (scale, res) in enumerate(tcnn_levels(L, N_min, per_level_scale)):
+        lo, hi = int(offsets[l]), int(offsets[l + 1])
+        pos = _fmaf(scale, coords, np.float32(0.5))
+        g = np.floor(pos).astype(np.float32)
+        frac = (pos - g).astype(np.float32)
+        gi = g.astype(np.int64).astype(np.uint32)
+        gl = np.asarray(dL_dy[l * F:(l + 1) * F], np.float64).T
+        for c in range(8):
+            w = np.ones(n, np.float32)
+            q = []
+            for d in range(3):
+                bit = (c >> d) & 1
+                w = (w * (frac[d] if bit else np.float32(1) - frac[d])).astype(np.float32)
+                q.append(gi[d] + np.uint32(bit))
+            idx = lo + tcnn_index(hi - lo, res, *q).astype(np.int64)
+            np.add.at(dt, idx, w.astype(np.float64)[:, None] * gl)
+    dcoords = np.einsum("kn,knd->dn", np.asarray(dL_dy, np.float64), np.asarray(dy_dcoords, np.float64))
+    return dt, dcoords

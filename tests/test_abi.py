@@ -48,6 +48,8 @@ def test_descriptor_sizes_match_c_structs():
     assert len(_lib.packbits_descriptor(5)) == 4
     assert len(_lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])) == 9 * 4 + 32 * 4 * 2 + 33 * 4
     assert len(_lib.nerf_mlp_descriptor(5, 0)) == 16
+    # jaxtcnn::HashGridDescriptor (deps/jax-tcnn/lib/impl/tcnnutils.h:11-29): 4 u32 + f32
+    assert len(_lib.tcnn_hashgrid_descriptor(7, 16, 2, 16, 1.38)) == 20
     assert len(_lib.train_rays_descriptor(1, 2, 3, 4, 1.0, 1.0, 1.0, 1.0, 1.0, 5, 6, 3, (1.0, 1.0, 1.0))) == 64
     assert len(_lib.loss_descriptor(4, 0.1)) == 12
     assert len(_lib.adam_descriptor(10, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0)) == 56
@@ -93,6 +95,7 @@ _Static_assert(sizeof(ProposalLossDescriptor) == 8, "");
 _Static_assert(sizeof(ProposalSampleDescriptor) == 40, "");
 _Static_assert(sizeof(ProposalRenderDescriptor) == sizeof(HashGridDescriptor) + 44, "");
 _Static_assert(sizeof(NerfFieldChunksDescriptor) == sizeof(HashGridDescriptor) + 16, "");
+_Static_assert(sizeof(TcnnHashGridDescriptor) == 20, "");
 int main(void) { return 0; }
 '''
     out = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{REPO / 'include'}", "-x", "c", "-", "-o",
