@@ -157,7 +157,8 @@ constexpr int kScanPer = 4;  // elements per thread in the scan kernels
 // that it is written and read with one or two vector accesses.
 template <int F>
 constexpr int entry_words() {
-    return F == 1 ? 2 : (F == 2 ? 4 : (F == 4 ? 8 : 16));
+    // row in slice + F values; F = 2 packs into 12 bytes (one dwordx3 access)
+    return F == 1 ? 2 : (F == 2 ? 3 : (F == 4 ? 8 : 16));
 }
 
 struct BwdPlan {
@@ -180,7 +181,7 @@ inline BwdPlan make_plan(const HashGridDescriptor &d) {
     const uint64_t max_entries = (uint64_t)d.n_points * d.L * 8;
     // a slice with c entries takes max(1, ceil(c / kPart)) parts
     p.max_parts = (uint32_t)(p.n_slices + max_entries / kPart);
-    const int ew = d.F == 1 ? 2 : (d.F == 2 ? 4 : (d.F == 4 ? 8 : 16));
+    const int ew = d.F == 1 ? 2 : (d.F == 2 ? 3 : (d.F == 4 ? 8 : 16));
     p.counts_off = 0;
     p.sums_off = align(p.counts_off + 4 * p.M);
     p.meta_off = align(p.sums_off + 4 * p.scan_blocks);        // [0] total entries, [1] total parts
@@ -409,6 +410,8 @@ __global__ void __launch_bounds__(kItemThreads)
         uint4 *e = reinterpret_cast<uint4 *>(entries + (size_t)at * EW);
         if constexpr (EW == 2) {
             *reinterpret_cast<uint2 *>(entries + (size_t)at * EW) = make_uint2(w[0], w[1]);
+        } else if constexpr (EW == 3) {
+            *reinterpret_cast<uint3 *>(entries + (size_t)at * EW) = make_uint3(w[0], w[1], w[2]);
         } else {
 #pragma unroll
             for (int q = 0; q < EW / 4; ++q) e[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
@@ -422,6 +425,11 @@ __device__ __forceinline__ void load_entry(const uint32_t *__restrict__ src, uin
         const uint2 q = *reinterpret_cast<const uint2 *>(src);
         w[0] = q.x;
         w[1] = q.y;
+    } else if constexpr (EW == 3) {
+        const uint3 q = *reinterpret_cast<const uint3 *>(src);
+        w[0] = q.x;
+        w[1] = q.y;
+        w[2] = q.z;
     } else {
 #pragma unroll
         for (int q = 0; q < EW / 4; ++q) {
