@@ -155,10 +155,42 @@ constexpr int kScanPer = 4;  // elements per thread in the scan kernels
 
 // An entry is (row within slice, F values), padded to a power of two of words so
 // that it is written and read with one or two vector accesses.
-template <int F>
+// Words of one sort entry: the row in its slice, then the F values, as f32 (F = 2:
+// 12 bytes, one dwordx3 access) or, for f16 tables with F = 2 or 4, as f16 pairs
+// (F = 2: 8 bytes) — the precision path accumulates its table gradient from
+// f16-rounded contributions, as half-precision grids are trained in tiny-cuda-nn.
+template <int F, bool kHalf>
 constexpr int entry_words() {
-    // row in slice + F values; F = 2 packs into 12 bytes (one dwordx3 access)
-    return F == 1 ? 2 : (F == 2 ? 3 : (F == 4 ? 8 : 16));
+    return kHalf && F == 2 ? 2 : (kHalf && F == 4 ? 3 : (F == 1 ? 2 : (F == 2 ? 3 : (F == 4 ? 8 : 16))));
+}
+
+inline int entry_words_host(uint32_t F, bool half) {
+    return half && F == 2 ? 2 : (half && F == 4 ? 3 : (F == 1 ? 2 : (F == 2 ? 3 : (F == 4 ? 8 : 16))));
+}
+
+template <int F, bool kHalf>
+__device__ __forceinline__ void pack_values(uint32_t *w, const float *v) {
+    if constexpr (kHalf && (F == 2 || F == 4)) {
+#pragma unroll
+        for (int q = 0; q < F / 2; ++q) {
+            const __half2 h = __floats2half2_rn(v[2 * q], v[2 * q + 1]);
+            w[1 + q] = *reinterpret_cast<const uint32_t *>(&h);
+        }
+    } else {
+#pragma unroll
+        for (int f = 0; f < F; ++f) w[1 + f] = __float_as_uint(v[f]);
+    }
+}
+
+template <int F, bool kHalf>
+__device__ __forceinline__ float entry_value(const uint32_t *w, int f) {
+    if constexpr (kHalf && (F == 2 || F == 4)) {
+        const uint32_t u = w[1 + f / 2];
+        const __half2 h = *reinterpret_cast<const __half2 *>(&u);
+        return (f & 1) ? __high2float(h) : __low2float(h);
+    } else {
+        return __uint_as_float(w[1 + f]);
+    }
 }
 
 struct BwdPlan {
@@ -181,7 +213,7 @@ inline BwdPlan make_plan(const HashGridDescriptor &d) {
     const uint64_t max_entries = (uint64_t)d.n_points * d.L * 8;
     // a slice with c entries takes max(1, ceil(c / kPart)) parts
     p.max_parts = (uint32_t)(p.n_slices + max_entries / kPart);
-    const int ew = d.F == 1 ? 2 : (d.F == 2 ? 3 : (d.F == 4 ? 8 : 16));
+    const int ew = entry_words_host(d.F, d.table_dtype == NGP_DTYPE_F16);
     p.counts_off = 0;
     p.sums_off = align(p.counts_off + 4 * p.M);
     p.meta_off = align(p.sums_off + 4 * p.scan_blocks);        // [0] total entries, [1] total parts
@@ -389,11 +421,11 @@ __global__ void __launch_bounds__(kScanThreads) plan_parts_kernel(BwdPlan p, con
 }
 
 // 3. entries at each item's offsets
-template <int F>
+template <int F, bool kHalf>
 __global__ void __launch_bounds__(kItemThreads)
     bwd_scatter_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos, const float *__restrict__ dL_denc,
                        const uint32_t *__restrict__ offsets, uint32_t *__restrict__ entries) {
-    constexpr int EW = entry_words<F>();
+    constexpr int EW = entry_words<F, kHalf>();
     __shared__ uint32_t cursor[kMaxSlices];
     const uint32_t item = blockIdx.x;
     for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) cursor[s] = offsets[(size_t)s * p.n_items + item];
@@ -402,11 +434,10 @@ __global__ void __launch_bounds__(kItemThreads)
         const uint32_t s = row / p.R;
         const uint32_t at = atomicAdd(&cursor[s], 1u);
         uint32_t w[EW];
+#pragma unroll
+        for (int k = 0; k < EW; ++k) w[k] = 0u;
         w[0] = row - s * p.R;
-#pragma unroll
-        for (int f = 0; f < F; ++f) w[1 + f] = __float_as_uint(v[f]);
-#pragma unroll
-        for (int f = F + 1; f < EW; ++f) w[f] = 0u;
+        pack_values<F, kHalf>(w, v);
         uint4 *e = reinterpret_cast<uint4 *>(entries + (size_t)at * EW);
         if constexpr (EW == 2) {
             *reinterpret_cast<uint2 *>(entries + (size_t)at * EW) = make_uint2(w[0], w[1]);
@@ -448,13 +479,13 @@ __device__ __forceinline__ void load_entry(const uint32_t *__restrict__ src, uin
 // end): f64 LDS atomics run several times faster than f32 ones here, and they
 // keep full relative precision whatever the magnitudes, with inf and NaN
 // propagating as in a float sum.
-template <int F>
+template <int F, bool kHalf>
 __global__ void __launch_bounds__(kReduceThreads)
     bwd_reduce_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ offsets,
                       const uint32_t *__restrict__ meta, const uint32_t *__restrict__ parts,
                       const uint32_t *__restrict__ part_slice, const uint32_t *__restrict__ entries,
                       float *__restrict__ partials, float *__restrict__ dtable) {
-    constexpr int EW = entry_words<F>();
+    constexpr int EW = entry_words<F, kHalf>();
     extern __shared__ double acc[];
     const uint32_t b = blockIdx.x;
     if (b >= meta[1]) return;
@@ -485,7 +516,7 @@ __global__ void __launch_bounds__(kReduceThreads)
             const bool in = e0 + u * kReduceThreads + threadIdx.x < end;
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                const float v = __uint_as_float(w[u][1 + f]);
+                const float v = entry_value<F, kHalf>(w[u], f);
                 if (in && v != 0.0f) atomicAdd(&acc[f * p.R + w[u][0]], (double)v);
             }
         }
@@ -532,7 +563,7 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-template <int F>
+template <int F, bool kHalf>
 bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const float *pos, const float *g, float *dtable,
                      uint8_t *ws, const char *target) {
     const BwdPlan p = make_plan(d);
@@ -544,7 +575,7 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
     float *partials = reinterpret_cast<float *>(ws + p.partials_off);
     uint32_t *entries = reinterpret_cast<uint32_t *>(ws + p.entries_off);
     const size_t lds = (size_t)p.R * F * sizeof(double);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_reduce_kernel<F>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_reduce_kernel<F, kHalf>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
         set_error(kLaunchFailed, "%s: cannot reserve %zu bytes of LDS: %s", target, lds, hipGetErrorString(e));
@@ -558,9 +589,9 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
                        sums);
     hipLaunchKernelGGL(plan_parts_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, counts, meta, parts,
                        part_slice);
-    hipLaunchKernelGGL(bwd_scatter_kernel<F>, dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, g, counts,
+    hipLaunchKernelGGL((bwd_scatter_kernel<F, kHalf>), dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, g, counts,
                        entries);
-    hipLaunchKernelGGL(bwd_reduce_kernel<F>, dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts, meta,
+    hipLaunchKernelGGL((bwd_reduce_kernel<F, kHalf>), dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts, meta,
                        parts, part_slice, entries, partials, dtable);
     hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices, kCombineSplit), dim3(256), 0, stream, d, p, parts,
                        partials, dtable);
@@ -694,11 +725,18 @@ extern "C" void hashgrid_encode_backward(ngp_stream_t stream_, void **buffers, c
         return;
     }
     bool ok = false;
+    const bool half = d.table_dtype == NGP_DTYPE_F16;
     switch (d.F) {
-        case 1: ok = launch_backward<1>(d, stream, pos, g, dtable, ws, target); break;
-        case 2: ok = launch_backward<2>(d, stream, pos, g, dtable, ws, target); break;
-        case 4: ok = launch_backward<4>(d, stream, pos, g, dtable, ws, target); break;
-        default: ok = launch_backward<8>(d, stream, pos, g, dtable, ws, target); break;
+        case 1: ok = launch_backward<1, false>(d, stream, pos, g, dtable, ws, target); break;
+        case 2:
+            ok = half ? launch_backward<2, true>(d, stream, pos, g, dtable, ws, target)
+                      : launch_backward<2, false>(d, stream, pos, g, dtable, ws, target);
+            break;
+        case 4:
+            ok = half ? launch_backward<4, true>(d, stream, pos, g, dtable, ws, target)
+                      : launch_backward<4, false>(d, stream, pos, g, dtable, ws, target);
+            break;
+        default: ok = launch_backward<8, false>(d, stream, pos, g, dtable, ws, target); break;
     }
     if (!ok) return;
     check_launch(target);

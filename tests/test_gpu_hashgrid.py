@@ -159,7 +159,7 @@ def _backward_raw(enc, pos_t, g_t):
     N = pos_t.shape[0]
     desc = enc.descriptor(N, 1.0)
     ws = torch.empty(_lib.hashgrid_backward_workspace_bytes(desc), dtype=torch.uint8, device=pos_t.device)
-    d = torch.empty_like(enc.table.detach())
+    d = torch.empty(tuple(enc.table.shape), dtype=torch.float32, device=pos_t.device)  # the gradient is f32
     _lib.call("hashgrid_encode_backward", [pos_t, g_t, d, ws], desc)
     return d
 
@@ -204,3 +204,33 @@ def test_backward_non_finite_reaches_its_rows(dev):
     absw = _c_backward(enc, pos, np.abs(gf))
     ok = ~bad
     assert (np.abs(d[ok].astype(np.float64) - want[ok]) <= 1e-5 * absw[ok] + 1e-30).all()
+
+
+def test_fp16_table_backward():
+    """f16 tables (the precision path) sort f16-rounded contributions: the gradient
+    is within 1e-3 of the float64 scatter relative to each row's absolute
+    contributions (f16 keeps 11 bits) plus 1e-6 absolute (contributions below f16's
+    smallest subnormal, 6e-8, round to zero; the loss scale keeps training's far
+    above it), untouched rows stay 0, and a contribution past the f16 range reaches
+    its rows as inf (what the loss-scale check looks for)."""
+    from jaxngp_amd import _lib
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(31)
+    enc = HashGridEncoder(**CONFIGS["c0"], param_dtype=torch.float16, generator=torch.Generator().manual_seed(4)).to(dev)
+    N = 30000
+    pos = rng.uniform(-1, 1, (N, 3)).astype(np.float32)
+    g = rng.normal(size=(N, 32)).astype(np.float32)
+    pos_t = torch.from_numpy(pos).to(dev)
+    desc = enc.descriptor(N, 1.0)
+    ws = torch.empty(_lib.hashgrid_backward_workspace_bytes(desc), dtype=torch.uint8, device=dev)
+    d = torch.full(tuple(enc.table.shape), 5.0, dtype=torch.float32, device=dev)
+    _lib.call("hashgrid_encode_backward", [pos_t, torch.from_numpy(g).to(dev), d, ws], desc)
+    dt = d.cpu().numpy().astype(np.float64)
+    want = _c_backward(enc, pos, g)
+    absw = _c_backward(enc, pos, np.abs(g))
+    assert (dt[absw == 0] == 0).all()
+    assert (np.abs(dt - want) <= 1e-3 * absw + 1e-6).all()
+    assert np.abs(dt - want).max() > 0  # the f16 rounding is visible: this is not the f32 path
+    g[5, 7] = 1e6  # past 65504 once weighted by anything above 0.066
+    _lib.call("hashgrid_encode_backward", [pos_t, torch.from_numpy(g).to(dev), d, ws], desc)
+    assert torch.isinf(d).any() and torch.isfinite(d).float().mean() > 0.999
