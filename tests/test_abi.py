@@ -135,15 +135,19 @@ def test_bad_statics_are_errors():
 
 
 def test_hashgrid_backward_workspace_size():
-    """Host-only query: the workspace holds 16-byte entries for up to 8 rows per
-    point and level (F=2), and an invalid descriptor is an error, not 0 bytes."""
+    """Host-only query: the workspace holds entries for up to 8 rows per point and
+    level (F=2: 12 bytes with an f32 table, 8 bytes with an f16 table), and an
+    invalid descriptor is an error, not 0 bytes."""
     from jaxngp_amd.encoders import level_table
     lv = level_table(16, 2 ** 19, 16, 2048)
-    desc = lambda n: _lib.hashgrid_descriptor(n, 16, 2, 2 ** 19, lv.first_hash_level, 0, lv.n_rows, 1.0,
-                                              list(lv.scales), lv.resolutions, lv.offsets)
-    small, big = _lib.hashgrid_backward_workspace_bytes(desc(1000)), _lib.hashgrid_backward_workspace_bytes(desc(2 ** 18))
-    assert big - small >= (2 ** 18 - 1000) * 16 * 8 * 16
-    assert big < 2 ** 30
+    desc = lambda n, dt=0: _lib.hashgrid_descriptor(n, 16, 2, 2 ** 19, lv.first_hash_level, dt, lv.n_rows, 1.0,
+                                                    list(lv.scales), lv.resolutions, lv.offsets)
+    for dt, entry in ((0, 12), (1, 8)):
+        small = _lib.hashgrid_backward_workspace_bytes(desc(1000, dt))
+        big = _lib.hashgrid_backward_workspace_bytes(desc(2 ** 18, dt))
+        assert big - small >= (2 ** 18 - 1000) * 16 * 8 * entry
+        assert big - small < (2 ** 18 - 1000) * 16 * 8 * (entry + 4)
+        assert big < 2 ** 30
     with pytest.raises(_lib.NGPError, match="F="):
         _lib.hashgrid_backward_workspace_bytes(_lib.hashgrid_descriptor(4, 1, 3, 8, 1, 0, 8, 1.0, [1.0], [2], [0, 8]))
     assert _lib.last_error() == (0, "")
