@@ -219,7 +219,12 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     The fallback marches `fallback_steps_cap` samples per ray and round: a ray's
     samples and colour do not depend on that cap (march_rays_inference resumes from
     the t it stopped at and defers a far-plane sample to the next round), so its
-    pixels equal the plain render's, in fewer rounds.  `march_steps_cap` is accepted
+    pixels equal the plain render's, in fewer rounds.  The one exception is the
+    reference's own: when a ray's last regular sample ends exactly on t_end (an f32
+    tie) and the far-plane sample fills a round's last slot, the ray survives the
+    round (integrating.cu:300) and the next round writes the far-plane sample again
+    (marching.cu:317, 367-394), so such a ray depends on where the rounds fall.  About
+    one ray in 10^5.  `march_steps_cap` is accepted
     for the same keyword set as render_image_inference.  Same return as
     render_image_inference; `stats` (if given) receives the fallback count."""
     dev = pose.device

@@ -80,3 +80,28 @@ def integrate_inputs(rng, n_rays, max_samples=64, contiguous=True, total_pad=7):
     drgbs = np.concatenate([sig[:, None], rgb], 1).astype(np.float32)
     bgs = rng.uniform(0, 1, (n_rays, 3)).astype(np.float32)
     return start, ns, bgs, dss, z, drgbs
+
+
+def march_all_rounds(o, d, t0, t1, bits, cap, K=1, G=128, diag=1024, bound=1.0, portion=0.0):
+    """Every sample the ray pool marches for one ray (the oracle's march), round after round, until the
+    integrator's rule for a transparent ray ends it (integrating.cu:300: terminated
+    once a round returns fewer than march_steps_cap samples).  Returns the z values
+    and the samples per round."""
+    import oracle
+
+    ts = np.array([t0], np.float32)
+    te = np.array([t1], np.float32)
+    term, idx, nri = np.ones(1, bool), np.zeros(1, np.uint32), np.uint32(0)
+    zs, per_round = [], []
+    for _ in range(100000):
+        out = oracle.march_rays_inference(diag, K, G, cap, bound, portion, o[None], d[None], ts, te, bits, nri, term,
+                                          idx)
+        nri, idx = out["next_ray_index"], out["indices"]
+        if idx[0] >= 1:
+            return np.array(zs, np.float32), per_round
+        k = int(out["n_samples"][0])
+        zs += list(out["z_vals"][0, :k])
+        per_round.append(k)
+        ts[0] = out["t_starts_out"][0]
+        term = np.array([k < cap])
+    raise AssertionError("ray did not finish")

@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 import oracle
-from tests.helpers import SQRT3, blob_bitfield, camera_rays, integrate_inputs, march_inputs, near_far_np
+from tests.helpers import (SQRT3, blob_bitfield, camera_rays, integrate_inputs, march_all_rounds, march_inputs,
+                           near_far_np)
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
@@ -470,11 +471,13 @@ def test_inference_render_independent_of_steps_cap(rng):
     """A ray's colour from the ray pool does not depend on march_steps_cap: the march
     resumes from the t it stopped at, a far-plane sample that does not fit is written
     in the next round, and the compositing carries (rgbd, T) between rounds in f32 in
-    the same order.  The proposal renderer's occupancy-grid fallback relies on this
-    to march more samples per round than the plain render and give the same pixels.
-    Two scenes: blobs of occupied cells with a dense field (early termination), and
-    every cell occupied with a faint field, where rays leave the box in an occupied
-    cell (far-plane samples, deferred when the chunk is full)."""
+    the same order.  The one exception is the exact tie pinned by
+    test_inference_far_plane_tie_is_marched_twice.  The proposal renderer's
+    occupancy-grid fallback relies on this to march more samples per round than the
+    plain render and give the same pixels.  Two scenes: blobs of occupied cells with a
+    dense field (early termination), and every cell occupied with a faint field, where
+    rays leave the box in an occupied cell (far-plane samples, deferred when the chunk
+    is full)."""
     n = 96
     o, d, t0, t1, _, bits, _ = march_inputs(rng, n)
     full_bits = np.full_like(bits, 0xFF)
@@ -490,6 +493,40 @@ def test_inference_render_independent_of_steps_cap(rng):
             deferred += deferred_c
         if b is full_bits:
             assert far > 0 and deferred > 0
+
+
+def test_inference_far_plane_tie_is_marched_twice():
+    """The reference's ray pool writes the far-plane sample twice when the last
+    regular sample ends exactly on t_end (z + ds == t_end in f32) and the far-plane
+    sample fills the round's last slot: the round returns march_steps_cap samples, so
+    the integrator keeps the ray (integrating.cu:300), the next round starts at
+    t_start == t_end, which the marcher does not reject (marching.cu:317 rejects only
+    t_end < t_start), and its far-plane block (marching.cu:367-394) writes the sample
+    again.  With any other cap, or with t_end one ulp later, the sample is written
+    once.  The kernels keep this behaviour (it is the reference's output); it is the
+    only way a pixel of the pool render depends on march_steps_cap."""
+    o = np.array([-0.9, -0.05, 0.1], np.float32)
+    d = np.array([1.0, 0.0, 0.0], np.float32)
+    bits = np.full(128 ** 3 // 8, 0xFF, np.uint8)
+    t0 = np.float32(0.0)
+    # the step chain t_{k+1} = t_k + ds of the oracle's own march (all cells occupied)
+    chain, _ = march_all_rounds(o, d, t0, np.float32(1.5), bits, 64)
+    assert len(chain) > 20 and np.all(np.diff(chain) > 0)
+    t_end = chain[7]  # the 7th sample ends exactly here
+    z8, rounds8 = march_all_rounds(o, d, t0, t_end, bits, 8)
+    z16, rounds16 = march_all_rounds(o, d, t0, t_end, bits, 16)
+    # 7 regular samples + the far-plane sample, once at cap 16, twice at cap 8
+    assert rounds16 == [8] and rounds8 == [8, 1]
+    np.testing.assert_array_equal(z16, np.append(chain[:7], t_end))
+    np.testing.assert_array_equal(z8, np.append(z16, t_end))
+    # one ulp further: the 8th regular sample fits, the far-plane sample is deferred
+    # to the next round at cap 8, and both caps march the same samples
+    t_late = np.nextafter(t_end, np.float32(np.inf))
+    z8, rounds8 = march_all_rounds(o, d, t0, t_late, bits, 8)
+    z16, rounds16 = march_all_rounds(o, d, t0, t_late, bits, 16)
+    assert rounds8 == [8, 1] and rounds16 == [9]
+    np.testing.assert_array_equal(z8, z16)
+    np.testing.assert_array_equal(z16, np.append(chain[:8], t_late))
 
 
 def _fd_loss(startidx, ns, bgs, dss, z, drgbs, g):
