@@ -143,6 +143,48 @@ void tcnn_hashgrid_encode(ngp_stream_t stream, void **buffers, const char *opaqu
  * out: dL_dparams f32[offsets[L], F] (fully written), dL_dcoords f32[3, N] */
 void tcnn_hashgrid_encode_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
+/* Sky-sphere background model of unbounded scenes (models/nerfs.py:133-213, built by
+ * make_skysphere_background_model_ngp, nerfs.py:383-419): the ray's far intersection
+ * with the sphere |x| = radius (radius = 4 bound), SH degree 2 of the hit point
+ * divided by the Frobenius norm of ALL the batch's hit points (nerfs.py:191-196:
+ * jnp.linalg.norm over the whole [n, 3] array, + 1e-15), SH degree 4 of the ray
+ * direction and the appearance embedding feed a 32->32->3 MLP without biases (ReLU
+ * hidden layers, sigmoid output); rays whose intersection t is not finite get 0.
+ * A ray that misses the sphere makes that norm NaN and with it every other ray's
+ * colour, as in the reference.  The parameters are one flat f32 array of the flax
+ * Dense kernels ([in, out], row-major): rgb_mlp/Dense_0 [20+E, 32],
+ * Dense_1 [32, 32], Dense_2 [32, 3]. */
+#define NGP_SKY_MAX_EXTRA_DIMS 16u
+typedef struct SkySphereBgDescriptor {
+    uint32_t n_rays;
+    uint32_t n_extra_dims;       /* E, appearance embedding width, 0..NGP_SKY_MAX_EXTRA_DIMS */
+    uint32_t appearance_per_ray; /* 1: appearance f32[n_rays, E]; 0: one f32[E] for every ray */
+    float radius;
+} SkySphereBgDescriptor;
+
+/* Number of f32 parameters: (20 + E) * 32 + 32 * 32 + 32 * 3. */
+uint32_t sky_sphere_bg_n_params(uint32_t n_extra_dims);
+
+/* Bytes of the workspace both sky-sphere targets take (256-byte aligned device
+ * memory); 0 with an error recorded for an invalid descriptor. */
+size_t sky_sphere_bg_workspace_size(const char *opaque, size_t opaque_len);
+
+/* Replaces SkySphereBg.__call__ (nerfs.py:160-213).  opaque: SkySphereBgDescriptor.
+ * in : rays_o f32[n, 3], rays_d f32[n, 3], appearance (see appearance_per_ray; may
+ *      be NULL when E = 0), params f32[sky_sphere_bg_n_params(E)]
+ * out: colors f32[n, 3], workspace u8[sky_sphere_bg_workspace_size(opaque)] */
+void sky_sphere_bg(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* The VJP of sky_sphere_bg with respect to the parameters and the per-ray
+ * appearance embeddings (the gradient jax.grad takes through bg_fn in
+ * app/nerf/_utils.py:128-133); rays and the batch norm get none.  Weight
+ * gradients are summed in a fixed order (deterministic).  Needs appearance_per_ray
+ * = 1 when E > 0.
+ * in : rays_o, rays_d, appearance, params (as sky_sphere_bg), dL_dcolors f32[n, 3]
+ * out: dL_dparams f32[n_params], dL_dappearance f32[n, E] (may be NULL when E = 0),
+ *      workspace u8[sky_sphere_bg_workspace_size(opaque)] */
+void sky_sphere_bg_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
 /* Fused NeRF MLP of models/nerfs.py:39-86 with make_nerf_ngp's shapes
  * (nerfs.py:422-454): density MLP 32->64->16, SH degree 4 of the direction,
  * colour MLP (16+16)->64->64->3, no biases, ReLU hidden layers, trunc_exp density

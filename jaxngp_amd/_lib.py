@@ -34,6 +34,8 @@ TARGETS = (
     "hashgrid_encode_backward",
     "tcnn_hashgrid_encode",
     "tcnn_hashgrid_encode_backward",
+    "sky_sphere_bg",
+    "sky_sphere_bg_backward",
     "nerf_mlp_forward",
     "nerf_mlp_backward",
     "train_make_rays",
@@ -81,6 +83,10 @@ def load() -> ctypes.CDLL:
     lib.ngp_version.restype = ctypes.c_char_p
     lib.hashgrid_encode_backward_workspace_size.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     lib.hashgrid_encode_backward_workspace_size.restype = ctypes.c_size_t
+    lib.sky_sphere_bg_workspace_size.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.sky_sphere_bg_workspace_size.restype = ctypes.c_size_t
+    lib.sky_sphere_bg_n_params.argtypes = [ctypes.c_uint32]
+    lib.sky_sphere_bg_n_params.restype = ctypes.c_uint32
     _lib = lib
     return lib
 
@@ -175,6 +181,15 @@ def hashgrid_backward_workspace_bytes(opaque: bytes) -> int:
     return n
 
 
+def sky_sphere_bg_workspace_bytes(opaque: bytes) -> int:
+    """Workspace bytes of sky_sphere_bg / sky_sphere_bg_backward for `opaque`."""
+    n = load().sky_sphere_bg_workspace_size(opaque, len(opaque))
+    if n == 0:
+        _check()
+        raise NGPError("sky_sphere_bg_workspace_size: invalid descriptor")
+    return n
+
+
 def call_raw(name: str, ptrs: Sequence[int], opaque: bytes, stream: int = 0) -> None:
     """Calls target `name` with raw pointers (host-side error tests use NULLs)."""
     arr = (ctypes.c_void_p * max(1, len(ptrs)))(*ptrs)
@@ -235,6 +250,19 @@ def hashgrid_descriptor(n_points, L, F, T, first_hash_level, table_dtype, n_rows
 def tcnn_hashgrid_descriptor(n_coords, L, F, N_min, per_level_scale) -> bytes:
     """jaxtcnn::HashGridDescriptor (deps/jax-tcnn/lib/impl/tcnnutils.h:11-29)."""
     return struct.pack("<4If", n_coords, L, F, N_min, per_level_scale)
+
+
+SKY_MAX_EXTRA_DIMS = 16
+
+
+def sky_sphere_bg_n_params(n_extra_dims: int) -> int:
+    """(20 + E) * 32 + 32 * 32 + 32 * 3 flat f32 parameters of the sky-sphere MLP."""
+    return (20 + int(n_extra_dims)) * 32 + 32 * 32 + 32 * 3
+
+
+def sky_sphere_bg_descriptor(n_rays, n_extra_dims, appearance_per_ray, radius) -> bytes:
+    """SkySphereBgDescriptor of include/ngp_hip.h."""
+    return struct.pack("<3If", n_rays, n_extra_dims, int(appearance_per_ray), radius)
 
 
 def nerf_mlp_descriptor(n_samples, n_workspace_blocks=0, enc_layout=ENC_POINT_MAJOR, param_dtype=DTYPE_F32) -> bytes:

@@ -28,7 +28,9 @@ State dict, as written and as read:
                          "density_mlp": {"Dense_0": {"kernel": f32[32, 64]}, "Dense_1": {"kernel": f32[64, 16]}},
                          "rgb_mlp": {"Dense_0": {"kernel": f32[32, 64]}, "Dense_1": {"kernel": f32[64, 64]},
                                      "Dense_2": {"kernel": f32[64, 3]}}},
-                "bg": None, "appearance_embeddings": f32[n_frames, 0]}
+                "bg": None, or {"rgb_mlp": {"Dense_0": {"kernel": f32[20+E, 32]}, "Dense_1": {"kernel":
+                      f32[32, 32]}, "Dense_2": {"kernel": f32[32, 3]}}} with a sky-sphere background,
+                "appearance_embeddings": f32[n_frames, 0]}
     opt_state: None when written (test.py restores into an empty state, whose
                opt_state is None, so rendering needs no optimizer state); ignored
                when read, so a restored Trainer starts its Adam moments at zero
@@ -136,13 +138,15 @@ def _mlp_flat(tree: dict) -> np.ndarray:
 
 
 def nerf_state_dict(step: int, table: np.ndarray, mlp_flat: np.ndarray, density: np.ndarray, occ_mask: np.ndarray,
-                    occupancy: np.ndarray, alive_indices: np.ndarray, n_frames: int = 0) -> dict:
-    """The state dict of a NeRFState (utils/types.py:1127-1145) with the given arrays."""
+                    occupancy: np.ndarray, alive_indices: np.ndarray, n_frames: int = 0,
+                    bg_tree: dict | None = None) -> dict:
+    """The state dict of a NeRFState (utils/types.py:1127-1145) with the given arrays;
+    `bg_tree` is a SkySphereBg's parameter tree (params["bg"], nerfs.py:133-213)."""
     nerf = {"position_encoder": {TABLE_NAME: np.asarray(table, np.float32)}}
     nerf.update(_mlp_tree(np.asarray(mlp_flat, np.float32)))
     return {
         "step": int(step),
-        "params": {"nerf": nerf, "bg": None,
+        "params": {"nerf": nerf, "bg": bg_tree,
                    "appearance_embeddings": np.zeros((int(n_frames), 0), np.float32)},
         "opt_state": None,
         "ogrid": {"density": np.asarray(density, np.float32), "occ_mask": np.asarray(occ_mask, np.bool_),
@@ -152,13 +156,14 @@ def nerf_state_dict(step: int, table: np.ndarray, mlp_flat: np.ndarray, density:
 
 
 def parse_nerf_state(state: dict) -> dict:
-    """{"step", "table", "mlp", "density", "occ_mask", "occupancy", "alive_indices"}
-    from a NeRFState state dict (either writer's)."""
+    """{"step", "table", "mlp", "density", "occ_mask", "occupancy", "alive_indices", "bg"}
+    from a NeRFState state dict (either writer's); "bg" is the background model's
+    parameter tree or None."""
     step = state.get("step")
     step = int(np.asarray(step)) if step is not None else 0
     nerf = state["params"]["nerf"]
     out = {"step": step, "table": np.asarray(nerf["position_encoder"][TABLE_NAME], np.float32),
-           "mlp": _mlp_flat(nerf)}
+           "mlp": _mlp_flat(nerf), "bg": state["params"].get("bg")}
     og = state.get("ogrid") or {}
     for k in ("density", "occ_mask", "occupancy", "alive_indices"):
         out[k] = None if og.get(k) is None else np.asarray(og[k])
@@ -188,10 +193,10 @@ def latest_checkpoint(path: str) -> str:
     return os.path.join(path, best[1])
 
 
-def save_checkpoint(ckpt_dir: str, trainer, n_frames: int | None = None) -> str:
+def save_checkpoint(ckpt_dir: str, trainer, n_frames: int | None = None, bg_model=None) -> str:
     """Writes the trainer's field and occupancy grid as <dir>/checkpoint_<step> in the
     reference's format and returns the path.  The table and MLP are f32 (the master
-    weights on the f16 path)."""
+    weights on the f16 path); `bg_model` (a SkySphereBg) goes to params["bg"]."""
     import torch
 
     os.makedirs(ckpt_dir, exist_ok=True)
@@ -205,7 +210,8 @@ def save_checkpoint(ckpt_dir: str, trainer, n_frames: int | None = None) -> str:
     with torch.no_grad():
         state = nerf_state_dict(trainer.step_count, table.cpu().numpy(), trainer.model.mlp_params.detach().cpu().numpy(),
                                 g.density.cpu().numpy(), g.occ_mask.cpu().numpy(), g.bitfield.cpu().numpy(),
-                                g.alive.cpu().numpy(), n_frames)
+                                g.alive.cpu().numpy(), n_frames,
+                                bg_model.param_tree() if bg_model is not None else None)
     path = checkpoint_path(ckpt_dir, trainer.step_count)
     tmp = path + ".tmp"
     with open(tmp, "wb") as f:
@@ -257,3 +263,11 @@ def restore_trainer(trainer, path: str) -> int:
     trainer.step_count = ck["step"]
     trainer._calls = None  # rebind the prepared calls to the restored buffers
     return ck["step"]
+
+
+def restore_background(bg_model, path: str) -> None:
+    """Loads params["bg"] of a checkpoint into a SkySphereBg of the same width."""
+    ck = load_checkpoint(path)
+    if ck["bg"] is None:
+        raise ValueError(f"{path!r} holds no background model parameters")
+    bg_model.load_param_tree(ck["bg"])

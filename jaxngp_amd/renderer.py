@@ -98,7 +98,7 @@ def march_and_integrate_inference(next_ray_index, terminated, indices, rays_rgbd
 @torch.no_grad()
 def render_image_inference(model, camera: Camera, pose: torch.Tensor, occupancy_bitfield, bg=(1.0, 1.0, 1.0), *,
                            K=1, G=128, diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8,
-                           pool: int = 1 << 18):
+                           pool: int = 1 << 18, bg_model=None, appearance=None):
     """Renders one frame (cuda.py:244-373).  Returns (bg [H,W,3] f32, rgb u8 [H,W,3],
     depth u8 [H,W]).
 
@@ -109,9 +109,14 @@ def render_image_inference(model, camera: Camera, pose: torch.Tensor, occupancy_
     march_steps_cap steps whichever slot holds it); the reference uses 8192
     slots, the default here is 2^18, which fills the GPU in fewer rounds.  The
     buffers are allocated once per frame and every launch is pre-bound; the field
-    runs fused (nerf_field_chunks) on the marched samples only.
+    runs fused (nerf_field_chunks) on the marched samples only.  With `bg_model`
+    (a SkySphereBg) the per-ray background is the model's colour for the frame's
+    rays and `appearance` (cuda.py:283-288), instead of the constant `bg`.
     """
     o, d = make_rays_worldspace(camera, pose)
+    if bg_model is not None:
+        with torch.no_grad():
+            bg = bg_model(o, d, appearance)
     rays_rgbd, rays_bg = render_rays_inference(model, o, d, occupancy_bitfield, bg, K=K, G=G,
                                                diagonal_n_steps=diagonal_n_steps, stepsize_portion=stepsize_portion,
                                                march_steps_cap=march_steps_cap, pool=pool)

@@ -805,3 +805,85 @@ def tcnn_grid_backward64(coords, dL_dy, dy_dcoords, offsets, L, F, N_min, per_le
             np.add.at(dt, idx, w.astype(np.float64)[:, None] * gl)
     dcoords = np.einsum("kn,knd->dn", np.asarray(dL_dy, np.float64), np.asarray(dy_dcoords, np.float64))
     return dt, dcoords
+
+
+# ---------------------------------------------------------------------------
+# sky-sphere background model (models/nerfs.py:133-213, make_skysphere_background_
+# model_ngp 383-419), float64.  Pinned by tests/test_oracle_sky.py.
+# ---------------------------------------------------------------------------
+
+SKY_IN0, SKY_HID, SKY_OUT = 20, 32, 3
+
+
+def sky_sphere_split(params, n_extra_dims):
+    """Flat parameters -> rgb_mlp Dense kernels W0 [20+E, 32], W1 [32, 32], W2 [32, 3]."""
+    p = np.asarray(params, np.float64)
+    K = SKY_IN0 + int(n_extra_dims)
+    W0 = p[:K * 32].reshape(K, 32)
+    W1 = p[K * 32:K * 32 + 1024].reshape(32, 32)
+    W2 = p[K * 32 + 1024:K * 32 + 1120].reshape(32, 3)
+    assert p.size == K * 32 + 1120
+    return W0, W1, W2
+
+
+def sky_sphere_hit64(rays_o, rays_d, radius):
+    """The larger root t of |o + t d|^2 = r^2 (nerfs.py:172-184); NaN where the ray
+    misses (np.maximum propagates NaN like jnp.maximum)."""
+    o = np.asarray(rays_o, np.float64)
+    d = np.asarray(rays_d, np.float64)
+    a = (d * d).sum(-1)
+    b = 2.0 * (o * d).sum(-1)
+    c = (o * o).sum(-1) - float(radius) ** 2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        s = np.sqrt(b * b - 4.0 * a * c)
+        return np.maximum((-b + s) / (2.0 * a), (-b - s) / (2.0 * a))
+
+
+def sky_sphere_inputs64(rays_o, rays_d, appearance, radius, n_extra_dims):
+    """MLP inputs [SH2(pos / ||pos||_F or 0), SH4(d), appearance] and finite(t)
+    (nerfs.py:186-207).  ||pos||_F is the norm of the whole [n, 3] array."""
+    o = np.asarray(rays_o, np.float64)
+    d = np.asarray(rays_d, np.float64)
+    n = o.shape[0]
+    t = sky_sphere_hit64(o, d, radius)
+    fin = np.isfinite(t)
+    with np.errstate(invalid="ignore", over="ignore"):
+        pos = o + t[:, None] * d
+        denom = np.sqrt((pos * pos).sum()) + 1e-15
+        pd = np.where(fin[:, None], pos / denom, 0.0)
+    E = int(n_extra_dims)
+    app = np.broadcast_to(np.asarray(appearance, np.float64).reshape(-1, E) if E else np.zeros((1, 0)), (n, E))
+    return np.concatenate([sh4_64(pd)[:, :4], sh4_64(d), app], axis=1), fin
+
+
+def sky_sphere_bg64(rays_o, rays_d, appearance, params, radius, n_extra_dims):
+    """colors [n, 3]: sigmoid(relu(relu(x W0) W1) W2) where t is finite, else 0."""
+    x, fin = sky_sphere_inputs64(rays_o, rays_d, appearance, radius, n_extra_dims)
+    W0, W1, W2 = sky_sphere_split(params, n_extra_dims)
+    with np.errstate(invalid="ignore", over="ignore"):
+        h1 = np.maximum(x @ W0, 0.0)
+        h2 = np.maximum(h1 @ W1, 0.0)
+        y = 1.0 / (1.0 + np.exp(-(h2 @ W2)))
+    return np.where(fin[:, None], y, 0.0)
+
+
+def sky_sphere_bg_backward64(rays_o, rays_d, appearance, params, radius, n_extra_dims, dL_dcolors):
+    """(dL/dparams flat, dL/dappearance [n, E]): the VJP jax.grad takes through
+    SkySphereBg.__call__.  The cotangent is masked by finite(t) and relu's VJP is
+    jax.nn.relu's select(z > 0, g, 0), a select rather than a product, so a NaN
+    pre-activation passes 0, not NaN."""
+    x, fin = sky_sphere_inputs64(rays_o, rays_d, appearance, radius, n_extra_dims)
+    W0, W1, W2 = sky_sphere_split(params, n_extra_dims)
+    z1 = x @ W0
+    h1 = np.maximum(z1, 0.0)
+    z2 = h1 @ W1
+    h2 = np.maximum(z2, 0.0)
+    y = 1.0 / (1.0 + np.exp(-(h2 @ W2)))
+    g = np.where(fin[:, None], np.asarray(dL_dcolors, np.float64), 0.0)
+    do = g * y * (1.0 - y)
+    with np.errstate(invalid="ignore"):
+        dz2 = np.where(z2 > 0, do @ W2.T, 0.0)
+        dz1 = np.where(z1 > 0, dz2 @ W1.T, 0.0)
+    dx = dz1 @ W0.T
+    dparams = np.concatenate([(x.T @ dz1).ravel(), (h1.T @ dz2).ravel(), (h2.T @ do).ravel()])
+    return dparams, dx[:, SKY_IN0:]

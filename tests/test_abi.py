@@ -17,7 +17,8 @@ HEADER = REPO / "include" / "ngp_hip.h"
 
 def _declared_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:void|int|size_t|const char \*)\s*\*?\s*([a-z_0-9]+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:void|int|size_t|uint32_t|const char \*)\s*\*?\s*([a-z_0-9]+)\s*\(", text,
+                                 re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -50,6 +51,7 @@ def test_descriptor_sizes_match_c_structs():
     assert len(_lib.nerf_mlp_descriptor(5, 0)) == 16
     # jaxtcnn::HashGridDescriptor (deps/jax-tcnn/lib/impl/tcnnutils.h:11-29): 4 u32 + f32
     assert len(_lib.tcnn_hashgrid_descriptor(7, 16, 2, 16, 1.38)) == 20
+    assert len(_lib.sky_sphere_bg_descriptor(7, 0, 1, 4.0)) == 16
     assert len(_lib.train_rays_descriptor(1, 2, 3, 4, 1.0, 1.0, 1.0, 1.0, 1.0, 5, 6, 3, (1.0, 1.0, 1.0))) == 64
     assert len(_lib.loss_descriptor(4, 0.1)) == 12
     assert len(_lib.adam_descriptor(10, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0)) == 56
@@ -96,6 +98,7 @@ _Static_assert(sizeof(ProposalSampleDescriptor) == 40, "");
 _Static_assert(sizeof(ProposalRenderDescriptor) == sizeof(HashGridDescriptor) + 44, "");
 _Static_assert(sizeof(NerfFieldChunksDescriptor) == sizeof(HashGridDescriptor) + 16, "");
 _Static_assert(sizeof(TcnnHashGridDescriptor) == 20, "");
+_Static_assert(sizeof(SkySphereBgDescriptor) == 16, "");
 int main(void) { return 0; }
 '''
     out = subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", f"-I{REPO / 'include'}", "-x", "c", "-", "-o",
@@ -150,4 +153,23 @@ def test_hashgrid_backward_workspace_size():
         assert big < 2 ** 30
     with pytest.raises(_lib.NGPError, match="F="):
         _lib.hashgrid_backward_workspace_bytes(_lib.hashgrid_descriptor(4, 1, 3, 8, 1, 0, 8, 1.0, [1.0], [2], [0, 8]))
+    assert _lib.last_error() == (0, "")
+
+
+def test_sky_sphere_host_checks():
+    """Parameter count, workspace size and descriptor errors need no GPU."""
+    lib = _lib.load()
+    for E in (0, 3, 16):
+        assert lib.sky_sphere_bg_n_params(E) == _lib.sky_sphere_bg_n_params(E) == (20 + E) * 32 + 1120
+    small = _lib.sky_sphere_bg_workspace_bytes(_lib.sky_sphere_bg_descriptor(10, 0, 1, 4.0))
+    big = _lib.sky_sphere_bg_workspace_bytes(_lib.sky_sphere_bg_descriptor(1 << 20, 16, 1, 4.0))
+    assert small % 256 == 0 and big % 256 == 0 and small < big <= 2304 + 512 * 2272 * 4 + 255
+    for bad in (_lib.sky_sphere_bg_descriptor(10, 17, 1, 4.0), _lib.sky_sphere_bg_descriptor(10, 0, 2, 4.0),
+                _lib.sky_sphere_bg_descriptor(10, 0, 1, 0.0), _lib.sky_sphere_bg_descriptor(10, 0, 1, float("nan"))):
+        with pytest.raises(_lib.NGPError, match="invalid descriptor"):
+            _lib.sky_sphere_bg_workspace_bytes(bad)
+    with pytest.raises(_lib.NGPError, match="per-ray appearance"):
+        _lib.call_raw("sky_sphere_bg_backward", [0] * 8, _lib.sky_sphere_bg_descriptor(10, 4, 0, 4.0))
+    with pytest.raises(_lib.NGPError, match="NULL"):
+        _lib.call_raw("sky_sphere_bg", [0] * 6, _lib.sky_sphere_bg_descriptor(10, 0, 1, 4.0))
     assert _lib.last_error() == (0, "")

@@ -5,6 +5,7 @@ map, and the mapping between the flax kernels and the flat MLP vector."""
 import numpy as np
 import msgpack
 import pytest
+import torch
 
 from jaxngp_amd import checkpoint as ck
 from jaxngp_amd.nerfs import MLP_LAYERS, N_PARAMS
@@ -95,3 +96,28 @@ def test_latest_checkpoint(tmp_path):
     (tmp_path / "checkpoint_7.tmp").write_bytes(b"x")
     assert ck.latest_checkpoint(str(tmp_path)).endswith("checkpoint_300")
     assert ck.checkpoint_path(str(tmp_path), 12).endswith("checkpoint_12")
+
+
+def test_background_model_parameters_round_trip(tmp_path):
+    """params["bg"] holds a SkySphereBg's tree (rgb_mlp/Dense_0..2 kernels, [in, out])
+    and restore_background loads it back."""
+    from jaxngp_amd.background import SkySphereBg
+    from jaxngp_amd.checkpoint import restore_background
+    bg = SkySphereBg(4.0, n_extra_dims=2, seed=1)
+    tree = bg.param_tree()
+    assert [tree["rgb_mlp"][k]["kernel"].shape for k in ("Dense_0", "Dense_1", "Dense_2")] == [(22, 32), (32, 32),
+                                                                                                (32, 3)]
+    z = np.zeros(8, np.float32)
+    state = ck.nerf_state_dict(3, np.zeros((8, 2), np.float32), np.zeros(9408, np.float32), z, z > 0,
+                               np.zeros(1, np.uint8), np.zeros(0, np.uint32), bg_tree=tree)
+    path = tmp_path / "checkpoint_3"
+    path.write_bytes(ck.to_bytes(state))
+    got = ck.load_checkpoint(str(path))
+    np.testing.assert_array_equal(got["bg"]["rgb_mlp"]["Dense_1"]["kernel"], tree["rgb_mlp"]["Dense_1"]["kernel"])
+    other = SkySphereBg(4.0, n_extra_dims=2, seed=2)
+    assert not torch.equal(other.params, bg.params)
+    restore_background(other, str(path))
+    assert torch.equal(other.params, bg.params)
+    # without a background model the field is nil, as the reference writes it
+    state["params"]["bg"] = None
+    assert ck.parse_nerf_state(ck.from_bytes(ck.to_bytes(state)))["bg"] is None
