@@ -457,6 +457,24 @@ typedef struct NgpTrainRaysDescriptor {
  *      noises f32[n], bgs f32[n,3], gt_rgb f32[n,3], pixel_ids u32[n] */
 void train_make_rays(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
+typedef struct NgpTrainBgDescriptor {
+    uint32_t n_rays;
+} NgpTrainBgDescriptor;
+
+/* With a background model (app/nerf/_utils.py:128-133, 150): the ground truth of
+ * the pixels train_make_rays sampled, blended over per-ray backgrounds,
+ * gt = rgb * alpha + bg * (1 - alpha) (utils/data.py:443-464).
+ * in : rgba u8[V,H,W,4], pixel_ids u32[n] (train_make_rays' output), bgs f32[n,3]
+ * out: gt_rgb f32[n,3] */
+void train_gt_blend(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* The gradient of the loss with respect to those backgrounds: through the
+ * composited colour (integrate_rays_backward's dL_dbgs) and through the blended
+ * ground truth (-dL_dfinal_rgbds[:3] * (1 - alpha)).
+ * in : rgba u8[V,H,W,4], pixel_ids u32[n], dL_dfinal_rgbds f32[n,4], dL_dbgs f32[n,3]
+ * out: dL_dbg f32[n,3] */
+void train_bg_grad(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
 /* Dynamic loss scaling state, kept in device memory so that no step waits for
  * the host (no counterpart in the reference, which trains in f32 only). */
 typedef struct NgpLossScaleState {

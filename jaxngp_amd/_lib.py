@@ -39,6 +39,8 @@ TARGETS = (
     "nerf_mlp_forward",
     "nerf_mlp_backward",
     "train_make_rays",
+    "train_gt_blend",
+    "train_bg_grad",
     "train_loss_grad",
     "adam_update",
     "loss_scale_check",
@@ -250,6 +252,10 @@ def hashgrid_descriptor(n_points, L, F, T, first_hash_level, table_dtype, n_rows
 def tcnn_hashgrid_descriptor(n_coords, L, F, N_min, per_level_scale) -> bytes:
     """jaxtcnn::HashGridDescriptor (deps/jax-tcnn/lib/impl/tcnnutils.h:11-29)."""
     return struct.pack("<4If", n_coords, L, F, N_min, per_level_scale)
+
+
+def train_bg_descriptor(n_rays) -> bytes:
+    return struct.pack("<I", n_rays)
 
 
 SKY_MAX_EXTRA_DIMS = 16

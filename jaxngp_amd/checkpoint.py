@@ -200,6 +200,8 @@ def save_checkpoint(ckpt_dir: str, trainer, n_frames: int | None = None, bg_mode
     import torch
 
     os.makedirs(ckpt_dir, exist_ok=True)
+    if bg_model is None:
+        bg_model = getattr(trainer, "bg_model", None)
     # the f16 path keeps f32 master weights beside its f16 table; the f32 path updates
     # the model's table in place
     table = trainer.table_master if getattr(trainer, "fp16", False) \
@@ -228,7 +230,8 @@ def load_checkpoint(path: str) -> dict:
 
 def restore_trainer(trainer, path: str) -> int:
     """Loads a checkpoint into a Trainer built for the same scene options: field
-    weights (and their f16 copies on the precision path), occupancy grid and step.
+    weights (and their f16 copies on the precision path), the background model's
+    parameters when the trainer has one, occupancy grid and step.
     The Adam moments restart at zero (the file carries no optimizer state).
     Returns the step."""
     import torch
@@ -260,6 +263,12 @@ def restore_trainer(trainer, path: str) -> int:
             g.alive_offsets = np.cumsum([0] + counts).tolist()
         for v in trainer.opt.values():
             v.zero_()
+        if getattr(trainer, "bg_model", None) is not None:
+            if ck["bg"] is None:
+                raise ValueError("the trainer has a background model but the checkpoint holds none")
+            trainer.bg_model.load_param_tree(ck["bg"])
+            trainer.bb["mm"].zero_()
+            trainer.bb["mv"].zero_()
     trainer.step_count = ck["step"]
     trainer._calls = None  # rebind the prepared calls to the restored buffers
     return ck["step"]

@@ -104,6 +104,44 @@ __global__ void __launch_bounds__(kBlock)
     pixel_ids[i] = (uint32_t)(pick & 0xffffffffu);
 }
 
+// Per-ray backgrounds from a background model (app/nerf/_utils.py:128-150): the
+// ground truth of each sampled pixel blended over its ray's background, and the
+// background's gradient.  pixel_ids are train_make_rays' view * H * W + pixel.
+__device__ __forceinline__ float pixel_alpha(const uint8_t *__restrict__ rgba, uint32_t pid, float rgb[3]) {
+    const uint8_t *p = rgba + 4 * (size_t)pid;
+    const float alpha = (float)p[3] / 255.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[c] = (float)p[c] / 255.0f;
+    return alpha;
+}
+
+__global__ void __launch_bounds__(kBlock)
+    gt_blend_kernel(uint32_t n, const uint8_t *__restrict__ rgba, const uint32_t *__restrict__ pixel_ids,
+                    const float *__restrict__ bgs, float *__restrict__ gt) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float rgb[3];
+    const float alpha = pixel_alpha(rgba, pixel_ids[i], rgb);
+    // the expression of make_rays_kernel (data.py:443-464)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gt[3 * (size_t)i + c] = rgb[c] * alpha + bgs[3 * (size_t)i + c] * (1.0f - alpha);
+}
+
+// dL/dbg = dL/dbg through the composited colour (integrate_rays_backward) plus
+// dL/dgt * dgt/dbg = -dL/dpred * (1 - alpha): the Huber loss is a function of
+// pred - gt, so its gradient in gt is minus its gradient in pred.
+__global__ void __launch_bounds__(kBlock)
+    bg_grad_kernel(uint32_t n, const uint8_t *__restrict__ rgba, const uint32_t *__restrict__ pixel_ids,
+                   const float *__restrict__ dL_drgbd, const float *__restrict__ dL_dbgs, float *__restrict__ dL_dbg) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float rgb[3];
+    const float one_minus_alpha = 1.0f - pixel_alpha(rgba, pixel_ids[i], rgb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        dL_dbg[3 * (size_t)i + c] = dL_dbgs[3 * (size_t)i + c] - dL_drgbd[4 * (size_t)i + c] * one_minus_alpha;
+}
+
 __global__ void zero_u32_kernel(uint32_t *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
@@ -253,6 +291,30 @@ extern "C" void train_loss_grad(ngp_stream_t stream_, void **buffers, const char
                            n_valid, scaled ? (const NgpLossScaleState *)buffers[6] : nullptr, (float *)buffers[4],
                            loss);
     }
+    check_launch(target);
+}
+
+extern "C" void train_gt_blend(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "train_gt_blend";
+    NgpTrainBgDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.n_rays == 0) return;
+    if (!check_buffers(buffers, 4, target)) return;
+    hipLaunchKernelGGL(gt_blend_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, d.n_rays,
+                       (const uint8_t *)buffers[0], (const uint32_t *)buffers[1], (const float *)buffers[2],
+                       (float *)buffers[3]);
+    check_launch(target);
+}
+
+extern "C" void train_bg_grad(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "train_bg_grad";
+    NgpTrainBgDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (d.n_rays == 0) return;
+    if (!check_buffers(buffers, 5, target)) return;
+    hipLaunchKernelGGL(bg_grad_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, d.n_rays,
+                       (const uint8_t *)buffers[0], (const uint32_t *)buffers[1], (const float *)buffers[2],
+                       (const float *)buffers[3], (float *)buffers[4]);
     check_launch(target);
 }
 

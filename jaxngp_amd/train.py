@@ -53,6 +53,9 @@ class TrainConfig:
     proposal: bool = False
     proposal_rays: int = 1024
     proposal_every: int = 4  # proposal update on every n-th step
+    # sky-sphere background model trained with the field, for scenes marked "bg"
+    # (app/nerf/_utils.py:128-133; random_bg is then ignored, as there)
+    background_model: bool = False
 
 
 def lr_at(step: int, lr: float) -> float:
@@ -155,6 +158,14 @@ class Trainer:
                 device=self.dev)
             if self.fp16:
                 self.prop.mlp_half = self.prop.mlp_params.detach().half()
+        self.bg_model = None
+        if cfg.background_model:
+            from .background import make_skysphere_background_model_ngp
+            self.bg_model = make_skysphere_background_model_ngp(self.bound, device=self.dev, seed=cfg.seed % 2 ** 31 + 2)
+            nb = self.bg_model.params.numel()
+            sky = _lib.sky_sphere_bg_descriptor(n, 0, 1, self.bg_model.radius)
+            self.bb = dict(dbg=z(n, 3), dparams=z(nb), mm=z(nb), mv=z(nb),
+                           ws=torch.empty(_lib.sky_sphere_bg_workspace_bytes(sky), dtype=torch.uint8, device=self.dev))
         self._calls = None
         self.mark_untrained_density_grid()
 
@@ -333,6 +344,21 @@ class Trainer:
                 p_adam_mlp=P("adam_update", [pr.mlp_params.detach(), bp["dparams"], bp["mm"], bp["mv"],
                                              pr.mlp_half if pr.mlp_half is not None else none], b""),
             )
+        if self.bg_model is not None:
+            bb, bgp = self.bb, self.bg_model.params.detach()
+            sky = _lib.sky_sphere_bg_descriptor(n, 0, 1, self.bg_model.radius)
+            tb = _lib.train_bg_descriptor(n)
+            calls.update(
+                bg_fwd=P("sky_sphere_bg", [b["o"], b["d"], none, bgp, b["bgs"], bb["ws"]], sky),
+                bg_blend=P("train_gt_blend", [data.rgba, b["pix"], b["bgs"], b["gt"]], tb),
+                bg_grad=P("train_bg_grad", [data.rgba, b["pix"], b["g_rgbd"], b["dbgs"], bb["dbg"]], tb),
+                bg_bwd=P("sky_sphere_bg_backward", [b["o"], b["d"], none, bgp, bb["dbg"], bb["dparams"], none, bb["ws"]],
+                         sky),
+                adam_bg=P("adam_update", [bgp, bb["dparams"], bb["mm"], bb["mv"], none] + state, b""),
+            )
+            if self.fp16:
+                calls["check_bg"] = P("loss_scale_check", [bb["dparams"], self.loss_state],
+                                      _lib.loss_scale_check_descriptor(bgp.numel()))
         if self.fp16:
             calls["check_table"] = P("loss_scale_check", [b["dtable"], self.loss_state],
                                      _lib.loss_scale_check_descriptor(b["dtable"].numel()))
@@ -346,17 +372,25 @@ class Trainer:
         """One training step (app/nerf/_utils.py:80-170), then the grid cadence.
         Kernel order: train_make_rays -> march_rays -> hashgrid_encode ->
         nerf_mlp_forward -> integrate_rays -> train_loss_grad -> integrate_rays_backward
-        -> nerf_mlp_backward -> hashgrid_encode_backward -> adam_update (table, MLP)."""
+        -> nerf_mlp_backward -> hashgrid_encode_backward -> adam_update (table, MLP).
+        With a background model, sky_sphere_bg + train_gt_blend follow train_make_rays
+        and train_bg_grad + sky_sphere_bg_backward follow integrate_rays_backward."""
         c = self.cfg
         if self._calls is None:
             self._calls = self._prepare()
         k = self._calls
         k["march"].set_buffer(5, self.grid.bitfield)
         cam, data = self.data.camera, self.data
-        flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg else 0)
+        bg = self.bg_model is not None
+        flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg and not bg else 0)
         k["rays"](_lib.train_rays_descriptor(self.n_rays, data.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx,
                                              cam.cy, self.bound, c.seed, self.step_count, flags, c.bg))
-        for name in ("march", "encode", "mlp", "integrate", "loss", "integrate_bwd", "mlp_bwd", "encode_bwd"):
+        order = ("march", "encode", "mlp", "integrate", "loss", "integrate_bwd", "mlp_bwd", "encode_bwd")
+        if bg:
+            # per-ray backgrounds from the model, the ground truth blended over them, and
+            # their gradient after the integrator's (app/nerf/_utils.py:128-150)
+            order = ("bg_fwd", "bg_blend") + order[:6] + ("bg_grad", "bg_bwd") + order[6:]
+        for name in order:
             k[name]()
         self.step_count += 1
         t = self.step_count
@@ -366,6 +400,8 @@ class Trainer:
         if self.fp16:
             k["check_table"]()
             k["check_mlp"]()
+            if bg:
+                k["check_bg"]()
         n_table = k["adam_table"].keep[0].numel()
         k["adam_table"](_lib.adam_descriptor(n_table, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
                                              mirror_dtype=_lib.DTYPE_F16 if k["mirror"] else _lib.DTYPE_F32,
@@ -373,6 +409,11 @@ class Trainer:
         k["adam_mlp"](_lib.adam_descriptor(N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
                                            mirror_dtype=_lib.DTYPE_F16 if self.fp16 else _lib.DTYPE_F32,
                                            flags=k["flags"]))
+        if bg:
+            # params["bg"] is optimised like the NeRF MLP: the "network" Adam and weight
+            # decay 1e-6 (app/nerf/_utils.py:28-76)
+            k["adam_bg"](_lib.adam_descriptor(k["adam_bg"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
+                                              flags=k["flags"]))
         if self.fp16:
             k["scale_update"]()
         if self.prop is not None and (t - 1) % c.proposal_every == 0:

@@ -187,3 +187,56 @@ def test_fp16_training_path(small_data):
             assert tr.model.position_encoder.table.dtype == torch.float16
             assert 1.0 <= m["loss_scale"] <= 2.0 ** 24
     assert finals["fp16"] < 1.5 * finals["fp32"], finals
+
+
+def test_background_model_training(tmp_path):
+    """TrainConfig(background_model=True): the reference's loss_fn with a sky-sphere
+    background (app/nerf/_utils.py:128-150).  Within a step: the per-ray backgrounds
+    are the model's colours (float64 oracle), the ground truth is blended over them
+    and the background gradient is dL/dbgs - dL/dpred (1 - alpha) (both bit for bit
+    against float32 torch), and the parameter gradient is the oracle's VJP of that
+    gradient at the pre-step parameters.  Training lowers the loss and moves the
+    background parameters."""
+    import oracle
+    dev = torch.device("cuda:0")
+    cam = sc.Camera(width=64, height=64, fx=89.0, fy=89.0, cx=32.0, cy=32.0)
+    blobs = sc.BlobScene.random(seed=3)
+    # cameras inside the sky sphere (radius 4 bound), as in the unbounded scenes it is for
+    data = sc.make_views(blobs, cam, sc.look_at_poses(12, radius=3.0, seed=1), dev, n_steps=256)
+    tr = Trainer(data, TrainConfig(total_samples=1 << 15, background_model=True))
+    p0 = tr.bg_model.params.detach().clone()
+    tr.step()
+    torch.cuda.synchronize()
+    b, bb = tr.b, tr.bb
+    o, d = b["o"].cpu().numpy(), b["d"].cpu().numpy()
+    none = np.zeros((o.shape[0], 0))
+    want_bg = oracle.sky_sphere_bg64(o, d, none, p0.cpu().numpy(), 4.0, 0)
+    assert np.abs(b["bgs"].cpu().numpy() - want_bg).max() <= 2e-6
+    # float32 restatements in numpy (IEEE division; torch divides by a scalar through
+    # its reciprocal)
+    pix = b["pix"].view(torch.int32).long()
+    rgba = data.rgba.reshape(-1, 4)[pix].cpu().numpy().astype(np.float32) / np.float32(255)
+    alpha = rgba[:, 3:]
+    one = np.float32(1)
+    bgs = b["bgs"].cpu().numpy()
+    np.testing.assert_array_equal(b["gt"].cpu().numpy(), rgba[:, :3] * alpha + bgs * (one - alpha))
+    dbg = b["dbgs"].cpu().numpy() - b["g_rgbd"].cpu().numpy()[:, :3] * (one - alpha)
+    np.testing.assert_array_equal(bb["dbg"].cpu().numpy(), dbg)
+    assert np.abs(dbg).max() > 0
+    dp64, _ = oracle.sky_sphere_bg_backward64(o, d, none, p0.cpu().numpy(), 4.0, 0, dbg)
+    assert np.abs(bb["dparams"].cpu().numpy() - dp64).max() <= 2e-5 * np.abs(dp64).max()
+    first = tr.metrics()["loss"]
+    losses = []
+    for i in range(300):
+        tr.step()
+        if i >= 280:
+            losses.append(tr.metrics()["loss"])
+    assert np.mean(losses) < 0.5 * first
+    assert not torch.equal(tr.bg_model.params.detach(), p0)
+    assert torch.isfinite(tr.bg_model.params).all()
+    from jaxngp_amd.checkpoint import restore_trainer, save_checkpoint
+    path = save_checkpoint(str(tmp_path), tr)
+    tr2 = Trainer(data, TrainConfig(total_samples=1 << 15, background_model=True, seed=5))
+    assert not torch.equal(tr2.bg_model.params, tr.bg_model.params)
+    restore_trainer(tr2, path)
+    assert torch.equal(tr2.bg_model.params, tr.bg_model.params)
