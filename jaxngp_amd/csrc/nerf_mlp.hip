@@ -288,7 +288,49 @@ __device__ __forceinline__ void dw_to_lds(float *dst, const v4f *acc, int out_di
         }
 }
 
-__global__ void __launch_bounds__(kThreads)
+// The workgroup's waves split each layer's dW: wave `part` owns column block
+// ob = part of an (IB, 4)-block layer, or row block ib = part of an (4, 1)-block
+// layer, and accumulates it over the tiles of all kWaves waves (their scratch
+// areas, in wave order), so it holds a quarter of the weight gradient.
+template <int IB, int OB>
+__device__ __forceinline__ void dw_accumulate_part(v4f *acc, const float *scratch, int part, int i, int g) {
+    static_assert(OB == 4 || (OB == 1 && IB == 4), "blocks split over four waves");
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const float *zs = scratch + w * 2 * T_FLOATS;
+        const float *ys = zs + T_FLOATS;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int r0 = (4 * t + g) * S_T;
+            if constexpr (OB == 4) {
+                const float bb = ys[r0 + 16 * part + i];
+#pragma unroll
+                for (int ib = 0; ib < IB; ++ib) acc[ib] = mfma(zs[r0 + 16 * ib + i], bb, acc[ib]);
+            } else {
+                acc[0] = mfma(zs[r0 + 16 * part + i], ys[r0 + i], acc[0]);
+            }
+        }
+    }
+}
+
+// Writes this wave's dW blocks into the flax-layout row [in][out_dim] (every
+// element of the layer is owned by exactly one lane of one wave).
+template <int IB, int OB>
+__device__ __forceinline__ void dw_part_to_row(float *dst, const v4f *acc, int out_dim, int part, int i, int g) {
+    if constexpr (OB == 4) {
+#pragma unroll
+        for (int ib = 0; ib < IB; ++ib)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[(16 * ib + 4 * g + r) * out_dim + 16 * part + i] = acc[ib][r];
+    } else {
+        if (i < out_dim) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[(16 * part + 4 * g + r) * out_dim + i] = acc[0][r];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     mlp_backward_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
                         const float *__restrict__ params, const float *__restrict__ dL_ddrgbs,
                         float *__restrict__ dL_denc, float *__restrict__ workspace) {
@@ -301,24 +343,25 @@ __global__ void __launch_bounds__(kThreads)
     float *zs = scratch + wv * 2 * T_FLOATS;
     float *ys = zs + T_FLOATS;
     const uint32_t n_tiles = (n + kTile - 1) / kTile;
-    const uint32_t wave = blockIdx.x * kWaves + wv;
-    const uint32_t n_waves = gridDim.x * kWaves;
+    // the workgroup takes kWaves consecutive tiles at a time (a tile past the end
+    // loads as zeros and contributes nothing), so all its waves meet at the barriers
+    const uint32_t n_groups = (n_tiles + kWaves - 1) / kWaves;
 
-    v4f gW1[8], gW2[4], gW3[8], gW4[16], gW5[4];
+    // this wave's quarter of each weight gradient (see dw_accumulate_part)
+    v4f gW1[2], gW2[1], gW3[2], gW4[4], gW5[1];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) gW1[k] = gW3[k] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 2; ++k) gW1[k] = gW3[k] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    gW2[0] = gW5[0] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int k = 0; k < 4; ++k) gW2[k] = gW5[k] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int k = 0; k < 16; ++k) gW4[k] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 4; ++k) gW4[k] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 
-    TileIn next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(wave, n_tiles - 1) * kTile, i, g);
-    for (uint32_t t = wave; t < n_tiles; t += n_waves) {
-        const size_t s0 = (size_t)t * kTile;
+    TileIn next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, ((size_t)blockIdx.x * kWaves + wv) * kTile, i, g);
+    for (uint32_t tg = blockIdx.x; tg < n_groups; tg += gridDim.x) {
+        const size_t s0 = ((size_t)tg * kWaves + wv) * kTile;
         const size_t s = s0 + i;
         const bool live = s < n;
         const TileIn in = next;
-        next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(t + n_waves, n_tiles - 1) * kTile, i, g);
+        next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, s0 + (size_t)gridDim.x * kWaves * kTile, i, g);
         Fwd f;
         f.x[0] = in.x[0];
         f.x[1] = in.x[1];
@@ -338,72 +381,60 @@ __global__ void __launch_bounds__(kThreads)
         // W5
         tile_to_lds<4>(zs, f.h3, i, g);
         tile_to_lds<1>(ys, &dr, i, g);
-        wave_lds_sync();
-        dw_accumulate<4, 1>(gW5, zs, ys, i, g);
+        __syncthreads();
+        dw_accumulate_part<4, 1>(gW5, scratch, wv, i, g);
         v4f d3[4];
         layer_bwd<64, 16, S_W5>(lds + L_W5, &dr, d3, i, g);
         relu_mask<4>(d3, f.h3);
-        wave_lds_sync();
+        __syncthreads();
         // W4
         tile_to_lds<4>(zs, f.h2, i, g);
         tile_to_lds<4>(ys, d3, i, g);
-        wave_lds_sync();
-        dw_accumulate<4, 4>(gW4, zs, ys, i, g);
+        __syncthreads();
+        dw_accumulate_part<4, 4>(gW4, scratch, wv, i, g);
         v4f d2[4];
         layer_bwd<64, 64, S_W4>(lds + L_W4, d3, d2, i, g);
         relu_mask<4>(d2, f.h2);
-        wave_lds_sync();
+        __syncthreads();
         // W3
         tile_to_lds<2>(zs, f.c, i, g);
         tile_to_lds<4>(ys, d2, i, g);
-        wave_lds_sync();
-        dw_accumulate<2, 4>(gW3, zs, ys, i, g);
+        __syncthreads();
+        dw_accumulate_part<2, 4>(gW3, scratch, wv, i, g);
         v4f dc[2];
         layer_bwd<32, 64, S_W3>(lds + L_W3, d2, dc, i, g);
         if (g == 0) dc[0][0] += dsig;
-        wave_lds_sync();
+        __syncthreads();
         // W2
         tile_to_lds<4>(zs, f.h1, i, g);
         tile_to_lds<1>(ys, &dc[0], i, g);
-        wave_lds_sync();
-        dw_accumulate<4, 1>(gW2, zs, ys, i, g);
+        __syncthreads();
+        dw_accumulate_part<4, 1>(gW2, scratch, wv, i, g);
         v4f d1[4];
         layer_bwd<64, 16, S_W2>(lds + L_W2, &dc[0], d1, i, g);
         relu_mask<4>(d1, f.h1);
-        wave_lds_sync();
+        __syncthreads();
         // W1
         tile_to_lds<2>(zs, f.x, i, g);
         tile_to_lds<4>(ys, d1, i, g);
-        wave_lds_sync();
-        dw_accumulate<2, 4>(gW1, zs, ys, i, g);
+        __syncthreads();
+        dw_accumulate_part<2, 4>(gW1, scratch, wv, i, g);
         v4f dx[2];
         layer_bwd<32, 64, S_W1>(lds + L_W1, d1, dx, i, g);
         if (live) {
 #pragma unroll
             for (int b = 0; b < 2; ++b) store_enc4(dL_denc, level_major, n, s, b, g, dx[b]);
         }
-        wave_lds_sync();
-    }
-
-    // block reduction of the weight gradients in LDS (reusing the weight area),
-    // then one workspace row per workgroup
-    __syncthreads();
-    float *red = lds;
-    for (int t = threadIdx.x; t < P_END; t += kThreads) red[t] = 0.0f;
-    __syncthreads();
-    // waves add in a fixed order, so the result does not depend on scheduling
-    for (int w = 0; w < kWaves; ++w) {
-        if (wv == w) {
-            dw_to_lds<2, 4>(red + P_W1, gW1, 64, i, g);
-            dw_to_lds<4, 1>(red + P_W2, gW2, 16, i, g);
-            dw_to_lds<2, 4>(red + P_W3, gW3, 64, i, g);
-            dw_to_lds<4, 4>(red + P_W4, gW4, 64, i, g);
-            dw_to_lds<4, 1>(red + P_W5, gW5, 3, i, g);
-        }
         __syncthreads();
     }
+
+    // one workspace row per workgroup; each element written by one lane of one wave
     float *row = workspace + (size_t)blockIdx.x * P_END;
-    for (int t = threadIdx.x; t < P_END; t += kThreads) row[t] = red[t];
+    dw_part_to_row<2, 4>(row + P_W1, gW1, 64, wv, i, g);
+    dw_part_to_row<4, 1>(row + P_W2, gW2, 16, wv, i, g);
+    dw_part_to_row<2, 4>(row + P_W3, gW3, 64, wv, i, g);
+    dw_part_to_row<4, 4>(row + P_W4, gW4, 64, wv, i, g);
+    dw_part_to_row<4, 1>(row + P_W5, gW5, 3, wv, i, g);
 }
 
 // out[p] = sum of workspace rows at column p.  Block: 64 columns x 16 waves; wave
@@ -624,38 +655,75 @@ __device__ __forceinline__ void dw_accumulate_h(v4f *acc, const float *zs, const
         for (int ob = 0; ob < OB; ++ob) acc[ib * OB + ob] = mfma_h16(a[ib], bb[ob], acc[ib * OB + ob]);
 }
 
-__global__ void __launch_bounds__(kThreads)
+// tile_to_lds for the f16 backward: the dW MFMAs take f16 operands, so the tile is
+// rounded to f16 once here (the same rounding as at the MFMA) and the scratch
+// takes half the LDS.
+template <int NB>
+__device__ __forceinline__ void tile_to_lds_h(_Float16 *scr, const v4f *z, int i, int g) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        h4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = (_Float16)z[b][r];
+        *reinterpret_cast<h4 *>(scr + i * S_T + 16 * b + 4 * g) = v;
+    }
+}
+
+// dw_accumulate_part with f16 operands (K = the tile's 16 samples per MFMA).
+template <int IB, int OB>
+__device__ __forceinline__ void dw_accumulate_part_h(v4f *acc, const _Float16 *scratch, int part, int i, int g) {
+    static_assert(OB == 4 || (OB == 1 && IB == 4), "blocks split over four waves");
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const _Float16 *zs = scratch + w * 2 * T_FLOATS;
+        const _Float16 *ys = zs + T_FLOATS;
+        const int cb = OB == 4 ? 16 * part : 0;  // this wave's output block in ys
+        h4 bb;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bb[q] = ys[(4 * g + q) * S_T + cb + i];
+#pragma unroll
+        for (int ib = 0; ib < (OB == 4 ? IB : 1); ++ib) {
+            const int rb = OB == 4 ? 16 * ib : 16 * part;  // input block in zs
+            h4 a;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] = zs[(4 * g + q) * S_T + rb + i];
+            acc[ib] = mfma_h16(a, bb, acc[ib]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     mlp_backward_h_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
                           const _Float16 *__restrict__ params, const float *__restrict__ dL_ddrgbs,
                           float *__restrict__ dL_denc, float *__restrict__ workspace) {
     __shared__ __align__(16) _Float16 lds[H_END];
-    __shared__ float scratch[kWaves * 2 * T_FLOATS];
+    __shared__ __align__(16) _Float16 scratch[kWaves * 2 * T_FLOATS];
     stage_weights_h<true>(lds, params);
     const int lane = lane_id();
     const int i = lane & 15, g = lane >> 4;
     const int wv = threadIdx.x / kWave;
-    float *zs = scratch + wv * 2 * T_FLOATS;
-    float *ys = zs + T_FLOATS;
+    _Float16 *zs = scratch + wv * 2 * T_FLOATS;
+    _Float16 *ys = zs + T_FLOATS;
     const uint32_t n_tiles = (n + kTile - 1) / kTile;
-    const uint32_t wave = blockIdx.x * kWaves + wv;
-    const uint32_t n_waves = gridDim.x * kWaves;
+    // kWaves consecutive tiles per workgroup step; tiles past the end load as zeros
+    const uint32_t n_groups = (n_tiles + kWaves - 1) / kWaves;
     const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
 
-    v4f gW1[8], gW2[4], gW3[8], gW4[16], gW5[4];
+    // this wave's quarter of each weight gradient (see dw_accumulate_part)
+    v4f gW1[2], gW2[1], gW3[2], gW4[4], gW5[1];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) gW1[k] = gW3[k] = zero;
+    for (int k = 0; k < 2; ++k) gW1[k] = gW3[k] = zero;
+    gW2[0] = gW5[0] = zero;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) gW2[k] = gW5[k] = zero;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) gW4[k] = zero;
+    for (int k = 0; k < 4; ++k) gW4[k] = zero;
 
-    TileIn next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(wave, n_tiles - 1) * kTile, i, g);
-    for (uint32_t t = wave; t < n_tiles; t += n_waves) {
-        const size_t s0 = (size_t)t * kTile;
+    TileIn next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, ((size_t)blockIdx.x * kWaves + wv) * kTile, i, g);
+    for (uint32_t tg = blockIdx.x; tg < n_groups; tg += gridDim.x) {
+        const size_t s0 = ((size_t)tg * kWaves + wv) * kTile;
         const size_t s = s0 + i;
         const bool live = s < n;
         const TileIn in = next;
-        next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, (size_t)min(t + n_waves, n_tiles - 1) * kTile, i, g);
+        next = load_tile(enc, level_major, dirs, dL_ddrgbs, n, s0 + (size_t)gridDim.x * kWaves * kTile, i, g);
         FwdH f;
         f.x[0] = in.x[0];
         f.x[1] = in.x[1];
@@ -672,50 +740,50 @@ __global__ void __launch_bounds__(kThreads)
             dsig = gin[0] * det_expf(fminf(fmaxf(f.c[0][0], -15.0f), 15.0f));
         }
         // W5
-        tile_to_lds<4>(zs, f.h3, i, g);
-        tile_to_lds<1>(ys, &dr, i, g);
-        wave_lds_sync();
-        dw_accumulate_h<4, 1>(gW5, zs, ys, i, g);
+        tile_to_lds_h<4>(zs, f.h3, i, g);
+        tile_to_lds_h<1>(ys, &dr, i, g);
+        __syncthreads();
+        dw_accumulate_part_h<4, 1>(gW5, scratch, wv, i, g);
         const h8 drin = pack_h8(dr, zero);
         v4f d3[4];
         layer_bwd_h<4, 1, BT5, true>(lds + H_B5, &drin, d3, i, g);
         relu_mask<4>(d3, f.h3);
-        wave_lds_sync();
+        __syncthreads();
         // W4
-        tile_to_lds<4>(zs, f.h2, i, g);
-        tile_to_lds<4>(ys, d3, i, g);
-        wave_lds_sync();
-        dw_accumulate_h<4, 4>(gW4, zs, ys, i, g);
+        tile_to_lds_h<4>(zs, f.h2, i, g);
+        tile_to_lds_h<4>(ys, d3, i, g);
+        __syncthreads();
+        dw_accumulate_part_h<4, 4>(gW4, scratch, wv, i, g);
         const h8 d3in[2] = {pack_h8(d3[0], d3[1]), pack_h8(d3[2], d3[3])};
         v4f d2[4];
         layer_bwd_h<4, 2, BT4, false>(lds + H_B4, d3in, d2, i, g);
         relu_mask<4>(d2, f.h2);
-        wave_lds_sync();
+        __syncthreads();
         // W3
-        tile_to_lds<2>(zs, f.c, i, g);
-        tile_to_lds<4>(ys, d2, i, g);
-        wave_lds_sync();
-        dw_accumulate_h<2, 4>(gW3, zs, ys, i, g);
+        tile_to_lds_h<2>(zs, f.c, i, g);
+        tile_to_lds_h<4>(ys, d2, i, g);
+        __syncthreads();
+        dw_accumulate_part_h<2, 4>(gW3, scratch, wv, i, g);
         const h8 d2in[2] = {pack_h8(d2[0], d2[1]), pack_h8(d2[2], d2[3])};
         v4f dc[2];
         layer_bwd_h<2, 2, BT3, false>(lds + H_B3, d2in, dc, i, g);
         if (g == 0) dc[0][0] += dsig;
-        wave_lds_sync();
+        __syncthreads();
         // W2
-        tile_to_lds<4>(zs, f.h1, i, g);
-        tile_to_lds<1>(ys, &dc[0], i, g);
-        wave_lds_sync();
-        dw_accumulate_h<4, 1>(gW2, zs, ys, i, g);
+        tile_to_lds_h<4>(zs, f.h1, i, g);
+        tile_to_lds_h<1>(ys, &dc[0], i, g);
+        __syncthreads();
+        dw_accumulate_part_h<4, 1>(gW2, scratch, wv, i, g);
         const h8 dcin = pack_h8(dc[0], zero);
         v4f d1[4];
         layer_bwd_h<4, 1, BT2, true>(lds + H_B2, &dcin, d1, i, g);
         relu_mask<4>(d1, f.h1);
-        wave_lds_sync();
+        __syncthreads();
         // W1
-        tile_to_lds<2>(zs, f.x, i, g);
-        tile_to_lds<4>(ys, d1, i, g);
-        wave_lds_sync();
-        dw_accumulate_h<2, 4>(gW1, zs, ys, i, g);
+        tile_to_lds_h<2>(zs, f.x, i, g);
+        tile_to_lds_h<4>(ys, d1, i, g);
+        __syncthreads();
+        dw_accumulate_part_h<2, 4>(gW1, scratch, wv, i, g);
         const h8 d1in[2] = {pack_h8(d1[0], d1[1]), pack_h8(d1[2], d1[3])};
         v4f dx[2];
         layer_bwd_h<2, 2, BT1, false>(lds + H_B1, d1in, dx, i, g);
@@ -723,28 +791,19 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
             for (int b = 0; b < 2; ++b) store_enc4(dL_denc, level_major, n, s, b, g, dx[b]);
         }
-        wave_lds_sync();
-    }
-
-    __syncthreads();
-    float *red = reinterpret_cast<float *>(lds);  // 9408 floats fit in the 48,640 bytes
-    for (int t = threadIdx.x; t < P_END; t += kThreads) red[t] = 0.0f;
-    __syncthreads();
-    for (int w = 0; w < kWaves; ++w) {
-        if (wv == w) {
-            dw_to_lds<2, 4>(red + P_W1, gW1, 64, i, g);
-            dw_to_lds<4, 1>(red + P_W2, gW2, 16, i, g);
-            dw_to_lds<2, 4>(red + P_W3, gW3, 64, i, g);
-            dw_to_lds<4, 4>(red + P_W4, gW4, 64, i, g);
-            dw_to_lds<4, 1>(red + P_W5, gW5, 3, i, g);
-        }
         __syncthreads();
     }
+
+    // one workspace row per workgroup; each element written by one lane of one wave
     float *row = workspace + (size_t)blockIdx.x * P_END;
-    for (int t = threadIdx.x; t < P_END; t += kThreads) row[t] = red[t];
+    dw_part_to_row<2, 4>(row + P_W1, gW1, 64, wv, i, g);
+    dw_part_to_row<4, 1>(row + P_W2, gW2, 16, wv, i, g);
+    dw_part_to_row<2, 4>(row + P_W3, gW3, 64, wv, i, g);
+    dw_part_to_row<4, 4>(row + P_W4, gW4, 64, wv, i, g);
+    dw_part_to_row<4, 1>(row + P_W5, gW5, 3, wv, i, g);
 }
 
-static_assert(H_END * 2 >= P_END * 4, "the weight area holds the gradient reduction");
+
 
 // ---------------------------------------------------------------------------
 // Fused field query of the inference renderer: hash encoding (L = 16, F = 2) and
