@@ -107,21 +107,22 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
 
     plain = tr._calls
     tr._calls = {k: (Timed(v) if hasattr(v, "name") and v.name in events else v) for k, v in plain.items()}
-    samples = torch.zeros((), dtype=torch.int64, device=dev)
-    rays = torch.zeros((), dtype=torch.int64, device=dev)
+    # per-step counters (next_loc, exceeded, n_valid, measured) logged on the device
+    # with one 16-byte copy per step, summed after the timed region
+    counts = torch.zeros(steps, 4, dtype=torch.int32, device=dev)
+    ctr = tr.counters.view(torch.int32)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         tr.step()
-        b = tr.b
-        samples += b["next_loc"].view(torch.int32).long()[0] - b["exceeded"].view(torch.int32).long()[0]
-        rays += b["n_valid"].view(torch.int32).long()[0]
+        counts[i].copy_(ctr)
     t_host = time.perf_counter() - t_start
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     tr._calls = plain
-    n_samples = int(samples.item())
-    n_valid = int(rays.item())
+    c = counts.long()
+    n_samples = int((c[:, 0] - c[:, 1]).sum().item())
+    n_valid = int(c[:, 2].sum().item())
     ms = elapsed / steps * 1e3
     log(f"[{cfg.precision}] timed {steps} steps: {ms:.3f} ms/step ({t_host / steps * 1e3:.3f} ms/step to "
         f"enqueue), {n_samples / steps:.0f} samples/step, {n_valid / steps:.0f} valid rays/step, "

@@ -146,12 +146,34 @@ __global__ void zero_u32_kernel(uint32_t *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
-__global__ void __launch_bounds__(kBlock) count_true_kernel(uint32_t n, const bool *__restrict__ flags,
-                                                            uint32_t *__restrict__ count) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t v = (i < n && flags[i]) ? 1u : 0u;
-    const uint32_t s = wave_sum(v);
-    if (lane_id() == 0 && s) atomicAdd(count, s);
+// One workgroup counts the valid rays and writes the count and a zero loss
+// accumulator, so the loss needs no separate clearing launches (the batch's flags
+// are a few hundred KB; 16 bytes per load).
+constexpr int kCountThreads = 1024;
+__global__ void __launch_bounds__(kCountThreads) count_valid_kernel(uint32_t n, const bool *__restrict__ flags,
+                                                                    uint32_t *__restrict__ count,
+                                                                    float *__restrict__ loss) {
+    __shared__ uint32_t wsum[kCountThreads / kWave];
+    const uint8_t *f = reinterpret_cast<const uint8_t *>(flags);
+    uint32_t c = 0;
+    const uint32_t n16 = (reinterpret_cast<uintptr_t>(f) & 15u) == 0 ? n / 16 : 0;
+    for (uint32_t q = threadIdx.x; q < n16; q += kCountThreads) {
+        const uint4 v = reinterpret_cast<const uint4 *>(f)[q];
+        // bools are 0 or 1: add the bytes of each word
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c += (w[k] & 0xffu) + ((w[k] >> 8) & 0xffu) + ((w[k] >> 16) & 0xffu) + (w[k] >> 24);
+    }
+    for (uint32_t i = 16 * n16 + threadIdx.x; i < n; i += kCountThreads) c += f[i] ? 1u : 0u;
+    c = wave_sum(c);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kCountThreads / kWave; ++w) t += wsum[w];
+        *count = t;
+        *loss = 0.0f;
+    }
 }
 
 // optax.huber_loss(pred, target, delta): 0.5 e^2 if |e| <= delta else delta (|e| - delta/2)
@@ -281,11 +303,9 @@ extern "C" void train_loss_grad(ngp_stream_t stream_, void **buffers, const char
     hipStream_t stream = (hipStream_t)stream_;
     uint32_t *n_valid = (uint32_t *)buffers[3];
     float *loss = (float *)buffers[5];
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, n_valid, 1);
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, (uint32_t *)loss, 1);
+    hipLaunchKernelGGL(count_valid_kernel, dim3(1), dim3(kCountThreads), 0, stream, d.n_rays,
+                       (const bool *)buffers[2], n_valid, loss);
     if (d.n_rays > 0) {
-        hipLaunchKernelGGL(count_true_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
-                           (const bool *)buffers[2], n_valid);
         hipLaunchKernelGGL(loss_grad_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
                            d.delta, (const float *)buffers[0], (const float *)buffers[1], (const bool *)buffers[2],
                            n_valid, scaled ? (const NgpLossScaleState *)buffers[6] : nullptr, (float *)buffers[4],

@@ -119,11 +119,14 @@ class Trainer:
         self.n_rays = n
         f32, u32 = torch.float32, torch.uint32
         z = lambda *shape, dtype=f32: torch.zeros(*shape, dtype=dtype, device=self.dev)
+        # the step's counters share one 16-byte tensor, so a caller can log them with one copy
+        self.counters = z(4, dtype=u32)
+        cnt = self.counters
         self.b = dict(
             o=z(n, 3), d=z(n, 3), t0=z(n), t1=z(n), noise=z(n), bgs=z(n, 3), gt=z(n, 3), pix=z(n, dtype=u32),
-            next_loc=z(1, dtype=u32), exceeded=z(1, dtype=u32), valid=z(n, dtype=torch.bool), ns=z(n, dtype=u32),
+            next_loc=cnt[0:1], exceeded=cnt[1:2], valid=z(n, dtype=torch.bool), ns=z(n, dtype=u32),
             start=z(n, dtype=u32), idcs=z(S, dtype=u32), xyzs=z(S, 3), dirs=z(S, 3), dss=z(S), zs=z(S),
-            enc=z(16, S, 2), drgbs=z(S, 4), measured=z(1, dtype=u32), rgbd=z(n, 4), op=z(n), n_valid=z(1, dtype=u32),
+            enc=z(16, S, 2), drgbs=z(S, 4), measured=cnt[3:4], rgbd=z(n, 4), op=z(n), n_valid=cnt[2:3],
             loss=z(1), g_rgbd=z(n, 4), dbgs=z(n, 3), dz=z(S), ddrgbs=z(S, 4), denc=z(16, S, 2), dparams=z(N_PARAMS),
             ws=z(mlp_workspace_blocks(S), N_PARAMS), dtable=z(*self.model.position_encoder.table.shape),
         )
