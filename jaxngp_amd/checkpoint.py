@@ -200,6 +200,8 @@ def save_checkpoint(ckpt_dir: str, trainer, n_frames: int | None = None, bg_mode
     import torch
 
     os.makedirs(ckpt_dir, exist_ok=True)
+    if hasattr(trainer, "sync"):
+        trainer.sync()  # the last update may still run on the trainer's side stream
     if bg_model is None:
         bg_model = getattr(trainer, "bg_model", None)
     # the f16 path keeps f32 master weights beside its f16 table; the f32 path updates
@@ -237,6 +239,8 @@ def restore_trainer(trainer, path: str) -> int:
     import torch
 
     ck = load_checkpoint(path)
+    if hasattr(trainer, "sync"):
+        trainer.sync()  # no pending update may overwrite what is restored
     m = trainer.model
     dev = m.mlp_params.device
     table = m.position_encoder.table

@@ -56,6 +56,9 @@ class TrainConfig:
     # sky-sphere background model trained with the field, for scenes marked "bg"
     # (app/nerf/_utils.py:128-133; random_bg is then ignored, as there)
     background_model: bool = False
+    # run the optimizer update on a side stream, overlapping the next step's ray
+    # generation and march (False: on the main stream, in order)
+    overlap_update: bool = True
 
 
 def lr_at(step: int, lr: float) -> float:
@@ -102,6 +105,9 @@ class Trainer:
         self.cfg = cfg
         self.data = data
         self.dev = device or data.rgba.device
+        # The optimizer runs on a side stream, overlapping the next step's ray
+        # generation and march (which read no parameters); see step().
+        self._adam_pending = False
         self.bound = float(data.scene.bound)
         self.K = max(1, int(1 + math.ceil(math.log2(self.bound))))  # SceneMeta.cascades, types.py:858-860
         self.portion = 0.0 if self.bound <= 1 else 1 / 256  # stepsize_portion (types.py:874-886), bound in (1, 4)
@@ -110,7 +116,7 @@ class Trainer:
         self.fp16 = cfg.precision == "fp16"
         if self.fp16:
             table_dtype = torch.float16
-        self.model = NeRF(bound=self.bound, device=self.dev, seed=cfg.seed % 2 ** 31, table_dtype=table_dtype)
+        self._model = NeRF(bound=self.bound, device=self.dev, seed=cfg.seed % 2 ** 31, table_dtype=table_dtype)
         self.grid = OccupancyGrid(self.K, cfg.grid_res, self.dev)
         self.step_count = 0
         self.rng = torch.Generator(device=self.dev).manual_seed(cfg.seed % 2 ** 31)
@@ -161,16 +167,41 @@ class Trainer:
                 device=self.dev)
             if self.fp16:
                 self.prop.mlp_half = self.prop.mlp_params.detach().half()
-        self.bg_model = None
+        self._bg_model = None
         if cfg.background_model:
             from .background import make_skysphere_background_model_ngp
-            self.bg_model = make_skysphere_background_model_ngp(self.bound, device=self.dev, seed=cfg.seed % 2 ** 31 + 2)
-            nb = self.bg_model.params.numel()
-            sky = _lib.sky_sphere_bg_descriptor(n, 0, 1, self.bg_model.radius)
+            self._bg_model = make_skysphere_background_model_ngp(self.bound, device=self.dev,
+                                                                 seed=cfg.seed % 2 ** 31 + 2)
+            nb = self._bg_model.params.numel()
+            sky = _lib.sky_sphere_bg_descriptor(n, 0, 1, self._bg_model.radius)
             self.bb = dict(dbg=z(n, 3), dparams=z(nb), mm=z(nb), mv=z(nb),
                            ws=torch.empty(_lib.sky_sphere_bg_workspace_bytes(sky), dtype=torch.uint8, device=self.dev))
+        self.main_stream = torch.cuda.current_stream(self.dev)
+        self.side_stream = torch.cuda.Stream(self.dev) if cfg.overlap_update else self.main_stream  # default priority
+        self._bwd_done = torch.cuda.Event()
+        self._adam_done = torch.cuda.Event()
         self._calls = None
         self.mark_untrained_density_grid()
+
+    # -- parameters and the optimizer's side stream ----------------------------------------
+
+    def sync(self) -> None:
+        """Makes work enqueued on the main stream from now on wait for the last step's
+        optimizer update (which runs on the side stream)."""
+        if self._adam_pending:
+            self.main_stream.wait_event(self._adam_done)
+
+    @property
+    def model(self) -> NeRF:
+        """The field; reading it orders the main stream after the last update."""
+        self.sync()
+        return self._model
+
+    @property
+    def bg_model(self):
+        """The background model (or None); as `model`."""
+        self.sync()
+        return self._bg_model
 
     # -- occupancy grid -------------------------------------------------------------------
 
@@ -295,6 +326,7 @@ class Trainer:
         else:
             mlp_w, pd, flags, state = params, _lib.DTYPE_F32, 0, []
         none = torch.empty(0, device=self.dev)  # a NULL mirror slot
+        side = self.side_stream.cuda_stream  # the optimizer's stream (see step)
         calls = dict(
             rays=P("train_make_rays", [data.transforms, data.rgba, b["o"], b["d"], b["t0"], b["t1"], b["noise"],
                                        b["bgs"], b["gt"], b["pix"]], b""),
@@ -316,9 +348,9 @@ class Trainer:
                                             b["ws"]], _lib.nerf_mlp_descriptor(S, b["ws"].shape[0], LM, pd)),
             encode_bwd=P("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"], b["hg_ws"]], hdesc),
             adam_table=P("adam_update", [tab_master, b["dtable"], o["tm"], o["tv"]] +
-                         [mirror if mirror is not None else none] + state, b""),
+                         [mirror if mirror is not None else none] + state, b"", stream=side),
             adam_mlp=P("adam_update", [params, b["dparams"], o["mm"], o["mv"]] +
-                       [m.mlp_half if self.fp16 else none] + state, b""),
+                       [m.mlp_half if self.fp16 else none] + state, b"", stream=side),
             mirror=mirror is not None,
             flags=flags,
         )
@@ -347,9 +379,9 @@ class Trainer:
                 p_adam_mlp=P("adam_update", [pr.mlp_params.detach(), bp["dparams"], bp["mm"], bp["mv"],
                                              pr.mlp_half if pr.mlp_half is not None else none], b""),
             )
-        if self.bg_model is not None:
-            bb, bgp = self.bb, self.bg_model.params.detach()
-            sky = _lib.sky_sphere_bg_descriptor(n, 0, 1, self.bg_model.radius)
+        if self._bg_model is not None:
+            bb, bgp = self.bb, self._bg_model.params.detach()
+            sky = _lib.sky_sphere_bg_descriptor(n, 0, 1, self._bg_model.radius)
             tb = _lib.train_bg_descriptor(n)
             calls.update(
                 bg_fwd=P("sky_sphere_bg", [b["o"], b["d"], none, bgp, b["bgs"], bb["ws"]], sky),
@@ -357,42 +389,49 @@ class Trainer:
                 bg_grad=P("train_bg_grad", [data.rgba, b["pix"], b["g_rgbd"], b["dbgs"], bb["dbg"]], tb),
                 bg_bwd=P("sky_sphere_bg_backward", [b["o"], b["d"], none, bgp, bb["dbg"], bb["dparams"], none, bb["ws"]],
                          sky),
-                adam_bg=P("adam_update", [bgp, bb["dparams"], bb["mm"], bb["mv"], none] + state, b""),
+                adam_bg=P("adam_update", [bgp, bb["dparams"], bb["mm"], bb["mv"], none] + state, b"", stream=side),
             )
             if self.fp16:
                 calls["check_bg"] = P("loss_scale_check", [bb["dparams"], self.loss_state],
-                                      _lib.loss_scale_check_descriptor(bgp.numel()))
+                                      _lib.loss_scale_check_descriptor(bgp.numel()), stream=side)
         if self.fp16:
             calls["check_table"] = P("loss_scale_check", [b["dtable"], self.loss_state],
-                                     _lib.loss_scale_check_descriptor(b["dtable"].numel()))
+                                     _lib.loss_scale_check_descriptor(b["dtable"].numel()), stream=side)
             calls["check_mlp"] = P("loss_scale_check", [b["dparams"], self.loss_state],
-                                   _lib.loss_scale_check_descriptor(N_PARAMS))
+                                   _lib.loss_scale_check_descriptor(N_PARAMS), stream=side)
             calls["scale_update"] = P("loss_scale_update", [self.loss_state],
-                                      _lib.loss_scale_update_descriptor(c.loss_scale_growth_interval))
+                                      _lib.loss_scale_update_descriptor(c.loss_scale_growth_interval), stream=side)
         return calls
 
     def step(self) -> None:
         """One training step (app/nerf/_utils.py:80-170), then the grid cadence.
-        Kernel order: train_make_rays -> march_rays -> hashgrid_encode ->
-        nerf_mlp_forward -> integrate_rays -> train_loss_grad -> integrate_rays_backward
-        -> nerf_mlp_backward -> hashgrid_encode_backward -> adam_update (table, MLP).
-        With a background model, sky_sphere_bg + train_gt_blend follow train_make_rays
-        and train_bg_grad + sky_sphere_bg_backward follow integrate_rays_backward."""
+        Kernel order: train_make_rays -> march_rays -> (wait for the previous update)
+        -> hashgrid_encode -> nerf_mlp_forward -> integrate_rays -> train_loss_grad ->
+        integrate_rays_backward -> nerf_mlp_backward -> hashgrid_encode_backward, then
+        adam_update (table, MLP) on the side stream, where it overlaps the next step's
+        train_make_rays and march_rays.  With a background model, sky_sphere_bg +
+        train_gt_blend follow the wait and train_bg_grad + sky_sphere_bg_backward
+        follow integrate_rays_backward.  Readers of the parameters outside the step go
+        through `model`, `bg_model` or `sync()`."""
         c = self.cfg
         if self._calls is None:
             self._calls = self._prepare()
         k = self._calls
         k["march"].set_buffer(5, self.grid.bitfield)
         cam, data = self.data.camera, self.data
-        bg = self.bg_model is not None
+        bg = self._bg_model is not None
         flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg and not bg else 0)
         k["rays"](_lib.train_rays_descriptor(self.n_rays, data.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx,
                                              cam.cy, self.bound, c.seed, self.step_count, flags, c.bg))
-        order = ("march", "encode", "mlp", "integrate", "loss", "integrate_bwd", "mlp_bwd", "encode_bwd")
+        # rays and march read no parameters: they overlap the last step's update on the
+        # side stream; everything after waits for it
+        k["march"]()
+        self.sync()
+        order = ("encode", "mlp", "integrate", "loss", "integrate_bwd", "mlp_bwd", "encode_bwd")
         if bg:
             # per-ray backgrounds from the model, the ground truth blended over them, and
             # their gradient after the integrator's (app/nerf/_utils.py:128-150)
-            order = ("bg_fwd", "bg_blend") + order[:6] + ("bg_grad", "bg_bwd") + order[6:]
+            order = ("bg_fwd", "bg_blend") + order[:5] + ("bg_grad", "bg_bwd") + order[5:]
         for name in order:
             k[name]()
         self.step_count += 1
@@ -400,25 +439,32 @@ class Trainer:
         lr = lr_at(t - 1, c.lr)
         b1, b2 = 0.9, 0.99
         bp1, bp2 = float(np.float32(b1) ** t), float(np.float32(b2) ** t)
-        if self.fp16:
-            k["check_table"]()
-            k["check_mlp"]()
+        # the update on the side stream, once the gradients exist; the next step's
+        # rays and march run meanwhile
+        self._bwd_done.record(self.main_stream)
+        self.side_stream.wait_event(self._bwd_done)
+        with torch.cuda.stream(self.side_stream):
+            if self.fp16:
+                k["check_table"]()
+                k["check_mlp"]()
+                if bg:
+                    k["check_bg"]()
+            n_table = k["adam_table"].keep[0].numel()
+            k["adam_table"](_lib.adam_descriptor(n_table, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
+                                                 mirror_dtype=_lib.DTYPE_F16 if k["mirror"] else _lib.DTYPE_F32,
+                                                 flags=k["flags"]))
+            k["adam_mlp"](_lib.adam_descriptor(N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
+                                               mirror_dtype=_lib.DTYPE_F16 if self.fp16 else _lib.DTYPE_F32,
+                                               flags=k["flags"]))
             if bg:
-                k["check_bg"]()
-        n_table = k["adam_table"].keep[0].numel()
-        k["adam_table"](_lib.adam_descriptor(n_table, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
-                                             mirror_dtype=_lib.DTYPE_F16 if k["mirror"] else _lib.DTYPE_F32,
-                                             flags=k["flags"]))
-        k["adam_mlp"](_lib.adam_descriptor(N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
-                                           mirror_dtype=_lib.DTYPE_F16 if self.fp16 else _lib.DTYPE_F32,
-                                           flags=k["flags"]))
-        if bg:
-            # params["bg"] is optimised like the NeRF MLP: the "network" Adam and weight
-            # decay 1e-6 (app/nerf/_utils.py:28-76)
-            k["adam_bg"](_lib.adam_descriptor(k["adam_bg"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
-                                              flags=k["flags"]))
-        if self.fp16:
-            k["scale_update"]()
+                # params["bg"] is optimised like the NeRF MLP: the "network" Adam and
+                # weight decay 1e-6 (app/nerf/_utils.py:28-76)
+                k["adam_bg"](_lib.adam_descriptor(k["adam_bg"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, bp1, bp2,
+                                                  1e-6, flags=k["flags"]))
+            if self.fp16:
+                k["scale_update"]()
+        self._adam_done.record(self.side_stream)
+        self._adam_pending = True
         if self.prop is not None and (t - 1) % c.proposal_every == 0:
             for name in ("p_points", "p_encode", "p_mlp", "p_loss", "p_mlp_bwd", "p_encode_bwd"):
                 k[name]()
@@ -436,6 +482,7 @@ class Trainer:
     # -- metrics (these synchronise) -------------------------------------------------------
 
     def metrics(self) -> dict:
+        self.sync()  # the loss-scale state is written by the update
         b = self.b
         u = lambda t: int(t.view(torch.int32)[0].item())
         out = dict(loss=float(b["loss"][0].item()), n_valid_rays=u(b["n_valid"]),
