@@ -348,9 +348,15 @@ __global__ void __launch_bounds__(kScanThreads)
             v[k] = b < n_blocks ? block_slots[(size_t)b * kRaysPerBlock] : kPastFill;
         }
         __syncthreads();
+        // first skipped block of the tile: block indices grow with the thread and slot,
+        // so the lowest lane holding one has the wave's first; one LDS atomic per wave
+        // (not one per skipped block, which serialised on the one address)
+        uint32_t my_first = 0xffffffffu;
 #pragma unroll
-        for (int k = 0; k < kScanPerThread; ++k)
-            if (v[k] == kPastFill) atomicMin(&first_past, t0 + tid * kScanPerThread + k);
+        for (int k = kScanPerThread - 1; k >= 0; --k)
+            if (v[k] == kPastFill) my_first = t0 + tid * kScanPerThread + k;
+        const uint64_t has = __ballot(my_first != 0xffffffffu);
+        if (has != 0ull && lane == __ffsll((unsigned long long)has) - 1) atomicMin(&first_past, my_first);
         __syncthreads();
         const uint32_t stop = first_past;
         uint32_t mine = 0;
