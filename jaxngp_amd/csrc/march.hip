@@ -10,11 +10,11 @@
 //   1. march_count:  one wavefront per ray counts its samples; the block total is
 //                    parked in rays_sample_startidx[first ray of the block] (slot
 //                    owned by the block, so no scratch memory).  Rays are counted
-//                    in ray-ordered launches (1024 rays, up to 8192, up to 65536,
-//                    the rest); between launches the running total goes to
-//                    next_sample_write_location, and a launch that finds the
-//                    sample buffer already full skips marching, which changes no
-//                    output (its rays are invalid regardless of their counts);
+//                    in ray-ordered launches (2048 rays, up to 16384, up to
+//                    131072, the rest); a later launch's workgroups first add the
+//                    parked totals of the earlier launches, and a launch that finds
+//                    the sample buffer already full skips marching, which changes
+//                    no output (its rays are invalid regardless of their counts);
 //   2. march_scan:   one workgroup scans the block totals, finds the ray where the
 //                    sample buffer fills up and writes the two helper counters;
 //   3. march_write:  one wavefront per ray takes its offset, decides validity,
@@ -253,18 +253,59 @@ __device__ __forceinline__ float start_t(const MarchParams &p, float t0, float n
     return t0 + step_size(p, t0) * noise;  // marching.cu:159-160
 }
 
-// Counts the rays of blocks [block_begin, block_end) (grid-stride).  With
-// check_running, a launch whose predecessors already took total_samples samples
-// (running total in *running) marks its blocks kPastFill without marching.
+// Whether the blocks before `block_begin` (earlier launches, finished) already took
+// total_samples samples: the workgroup adds their parked totals 1024 at a time and
+// stops as soon as the sum reaches total_samples or meets a block an earlier
+// launch marked kPastFill (that launch saw the buffer full).  Once the batch fills
+// in the first launch, as in training, that is one chunk per workgroup.
+__device__ bool filled_before(uint32_t block_begin, uint32_t total_samples,
+                              const uint32_t *__restrict__ block_totals) {
+    constexpr int kPer = 4;
+    __shared__ uint64_t part[kBlock / kWave];
+    __shared__ int verdict;
+    const int wave = (int)threadIdx.x / kWave;
+    uint64_t run = 0;
+    for (uint32_t c0 = 0; c0 < block_begin; c0 += kBlock * kPer) {
+        uint64_t sum = 0;
+        bool past = false;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const uint32_t b = c0 + k * kBlock + threadIdx.x;
+            if (b < block_begin) {
+                const uint32_t v = block_totals[(size_t)b * kRaysPerBlock];
+                if (v == kPastFill) past = true;
+                else sum += v;
+            }
+        }
+        sum = wave_sum64(sum);
+        const bool wave_past = __ballot(past) != 0ull;
+        if (lane_id() == 0) part[wave] = wave_past ? ~0ull : sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            bool full = false;
+            for (int w = 0; w < kBlock / kWave; ++w) {
+                if (part[w] == ~0ull) full = true;
+                else run += part[w];
+            }
+            verdict = full || run >= total_samples;
+        }
+        __syncthreads();
+        if (verdict) return true;
+    }
+    return false;
+}
+
+// Counts the rays of blocks [block_begin, block_end) (grid-stride).  A launch whose
+// predecessors already took total_samples samples marks its blocks kPastFill
+// without marching.
 __global__ void __launch_bounds__(kBlock)
     march_count_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples, uint32_t block_begin,
-                       uint32_t block_end, bool check_running, const uint32_t *__restrict__ running,
-                       const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+                       uint32_t block_end, const float *__restrict__ rays_o, const float *__restrict__ rays_d,
                        const float *__restrict__ t_starts, const float *__restrict__ t_ends,
                        const float *__restrict__ noises, const uint8_t *__restrict__ bits,
                        uint32_t *__restrict__ rays_n_samples,
                        uint32_t *__restrict__ block_totals /* = rays_sample_startidx */) {
-    if (check_running && *running >= total_samples) {
+    if (block_begin > 0 && filled_before(block_begin, total_samples, block_totals)) {
         for (uint32_t b = block_begin + blockIdx.x * kBlock + threadIdx.x; b < block_end; b += gridDim.x * kBlock)
             block_totals[(size_t)b * kRaysPerBlock] = kPastFill;
         return;
@@ -291,26 +332,6 @@ __global__ void __launch_bounds__(kBlock)
             block_totals[(size_t)blk * kRaysPerBlock] = (uint32_t)min(s, (uint64_t)kPastFill - 1);
         }
         __syncthreads();
-    }
-}
-
-// Single workgroup: *running (+)= the parked totals of blocks [b0, b1), saturated.
-__global__ void __launch_bounds__(kScanThreads)
-    march_accumulate_kernel(uint32_t b0, uint32_t b1, bool first, const uint32_t *__restrict__ block_totals,
-                            uint32_t *__restrict__ running) {
-    __shared__ uint64_t part[kScanThreads / kWave];
-    uint64_t s = 0;
-    for (uint32_t b = b0 + threadIdx.x; b < b1; b += kScanThreads) {
-        const uint32_t v = block_totals[(size_t)b * kRaysPerBlock];
-        s += v == kPastFill ? 0u : v;
-    }
-    s = wave_sum64(s);
-    if (lane_id() == 0) part[threadIdx.x / kWave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = first ? 0ull : *running;
-        for (int w = 0; w < kScanThreads / kWave; ++w) t += part[w];
-        *running = t > 0xffffffffull ? 0xffffffffu : (uint32_t)t;
     }
 }
 
@@ -731,14 +752,9 @@ extern "C" void march_rays(ngp_stream_t stream_, void **buffers, const char *opa
     for (int c = 0; c < 4; ++c) {
         if (bounds[c + 1] <= bounds[c]) continue;
         hipLaunchKernelGGL(march_count_kernel, dim3(std::min(bounds[c + 1] - bounds[c], kMaxCountBlocks)), dim3(kBlock),
-                           0, stream, p, word_ok, d.n_rays, d.total_samples, bounds[c], bounds[c + 1], c > 0, next_loc,
-                           rays_o, rays_d, t_starts, t_ends, noises, bits, n_samples, startidx);
+                           0, stream, p, word_ok, d.n_rays, d.total_samples, bounds[c], bounds[c + 1], rays_o, rays_d,
+                           t_starts, t_ends, noises, bits, n_samples, startidx);
         if (!check_launch(target)) return;
-        if (bounds[c + 1] < n_blocks) {
-            hipLaunchKernelGGL(march_accumulate_kernel, dim3(1), dim3(kScanThreads), 0, stream, bounds[c],
-                               bounds[c + 1], c == 0, startidx, next_loc);
-            if (!check_launch(target)) return;
-        }
     }
     hipLaunchKernelGGL(march_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, d.n_rays, n_blocks, d.total_samples,
                        n_samples, startidx, next_loc, exceeded);

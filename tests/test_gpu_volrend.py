@@ -92,6 +92,24 @@ def test_march_rays_bit_exact(dev, case):
     _assert_march_equal(got, want)
 
 
+def test_march_rays_fill_in_later_launches(dev):
+    """Rays are counted in ray-ordered launches (2048, 16384, 131072, the rest); a
+    later launch first adds the earlier launches' parked block totals.  A sparse
+    field over 2^18 rays, with buffer sizes the oracle's own cumulative counts put
+    inside the third launch, inside the fourth, and past every ray (no fill)."""
+    rng = np.random.default_rng(77)
+    n = 2 ** 18
+    o, d, t0, t1, noise, bits, _ = march_inputs(rng, n)
+    bits = bits & rng.integers(0, 256, bits.shape, dtype=np.uint8) & rng.integers(0, 256, bits.shape,
+                                                                                    dtype=np.uint8)
+    full = oracle.march_rays(2 ** 23, 1024, 1, 128, 1.0, 0.0, o, d, t0, t1, noise, bits)
+    cum = np.cumsum(full["rays_n_samples"].astype(np.int64))
+    assert cum[-1] > 0
+    for S in (int(cum[60000]) + 1, int(cum[200000]) + 1, int(cum[-1]) + 1000):
+        got, want = _march_both(dev, S, 1024, 1, 128, 1.0, 0.0, o, d, t0, t1, noise, bits)
+        _assert_march_equal(got, want)
+
+
 @pytest.mark.parametrize("dist", [3.0e4, 1.0e5])
 def test_march_rays_far_origins(dev, dist):
     """Origins far outside the box: at t ~ 3e4 a step of 2*sqrt(3)/1024 rounds to one
