@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(kBlock)
 // (R rows of F floats fill 128 KiB of LDS):
 //   1. count:   work item = (level, 1024-point chunk); LDS histogram of the item's
 //               contributions per slice, stored slice-major;
-//   2. scan:    exclusive scan of the counts -> each item's write offset per slice;
+//   2. scan:    exclusive scan of the counts -> each item's write offset per slice
+//               (per-block sums, then each block adds the sums before its own);
 //   3. scatter: the item recomputes its contributions (with dL_dencodings) and
 //               writes (row within slice, values) entries at its offsets;
 //   4. reduce:  workgroup s sums slice s's contiguous entries with LDS atomics and
@@ -357,34 +358,36 @@ __global__ void __launch_bounds__(kScanThreads) scan_sums_kernel(uint64_t M, con
     }
 }
 
-// 2b. one workgroup: exclusive scan of the block sums, total entries into meta[0]
-__global__ void __launch_bounds__(kScanThreads) scan_block_sums_kernel(uint64_t n, uint32_t *__restrict__ sums,
-                                                                       uint32_t *__restrict__ meta) {
-    uint32_t carry = 0;
-    for (uint64_t c = 0; c < n; c += kScanThreads) {
-        const uint64_t i = c + threadIdx.x;
-        const uint32_t v = i < n ? sums[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<kScanThreads>(v, &tot);
-        if (i < n) sums[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) meta[0] = carry;
-}
-
-// 2c. exclusive scan inside each block plus the block's prefix, in place
+// 2b. exclusive scan inside each block plus the block's prefix (the sum of the
+// block sums before it, added by the workgroup itself), in place; the last block
+// writes the total entries into meta[0]
 __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(uint64_t M, uint32_t *__restrict__ a,
-                                                                  const uint32_t *__restrict__ sums) {
+                                                                  const uint32_t *__restrict__ sums,
+                                                                  uint32_t *__restrict__ meta) {
+    __shared__ uint32_t ws[kScanThreads / kWave];
+    __shared__ uint32_t prefix;
+    uint32_t v = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kScanThreads) v += sums[j];
+    v = wave_sum(v);
+    if (lane_id() == 0) ws[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) t += ws[w];
+        prefix = t;
+    }
+    __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kScanThreads * kScanPer;
-    uint32_t carry = sums[blockIdx.x];
+    uint32_t carry = prefix;
     for (int q = 0; q < kScanPer; ++q) {
         const uint64_t i = base + (uint64_t)q * kScanThreads + threadIdx.x;
-        const uint32_t v = i < M ? a[i] : 0u;
+        const uint32_t x = i < M ? a[i] : 0u;
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<kScanThreads>(v, &tot);
+        const uint32_t ex = block_exclusive_scan<kScanThreads>(x, &tot);
         if (i < M) a[i] = carry + ex;
         carry += tot;
     }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) meta[0] = carry;
 }
 
 __device__ __forceinline__ uint32_t slice_begin(const BwdPlan &p, const uint32_t *offsets, const uint32_t *meta,
@@ -584,9 +587,8 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
     hipLaunchKernelGGL(bwd_count_kernel<F>, dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, counts);
     hipLaunchKernelGGL(scan_sums_kernel, dim3((uint32_t)p.scan_blocks), dim3(kScanThreads), 0, stream, p.M, counts,
                        sums);
-    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kScanThreads), 0, stream, p.scan_blocks, sums, meta);
     hipLaunchKernelGGL(scan_apply_kernel, dim3((uint32_t)p.scan_blocks), dim3(kScanThreads), 0, stream, p.M, counts,
-                       sums);
+                       sums, meta);
     hipLaunchKernelGGL(plan_parts_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, counts, meta, parts,
                        part_slice);
     hipLaunchKernelGGL((bwd_scatter_kernel<F, kHalf>), dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, g, counts,
