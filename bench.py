@@ -176,7 +176,8 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
     2^18 slots, median over the timed frames after three warm-up renders; PSNR of the
     u8 render against the float64 ground truth blended on white.  With `proposal`,
     the frames are rendered with 48 proposal-placed samples per ray (occupancy-grid
-    fallback below predicted mass 0.98) and the same field is also rendered the plain
+    fallback below predicted mass 0.98, on its own pool of 2^16 slots, the
+    renderer's default) and the same field is also rendered the plain
     way, for the PSNR difference; the same comparison at the fallback threshold 0.9
     is reported beside it."""
     import torch
