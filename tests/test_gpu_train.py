@@ -240,3 +240,28 @@ def test_background_model_training(tmp_path):
     assert not torch.equal(tr2.bg_model.params, tr.bg_model.params)
     restore_trainer(tr2, path)
     assert torch.equal(tr2.bg_model.params, tr.bg_model.params)
+
+
+def test_overlapped_update_matches_in_order(small_data):
+    """The optimizer update on the side stream (overlapping the next step's ray
+    generation and march) waits for the backward and is waited for by the next
+    step's encoder.  From the same seeds: the MLP after one step is bit-identical
+    to the in-order update's (its gradient and update are fixed-order), and the
+    second step's MLP gradient agrees up to the table gradient's rounding (its f64
+    sums take their terms in any order).  An update started before the backward
+    finished, or a second step that read the table before the update landed (the
+    table moves from its 1e-4 init by lr = 1e-2 per step), would change them
+    grossly.  (Longer runs are no sharper test: Adam with eps = 1e-15 moves a
+    gradient element near zero by a full +-lr whichever rounding it got.)"""
+    res = {}
+    for ov in (True, False):
+        tr = Trainer(small_data, TrainConfig(total_samples=1 << 16, overlap_update=ov))
+        tr.step()
+        m1 = tr.model.mlp_params.detach().clone()
+        tr.step()
+        torch.cuda.synchronize()
+        res[ov] = (m1, tr.b["dparams"].clone())
+    assert torch.equal(res[True][0], res[False][0])
+    g1, g0 = res[True][1], res[False][1]
+    assert float(g0.abs().max()) > 0
+    assert float((g1 - g0).abs().max()) <= 1e-3 * float(g0.abs().max())
