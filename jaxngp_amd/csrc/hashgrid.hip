@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(kBlock)
 constexpr int kChunk = 1024;         // points per work item
 constexpr int kItemThreads = 256;    // 4 points per thread per item
 constexpr int kReduceThreads = 1024;
-constexpr uint32_t kPart = 32768;    // entries one reduce workgroup sums
+constexpr uint32_t kPart = 65536;    // entries one reduce workgroup sums
 constexpr uint32_t kMaxSlices = 8192;
 constexpr uint32_t kSliceBytes = 131072;
 constexpr int kScanThreads = 1024;
