@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kBlock)
 // addresses, two orders of magnitude below the gather rate, so contributions are
 // routed to LDS by a deterministic counting sort over row slices of R rows
 // (R rows of F floats fill 128 KiB of LDS):
-//   1. count:   work item = (level, 1024-point chunk); LDS histogram of the item's
+//   1. count:   work item = (level, 2048-point chunk); LDS histogram of the item's
 //               contributions per slice, stored slice-major;
 //   2. scan:    exclusive scan of the counts -> each item's write offset per slice
 //               (per-block sums, then each block adds the sums before its own);
@@ -145,8 +145,8 @@ __global__ void __launch_bounds__(kBlock)
 // Within an item, lanes are 64 consecutive samples of one level; runs of equal rows
 // (coarse levels: consecutive samples of a ray share cells) are summed with a
 // segmented scan first, so one entry carries a run.
-constexpr int kChunk = 1024;         // points per work item
-constexpr int kItemThreads = 256;    // 4 points per thread per item
+constexpr int kChunk = 2048;         // points per work item
+constexpr int kItemThreads = 256;    // 8 points per thread per item
 constexpr int kReduceThreads = 1024;
 constexpr uint32_t kPart = 65536;    // entries one reduce workgroup sums
 constexpr uint32_t kMaxSlices = 8192;
@@ -265,60 +265,66 @@ __device__ __forceinline__ void item_contributions(const HashGridDescriptor &d, 
     const uint32_t res = d.resolutions[l];
     const bool dense = l < d.first_hash_level;
     constexpr int kSubs = kChunk / kItemThreads;
-    // every input of the item is loaded before any is used, so the loads overlap
-    float xyz[kSubs][3], gin[kSubs][F];
+    // the inputs of a group of sub-chunks are all loaded before any is used, so
+    // the loads overlap; 8 features per level hold half as many in flight (all
+    // eight sub-chunks of them would not fit the registers)
+    constexpr int kGroup = F >= 8 ? kSubs / 2 : kSubs;
+    for (int g0 = 0; g0 < kSubs; g0 += kGroup) {
+        float xyz[kGroup][3], gin[kGroup][F];
 #pragma unroll
-    for (int sub = 0; sub < kSubs; ++sub) {
-        const uint32_t n = k * kChunk + sub * kItemThreads + threadIdx.x;
-        const uint32_t np = n < d.n_points ? n : d.n_points - 1;
+        for (int sub = 0; sub < kGroup; ++sub) {
+            const uint32_t n = k * kChunk + (g0 + sub) * kItemThreads + threadIdx.x;
+            const uint32_t np = n < d.n_points ? n : d.n_points - 1;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) xyz[sub][a] = pos[3 * (size_t)np + a];
-        if (kValues) {
-            const size_t gb = level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
+            for (int a = 0; a < 3; ++a) xyz[sub][a] = pos[3 * (size_t)np + a];
+            if (kValues) {
+                const size_t gb =
+                    level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
 #pragma unroll
-            for (int f = 0; f < F; ++f) gin[sub][f] = dL_denc[gb + f];
-        }
-    }
-#pragma unroll
-    for (int sub = 0; sub < kSubs; ++sub) {
-        const uint32_t n = k * kChunk + sub * kItemThreads + threadIdx.x;
-        const bool active = n < d.n_points;
-        const float px = (xyz[sub][0] + d.bound) / two_b;
-        const float py = (xyz[sub][1] + d.bound) / two_b;
-        const float pz = (xyz[sub][2] + d.bound) / two_b;
-        uint32_t idx[8];
-        float w[8];
-        level_vertices(px, py, pz, scale, res, dense, d.T, pow2, d.offsets[l], d.n_rows - 1u, idx, w);
-        float g[F];
-#pragma unroll
-        for (int f = 0; f < F; ++f) g[f] = kValues && active ? gin[sub][f] : 0.0f;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            // inactive lanes get keys no active lane has, so they form no runs
-            const uint32_t key = active ? idx[c] : 0xffffffffu - (uint32_t)lane;
-            float v[F];
-#pragma unroll
-            for (int f = 0; f < F; ++f) v[f] = w[c] * g[f];
-            const uint32_t prev = dpp_u32<kWaveShr1, 0xf>(key, ~key);  // lane 0: no predecessor
-            const bool head = prev != key;
-            bool tail = true;
-            if (__ballot(!head) != 0) {
-                if (kValues) {
-                    // segmented inclusive sums of runs: (v, flag) pairs combined by
-                    // (a, fa) + (b, fb) = (fb ? b : a + b, fa | fb), shifts within
-                    // 16-lane rows, then row 15 -> rows 1 and 3, lane 31 -> rows 2 and 3
-                    uint32_t fl = head ? 1u : 0u;
-                    seg_step<kRowShr1, 0xf, F>(v, fl);
-                    seg_step<kRowShr2, 0xf, F>(v, fl);
-                    seg_step<kRowShr4, 0xf, F>(v, fl);
-                    seg_step<kRowShr8, 0xf, F>(v, fl);
-                    seg_step<kRowBcast15, 0xa, F>(v, fl);
-                    seg_step<kRowBcast31, 0xc, F>(v, fl);
-                }
-                const uint32_t next_head = dpp_u32<kWaveShl1, 0xf>(head ? 1u : 0u, 1u);  // lane 63: end of wave
-                tail = next_head != 0u;
+                for (int f = 0; f < F; ++f) gin[sub][f] = dL_denc[gb + f];
             }
-            if (tail && active) emit(key, v);
+        }
+#pragma unroll
+        for (int sub = 0; sub < kGroup; ++sub) {
+            const uint32_t n = k * kChunk + (g0 + sub) * kItemThreads + threadIdx.x;
+            const bool active = n < d.n_points;
+            const float px = (xyz[sub][0] + d.bound) / two_b;
+            const float py = (xyz[sub][1] + d.bound) / two_b;
+            const float pz = (xyz[sub][2] + d.bound) / two_b;
+            uint32_t idx[8];
+            float w[8];
+            level_vertices(px, py, pz, scale, res, dense, d.T, pow2, d.offsets[l], d.n_rows - 1u, idx, w);
+            float g[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) g[f] = kValues && active ? gin[sub][f] : 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                // inactive lanes get keys no active lane has, so they form no runs
+                const uint32_t key = active ? idx[c] : 0xffffffffu - (uint32_t)lane;
+                float v[F];
+#pragma unroll
+                for (int f = 0; f < F; ++f) v[f] = w[c] * g[f];
+                const uint32_t prev = dpp_u32<kWaveShr1, 0xf>(key, ~key);  // lane 0: no predecessor
+                const bool head = prev != key;
+                bool tail = true;
+                if (__ballot(!head) != 0) {
+                    if (kValues) {
+                        // segmented inclusive sums of runs: (v, flag) pairs combined by
+                        // (a, fa) + (b, fb) = (fb ? b : a + b, fa | fb), shifts within
+                        // 16-lane rows, then row 15 -> rows 1 and 3, lane 31 -> rows 2 and 3
+                        uint32_t fl = head ? 1u : 0u;
+                        seg_step<kRowShr1, 0xf, F>(v, fl);
+                        seg_step<kRowShr2, 0xf, F>(v, fl);
+                        seg_step<kRowShr4, 0xf, F>(v, fl);
+                        seg_step<kRowShr8, 0xf, F>(v, fl);
+                        seg_step<kRowBcast15, 0xa, F>(v, fl);
+                        seg_step<kRowBcast31, 0xc, F>(v, fl);
+                    }
+                    const uint32_t next_head = dpp_u32<kWaveShl1, 0xf>(head ? 1u : 0u, 1u);  // lane 63: end of wave
+                    tail = next_head != 0u;
+                }
+                if (tail && active) emit(key, v);
+            }
         }
     }
 }

@@ -59,6 +59,7 @@ CONFIGS = {
     "proposal": dict(L=5, T=2 ** 17, F=2, N_min=16, N_max=512),
     "small_f4": dict(L=8, T=2 ** 12, F=4, N_min=4, N_max=64),
     "f1_dense_only": dict(L=3, T=2 ** 20, F=1, N_min=8, N_max=32),
+    "f8": dict(L=4, T=2 ** 14, F=8, N_min=16, N_max=256),
 }
 
 
