@@ -55,7 +55,7 @@ def _c_backward(enc_mod, pos, g, bound=1.0):
 
 
 CONFIGS = {
-    "c0": dict(L=16, T=2 ** 19, F=2, N_min=16, N_max=2048),
+    "l16_t2e19": dict(L=16, T=2 ** 19, F=2, N_min=16, N_max=2048),
     "proposal": dict(L=5, T=2 ** 17, F=2, N_min=16, N_max=512),
     "small_f4": dict(L=8, T=2 ** 12, F=4, N_min=4, N_max=64),
     "f1_dense_only": dict(L=3, T=2 ** 20, F=1, N_min=8, N_max=32),
@@ -90,10 +90,10 @@ def test_hashgrid_forward_backward(dev, name):
     assert (np.abs(dt - want) <= 1e-5 * absw + 1e-30).all()
 
 
-def test_hashgrid_c0_full_size(dev):
+def test_hashgrid_encoder_workload_full_size(dev):
     """Encoder workload: N = 2^18 points, L=16, F=2, T=2^19, table U(-1e-4, 1e-4)."""
     rng = np.random.default_rng(0)
-    enc = HashGridEncoder(**CONFIGS["c0"], generator=torch.Generator().manual_seed(1)).to(dev)
+    enc = HashGridEncoder(**CONFIGS["l16_t2e19"], generator=torch.Generator().manual_seed(1)).to(dev)
     assert enc.levels.n_rows == 6098120 and enc.levels.first_hash_level == 5
     N = 2 ** 18
     xyz = rng.uniform(0, 1, (N, 3)).astype(np.float32)
@@ -127,7 +127,7 @@ def test_level_major_layout_matches_point_major(dev):
     encodings transposed, and the same table gradient from the transposed cotangent."""
     from jaxngp_amd import _lib
     rng = np.random.default_rng(9)
-    enc = HashGridEncoder(**CONFIGS["c0"], generator=torch.Generator().manual_seed(1)).to(dev)
+    enc = HashGridEncoder(**CONFIGS["l16_t2e19"], generator=torch.Generator().manual_seed(1)).to(dev)
     with torch.no_grad():
         enc.table.uniform_(-1, 1)
     N = 30001
@@ -170,7 +170,7 @@ def test_backward_scale_free(dev):
     cotangents scaled by 2^-60 or 2^+60 give the gradient scaled by the same power
     of two, within the f32 tolerance (contributions stay in the f32 normal range)."""
     rng = np.random.default_rng(21)
-    enc = HashGridEncoder(**CONFIGS["c0"], generator=torch.Generator().manual_seed(1)).to(dev)
+    enc = HashGridEncoder(**CONFIGS["l16_t2e19"], generator=torch.Generator().manual_seed(1)).to(dev)
     N = 50000
     pos = rng.uniform(-1, 1, (N, 3)).astype(np.float32)
     g = rng.normal(size=(N, 32)).astype(np.float32)
@@ -217,7 +217,7 @@ def test_fp16_table_backward():
     from jaxngp_amd import _lib
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(31)
-    enc = HashGridEncoder(**CONFIGS["c0"], param_dtype=torch.float16, generator=torch.Generator().manual_seed(4)).to(dev)
+    enc = HashGridEncoder(**CONFIGS["l16_t2e19"], param_dtype=torch.float16, generator=torch.Generator().manual_seed(4)).to(dev)
     N = 30000
     pos = rng.uniform(-1, 1, (N, 3)).astype(np.float32)
     g = rng.normal(size=(N, 32)).astype(np.float32)
