@@ -300,20 +300,22 @@ def main() -> None:
     assert set(algo) == set(ROOFLINE_CANDIDATES)
     bound, work = algo[dom]
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-    # workload (copy-calibrated), committed beside the kernel-time summary
-    from jaxngp_amd.profiling import read_traffic
-    traffic = read_traffic(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                                        "training_step_fp32_hbm_traffic.txt")).get(dom)
+    # workload (copy-calibrated), and the L2 hit rate from TCC_HIT_sum / TCC_MISS_sum,
+    # committed beside the kernel-time summary
+    from jaxngp_amd.profiling import read_l2_hit_rate, read_traffic
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    traffic = read_traffic(os.path.join(prof, "training_step_fp32_hbm_traffic.txt")).get(dom)
+    l2_hit_rate = read_l2_hit_rate(os.path.join(prof, "training_step_fp32_l2_hit_rate.txt")).get(dom)
     if bound == "hbm":
         achieved = work / (dom_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                "algorithmic_bytes_per_launch": work, "ms_per_launch": dom_ms}
+                "algorithmic_bytes_per_launch": work, "ms_per_launch": dom_ms, "l2_hit_rate": l2_hit_rate}
     else:
         achieved = work / (dom_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_MFMA_PEAK_TFLOPS, "traffic": traffic, "kernel": dom,
-                "algorithmic_flops_per_launch": work, "ms_per_launch": dom_ms}
+                "algorithmic_flops_per_launch": work, "ms_per_launch": dom_ms, "l2_hit_rate": l2_hit_rate}
 
     out = {
         "metric": "train samples/s & ms per 800² frame: proposal+FP16 vs plain, ΔPSNR≤0.1dB",

@@ -2,6 +2,7 @@
 
     python -m jaxngp_amd.profiling stats  KERNEL_STATS_CSV [TOP]
     python -m jaxngp_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES
+    python -m jaxngp_amd.profiling l2 HIT_MISS_DIR
 
 `stats` prints the kernels of a `--kernel-trace --stats` run by total time.
 `traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of a workload, and
@@ -9,6 +10,8 @@ the same two counters read on a plain device copy of COPY_BYTES bytes, into HBM
 bytes per launch of each target: every counter is scaled so that the copy reads and
 writes exactly COPY_BYTES (the factor is printed), summed over the target's kernels
 and divided by the target's launches.
+`l2` turns one `--pmc TCC_HIT_sum TCC_MISS_sum` run into each target's L2 hit rate,
+hits / (hits + misses) summed over the target's kernels.
 """
 from __future__ import annotations
 
@@ -19,12 +22,12 @@ import sys
 
 # the kernels each target launches, and one that runs exactly once per launch
 TARGET_KERNELS = {
-    "march_rays": (["march_count_kernel", "march_accumulate_kernel", "march_scan_kernel", "march_emit_kernel",
-                    "march_finish_kernel", "march_empty_kernel"], "march_scan_kernel"),
+    "march_rays": (["march_count_kernel", "march_scan_kernel", "march_emit_kernel", "march_finish_kernel",
+                    "march_empty_kernel"], "march_scan_kernel"),
     "hashgrid_encode": (["hashgrid_fwd_lm_kernel", "hashgrid_fwd_kernel"], None),
-    "hashgrid_encode_backward": (["bwd_count_kernel", "scan_sums_kernel", "scan_block_sums_kernel",
-                                  "scan_apply_kernel", "plan_parts_kernel", "bwd_scatter_kernel", "bwd_reduce_kernel",
-                                  "bwd_combine_kernel", "zero_table_kernel"], "plan_parts_kernel"),
+    "hashgrid_encode_backward": (["bwd_count_kernel", "scan_sums_kernel", "scan_apply_kernel", "plan_parts_kernel",
+                                  "bwd_scatter_kernel", "bwd_reduce_kernel", "bwd_combine_kernel",
+                                  "zero_table_kernel"], "plan_parts_kernel"),
     "nerf_mlp_forward": (["mlp_forward_kernel", "mlp_forward_h_kernel"], None),
     "nerf_mlp_backward": (["mlp_backward_kernel", "mlp_backward_h_kernel", "sum_rows_kernel"], "sum_rows_kernel"),
     "integrate_rays": (["integrate_kernel"], "integrate_kernel"),
@@ -52,11 +55,14 @@ def stats(path: str, top: int = 40) -> str:
     return "\n".join(out)
 
 
-def _counter(d: str) -> dict:
-    """kernel base name -> [dispatches, summed counter value (KiB)]"""
+def _counter(d: str, name: str | None = None) -> dict:
+    """kernel base name -> [dispatches, summed counter value], over the rows of
+    counter `name` (all rows when None: a run of one counter)"""
     agg = collections.defaultdict(lambda: [0, 0.0])
     for f in glob.glob(d + "/**/*counter_collection*.csv", recursive=True):
         for r in csv.DictReader(open(f)):
+            if name is not None and r["Counter_Name"] != name:
+                continue
             a = agg[_base(r["Kernel_Name"])]
             a[0] += 1
             a[1] += float(r["Counter_Value"] or 0)
@@ -86,17 +92,46 @@ def traffic(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes) ->
     return "\n".join(out)
 
 
-def read_traffic(path: str) -> dict:
-    """{target: HBM bytes per launch} from a summary written by `traffic`."""
+def l2(hit_miss_dir: str) -> str:
+    h, m = _counter(hit_miss_dir, "TCC_HIT_sum"), _counter(hit_miss_dir, "TCC_MISS_sum")
+    out = [f"{'target':26s} {'hits_M':>10s} {'misses_M':>10s} {'hit_rate':>9s}  kernels"]
+    for tgt, (kernels, _) in TARGET_KERNELS.items():
+        present = [k for k in kernels if k in h and k in m]
+        if not present:
+            continue
+        hs, ms = sum(h[k][1] for k in present), sum(m[k][1] for k in present)
+        rate = hs / (hs + ms) if hs + ms else float("nan")
+        out.append(f"{tgt:26s} {hs / 1e6:10.2f} {ms / 1e6:10.2f} {rate:9.3f}  {','.join(present)}")
+        out.append(f"l2_hit_rate {tgt} {rate:.4f}")
+    out.append("")
+    out.append(f"{'kernel':40s} {'hit_rate':>9s}")
+    for k in sorted(set(h) & set(m)):
+        if any(k in ks for ks, _ in TARGET_KERNELS.values()):
+            tot = h[k][1] + m[k][1]
+            out.append(f"{k:40s} {h[k][1] / tot if tot else float('nan'):9.3f}")
+    return "\n".join(out)
+
+
+def _read_keyed(path: str, key: str) -> dict:
     res = {}
     try:
         for line in open(path):
             p = line.split()
-            if len(p) == 3 and p[0] == "traffic_bytes_per_launch":
+            if len(p) == 3 and p[0] == key:
                 res[p[1]] = float(p[2])
     except OSError:
         pass
     return res
+
+
+def read_traffic(path: str) -> dict:
+    """{target: HBM bytes per launch} from a summary written by `traffic`."""
+    return _read_keyed(path, "traffic_bytes_per_launch")
+
+
+def read_l2_hit_rate(path: str) -> dict:
+    """{target: L2 hit rate} from a summary written by `l2`."""
+    return _read_keyed(path, "l2_hit_rate")
 
 
 if __name__ == "__main__":
@@ -104,5 +139,7 @@ if __name__ == "__main__":
         print(stats(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 40))
     elif sys.argv[1] == "traffic":
         print(traffic(*sys.argv[2:6], float(sys.argv[6])))
+    elif sys.argv[1] == "l2":
+        print(l2(sys.argv[2]))
     else:
         raise SystemExit(__doc__)

@@ -1,0 +1,52 @@
+"""The rocprofv3 summaries bench.py and the committed profiles rely on, on
+synthetic counter files in rocprofv3's CSV layout (one row per dispatch and
+counter)."""
+import csv
+
+import pytest
+
+from jaxngp_amd import profiling
+
+FIELDS = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+
+
+def _write(d, rows):
+    d.mkdir(parents=True, exist_ok=True)
+    with open(d / "run_counter_collection.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, FIELDS)
+        w.writeheader()
+        for i, (k, c, v) in enumerate(rows):
+            w.writerow({"Dispatch_Id": i, "Kernel_Name": k, "Counter_Name": c, "Counter_Value": v})
+
+
+def test_traffic_scales_by_the_copy_and_divides_by_launches(tmp_path):
+    k = "void ngp::(anonymous namespace)::integrate_kernel(ngp::Desc, float const*)"
+    _write(tmp_path / "f", [(k, "FETCH_SIZE", 100.0), (k, "FETCH_SIZE", 300.0)])
+    _write(tmp_path / "w", [(k, "WRITE_SIZE", 50.0), (k, "WRITE_SIZE", 50.0)])
+    copy = "__amd_rocclr_copyBuffer"
+    _write(tmp_path / "cf", [(copy, "FETCH_SIZE", 512.0)])  # half of the 1 MiB the copy reads
+    _write(tmp_path / "cw", [(copy, "WRITE_SIZE", 1024.0)])
+    out = profiling.traffic(*(str(tmp_path / n) for n in ("f", "w", "cf", "cw")), 1048576.0)
+    (tmp_path / "t.txt").write_text(out)
+    # per launch: fetch (400 / 2) KiB x 2, write (100 / 2) KiB x 1
+    assert profiling.read_traffic(str(tmp_path / "t.txt")) == {"integrate_rays": (400 + 50) * 1024.0}
+
+
+def test_l2_hit_rate_per_target_and_kernel(tmp_path):
+    a = "void ngp::(anonymous namespace)::bwd_scatter_kernel<2, false>(ngp::HashGridDescriptor)"
+    b = "void ngp::(anonymous namespace)::bwd_reduce_kernel<2, false>(ngp::HashGridDescriptor)"
+    _write(tmp_path / "hm", [(a, "TCC_HIT_sum", 30.0), (a, "TCC_MISS_sum", 10.0),
+                             (b, "TCC_HIT_sum", 10.0), (b, "TCC_MISS_sum", 50.0),
+                             ("other_kernel", "TCC_HIT_sum", 1.0), ("other_kernel", "TCC_MISS_sum", 0.0)])
+    out = profiling.l2(str(tmp_path / "hm"))
+    rates = {p[1]: float(p[2]) for p in (line.split() for line in out.splitlines()) if p and p[0] == "l2_hit_rate"}
+    assert rates == {"hashgrid_encode_backward": pytest.approx(40 / 100)}
+    (tmp_path / "l2.txt").write_text(out)
+    assert profiling.read_l2_hit_rate(str(tmp_path / "l2.txt")) == {"hashgrid_encode_backward": 0.4}
+    per_kernel = dict(line.split() for line in out.splitlines()[-2:])
+    assert float(per_kernel["bwd_scatter_kernel"]) == pytest.approx(0.75)
+    assert float(per_kernel["bwd_reduce_kernel"]) == pytest.approx(1 / 6, abs=1e-3)
+
+
+def test_read_traffic_of_a_missing_file_is_empty(tmp_path):
+    assert profiling.read_traffic(str(tmp_path / "none.txt")) == {}
