@@ -40,14 +40,19 @@ constexpr int kTile = 16;
 constexpr int P_W1 = 0, P_W2 = 2048, P_W3 = 3072, P_W4 = 5120, P_W5 = 9216, P_END = 9408;
 static_assert(P_END == NGP_MLP_N_PARAMS, "parameter count");
 
-// LDS copies: W[in][out_padded + 1] (odd row stride)
-constexpr int S_W1 = 65, S_W2 = 17, S_W3 = 65, S_W4 = 65, S_W5 = 17;
+// LDS copies: W[in][out_padded + 4].  A row stride of 4 mod 16 puts the four lane
+// groups g of an MFMA operand read (rows 4g + r, columns i) 16 banks apart, so a
+// forward layer's weight reads are free of bank conflicts (an odd stride put them
+// 4 banks apart, four lanes on each bank).
+constexpr int S_W1 = 68, S_W2 = 20, S_W3 = 68, S_W4 = 68, S_W5 = 20;
 constexpr int L_W1 = 0;
 constexpr int L_W2 = L_W1 + 32 * S_W1;
 constexpr int L_W3 = L_W2 + 64 * S_W2;
 constexpr int L_W4 = L_W3 + 32 * S_W3;
 constexpr int L_W5 = L_W4 + 64 * S_W4;
-constexpr int L_END = L_W5 + 64 * S_W5;  // 10496 floats
+constexpr int L_END = L_W5 + 64 * S_W5;  // 11264 floats
+static_assert(S_W1 % 16 == 4 && S_W2 % 16 == 4 && S_W3 % 16 == 4 && S_W4 % 16 == 4 && S_W5 % 16 == 4,
+              "weight row strides of 4 mod 16");
 
 // per-wave transpose scratch of the backward pass: [16 samples][68]
 constexpr int S_T = 68;

@@ -3,6 +3,7 @@
     python -m jaxngp_amd.profiling stats  KERNEL_STATS_CSV [TOP]
     python -m jaxngp_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES
     python -m jaxngp_amd.profiling l2 HIT_MISS_DIR
+    python -m jaxngp_amd.profiling counters PMC_DIR
 
 `stats` prints the kernels of a `--kernel-trace --stats` run by total time.
 `traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of a workload, and
@@ -12,6 +13,8 @@ writes exactly COPY_BYTES (the factor is printed), summed over the target's kern
 and divided by the target's launches.
 `l2` turns one `--pmc TCC_HIT_sum TCC_MISS_sum` run into each target's L2 hit rate,
 hits / (hits + misses) summed over the target's kernels.
+`counters` prints, for each kernel of the targets, every counter of a `--pmc` run
+averaged per dispatch.
 """
 from __future__ import annotations
 
@@ -33,6 +36,10 @@ TARGET_KERNELS = {
     "integrate_rays": (["integrate_kernel"], "integrate_kernel"),
     "integrate_rays_backward": (["integrate_backward_kernel"], "integrate_backward_kernel"),
 }
+
+
+# kernels outside the targets above whose counters are also of interest
+EXTRA_KERNELS = ("mlp_forward_h_kernel", "mlp_backward_h_kernel", "field_chunks_kernel", "proposal_render_kernel")
 
 
 def _short(name: str) -> str:
@@ -124,6 +131,23 @@ def _read_keyed(path: str, key: str) -> dict:
     return res
 
 
+def counters(pmc_dir: str) -> str:
+    rows = collections.defaultdict(lambda: collections.defaultdict(lambda: [0, 0.0]))
+    for f in glob.glob(pmc_dir + "/**/*counter_collection*.csv", recursive=True):
+        for r in csv.DictReader(open(f)):
+            a = rows[_base(r["Kernel_Name"])][r["Counter_Name"]]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"] or 0)
+    ours = {k for ks, _ in TARGET_KERNELS.values() for k in ks} | set(EXTRA_KERNELS)
+    names = sorted({c for k in rows if k in ours for c in rows[k]})
+    out = [f"{'kernel (average per dispatch)':32s} {'dispatches':>10s} " + " ".join(f"{c:>26s}" for c in names)]
+    for k in sorted(k for k in rows if k in ours):
+        n = max(v[0] for v in rows[k].values())
+        out.append(f"{k:32s} {n:10d} " + " ".join(
+            f"{rows[k][c][1] / rows[k][c][0]:26.4g}" if rows[k][c][0] else f"{'-':>26s}" for c in names))
+    return "\n".join(out)
+
+
 def read_traffic(path: str) -> dict:
     """{target: HBM bytes per launch} from a summary written by `traffic`."""
     return _read_keyed(path, "traffic_bytes_per_launch")
@@ -141,5 +165,7 @@ if __name__ == "__main__":
         print(traffic(*sys.argv[2:6], float(sys.argv[6])))
     elif sys.argv[1] == "l2":
         print(l2(sys.argv[2]))
+    elif sys.argv[1] == "counters":
+        print(counters(sys.argv[2]))
     else:
         raise SystemExit(__doc__)

@@ -50,3 +50,14 @@ def test_l2_hit_rate_per_target_and_kernel(tmp_path):
 
 def test_read_traffic_of_a_missing_file_is_empty(tmp_path):
     assert profiling.read_traffic(str(tmp_path / "none.txt")) == {}
+
+
+def test_counters_average_per_dispatch(tmp_path):
+    k = "void ngp::(anonymous namespace)::mlp_backward_kernel(ngp::MlpDescriptor)"
+    _write(tmp_path / "c", [(k, "SQ_BUSY_CYCLES", 100.0), (k, "SQ_VALU_MFMA_BUSY_CYCLES", 30.0),
+                            (k, "SQ_BUSY_CYCLES", 300.0), (k, "SQ_VALU_MFMA_BUSY_CYCLES", 50.0),
+                            ("at::native::foo", "SQ_BUSY_CYCLES", 7.0)])
+    lines = profiling.counters(str(tmp_path / "c")).splitlines()
+    assert lines[0].split()[-2:] == ["SQ_BUSY_CYCLES", "SQ_VALU_MFMA_BUSY_CYCLES"]
+    assert lines[1].split() == ["mlp_backward_kernel", "2", "200", "40"]
+    assert len(lines) == 2  # kernels outside the library are left out
