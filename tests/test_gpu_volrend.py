@@ -234,7 +234,7 @@ def test_integrate_rays_near_plane_and_broadcast_bg(dev):
 
 
 def test_integrate_rays_2pow20_samples_properties(dev):
-    """Config-2 size: 1024 rays x 1024 samples; opacity in [0,1], measured <= S, and
+    """Integrator workload at 2^20 samples (1024 rays x 1024 samples): opacity in [0,1], measured <= S, and
     the f32 result within float64 error on a subset."""
     rng = np.random.default_rng(8)
     n_rays, m = 1024, 1024
