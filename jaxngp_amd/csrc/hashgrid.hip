@@ -72,7 +72,8 @@ __global__ void __launch_bounds__(kBlock)
 // stay resident in that XCD's L2 for the gathers / atomics, instead of every L2
 // streaming the whole 46.5 MiB table.
 constexpr uint32_t kXcds = 8;
-constexpr int kPointsPerBlock = kBlock;
+constexpr int kLmPointsPerThread = 2;
+constexpr int kPointsPerBlock = kBlock * kLmPointsPerThread;
 
 struct LevelBlock {
     uint32_t level, point0;
@@ -96,37 +97,49 @@ inline uint32_t level_grid(uint32_t n_points, uint32_t L) {
     return kXcds * n_pg * div_up(L, kXcds);
 }
 
-// Level-major forward: one thread per point of the block's level, output
-// enc[(l * N + n) * F + f] written as one contiguous run per wavefront.
+// Level-major forward: each thread takes kLmPointsPerThread points of the block's
+// level, kBlock apart, so the gathers of both are in flight together; output
+// enc[(l * N + n) * F + f] written as one contiguous run per wavefront.  Points past
+// the end load the last point and store nothing.
 template <int F, typename TT>
 __global__ void __launch_bounds__(kBlock)
     hashgrid_fwd_lm_kernel(HashGridDescriptor d, const float *__restrict__ pos, const TT *__restrict__ table,
                            float *__restrict__ enc) {
+    constexpr int P = kLmPointsPerThread;
     const LevelBlock lb = level_block(d.n_points, d.L);
     if (!lb.live) return;
-    const uint32_t n = lb.point0 + threadIdx.x;
-    if (n >= d.n_points) return;
     const uint32_t l = lb.level;
     const float two_b = 2.0f * d.bound;
-    const float px = (pos[3 * (size_t)n + 0] + d.bound) / two_b;
-    const float py = (pos[3 * (size_t)n + 1] + d.bound) / two_b;
-    const float pz = (pos[3 * (size_t)n + 2] + d.bound) / two_b;
-    uint32_t idx[8];
-    float w[8];
-    level_vertices(px, py, pz, d.scales[l], d.resolutions[l], l < d.first_hash_level, d.T, (d.T & (d.T - 1u)) == 0,
-                   d.offsets[l], d.n_rows - 1u, idx, w);
-    float acc[F];
+    uint32_t idx[P][8];
+    float w[P][8];
 #pragma unroll
-    for (int f = 0; f < F; ++f) acc[f] = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const TT *row = table + (size_t)idx[c] * F;
-#pragma unroll
-        for (int f = 0; f < F; ++f) acc[f] = acc[f] + to_f32<TT>(row[f]) * w[c];
+    for (int j = 0; j < P; ++j) {
+        const uint32_t n = lb.point0 + j * kBlock + threadIdx.x;
+        const uint32_t np = n < d.n_points ? n : d.n_points - 1;
+        const float px = (pos[3 * (size_t)np + 0] + d.bound) / two_b;
+        const float py = (pos[3 * (size_t)np + 1] + d.bound) / two_b;
+        const float pz = (pos[3 * (size_t)np + 2] + d.bound) / two_b;
+        level_vertices(px, py, pz, d.scales[l], d.resolutions[l], l < d.first_hash_level, d.T,
+                       (d.T & (d.T - 1u)) == 0, d.offsets[l], d.n_rows - 1u, idx[j], w[j]);
     }
-    float *out = enc + ((size_t)l * d.n_points + n) * F;
 #pragma unroll
-    for (int f = 0; f < F; ++f) out[f] = acc[f];
+    for (int j = 0; j < P; ++j) {
+        float acc[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) acc[f] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const TT *row = table + (size_t)idx[j][c] * F;
+#pragma unroll
+            for (int f = 0; f < F; ++f) acc[f] = acc[f] + to_f32<TT>(row[f]) * w[j][c];
+        }
+        const uint32_t n = lb.point0 + j * kBlock + threadIdx.x;
+        if (n < d.n_points) {
+            float *out = enc + ((size_t)l * d.n_points + n) * F;
+#pragma unroll
+            for (int f = 0; f < F; ++f) out[f] = acc[f];
+        }
+    }
 }
 
 // Backward without global atomics.  Device-scope atomics on this chip (float or
