@@ -12,7 +12,9 @@
  * bytes of the target's descriptor struct below and must be exactly
  * sizeof(descriptor) long.  Targets never throw: on a bad descriptor, a null
  * buffer or a failed launch they record an error that ngp_get_last_error()
- * returns (and clears) on the calling thread, and launch nothing.
+ * returns (and clears), and launch nothing.  The error state is process-wide, so
+ * a thread other than the one that ran the target (e.g. the host thread of a
+ * framework whose executor threads run the targets) sees it.
  *
  * Work is enqueued asynchronously on `stream` (a hipStream_t; NULL is the legacy
  * default stream).  No target synchronises the stream.
@@ -542,9 +544,9 @@ void adam_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t
  * Status.
  * ------------------------------------------------------------------------- */
 
-/* Returns 0 when no error was recorded on this thread since the last call,
- * otherwise a nonzero code; copies a NUL-terminated message into buf (if buf is
- * non-NULL) and clears the error. */
+/* Returns 0 when no error was recorded (by any thread) since the last call,
+ * otherwise the code of the first error recorded since then; copies its
+ * NUL-terminated message into buf (if buf is non-NULL) and clears the error. */
 int ngp_get_last_error(char *buf, size_t buf_len);
 
 /* Library version string. */

@@ -191,6 +191,22 @@ class Trainer:
         if self._adam_pending:
             self.main_stream.wait_event(self._adam_done)
 
+    def close(self) -> None:
+        """Orders the main stream after the pending optimizer update.  The parameters,
+        gradients and moments it reads and writes were allocated on the main stream, so
+        once the trainer is dropped the caching allocator may hand their blocks to new
+        main-stream tensors; those must not be written while the side stream still
+        works on them.  Called by __del__; safe to call more than once."""
+        if getattr(self, "_adam_pending", False):
+            self.main_stream.wait_event(self._adam_done)
+            self._adam_pending = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     @property
     def model(self) -> NeRF:
         """The field; reading it orders the main stream after the last update."""

@@ -173,3 +173,26 @@ def test_sky_sphere_host_checks():
     with pytest.raises(_lib.NGPError, match="NULL"):
         _lib.call_raw("sky_sphere_bg", [0] * 6, _lib.sky_sphere_bg_descriptor(10, 0, 1, 4.0))
     assert _lib.last_error() == (0, "")
+
+
+def test_error_is_seen_from_another_thread():
+    """A host framework may run targets on its own executor threads (XLA does) while
+    another thread polls ngp_get_last_error: the error state is process-wide."""
+    import threading
+    fn = _lib.target("march_rays")
+    arr = (_lib.ctypes.c_void_p * 16)(*([None] * 16))
+    bad = _lib.marching_descriptor(4, 0, 1024, 1, 128, 1.0, 0.0)  # total_samples = 0
+    t = threading.Thread(target=lambda: fn(_lib.ctypes.c_void_p(0), arr, bad, len(bad)))
+    t.start()
+    t.join()
+    code, msg = _lib.last_error()
+    assert code != 0 and "total_samples" in msg
+    assert _lib.last_error() == (0, "")
+    # the first error recorded since the last poll is the one reported
+    t1 = threading.Thread(target=lambda: fn(_lib.ctypes.c_void_p(0), arr, bad, len(bad)))
+    t1.start()
+    t1.join()
+    _lib.target("integrate_rays")(_lib.ctypes.c_void_p(0), arr, b"\x00", 1)
+    code, msg = _lib.last_error()
+    assert "total_samples" in msg
+    assert _lib.last_error() == (0, "")

@@ -215,6 +215,50 @@ def test_c_encoder_matches_float64(rng):
     np.testing.assert_allclose(enc, want, rtol=0, atol=1e-5 * 1e-2)
 
 
+def _c_table_gradient(lv, T, F, pos, g):
+    import ctypes
+    N, L = pos.shape[0], len(lv["scales_f32"])
+    out = np.zeros((lv["n_rows"], F), np.float64)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    scales = np.ascontiguousarray(lv["scales_f32"], np.float32)
+    res = np.ascontiguousarray(lv["resolutions"], np.uint32)
+    offs = np.ascontiguousarray(lv["offsets"], np.uint32)
+    oracle.lib().oracle_hashgrid_bwd(ctypes.c_uint32(N), ctypes.c_uint32(L), ctypes.c_uint32(F), ctypes.c_uint32(T),
+                                     ctypes.c_uint32(lv["first_hash_level"]), ctypes.c_uint32(lv["n_rows"]),
+                                     ctypes.c_float(1.0), P(scales), P(res), P(offs),
+                                     P(np.ascontiguousarray(pos, np.float32)), P(np.ascontiguousarray(g, np.float32)),
+                                     P(out))
+    return out
+
+
+@pytest.mark.parametrize("cfg", [dict(L=16, T=2 ** 19, F=2, N_min=16, N_max=2048),
+                                 dict(L=4, T=2 ** 14, F=8, N_min=16, N_max=256)], ids=["l16_t2e19", "f8"])
+def test_c_table_gradient_checker_is_the_adjoint(cfg, rng):
+    """The C table gradient (the checker of the GPU table-gradient tests) is the
+    adjoint of the float64 forward, <enc(theta), g> = <theta, dtheta>, for random
+    tables theta, and matches the adjoint-pinned float64 scatter row by row.  A wrong
+    corner, weight, feature or level index in the C scatter fails one of these."""
+    L, T, F = cfg["L"], cfg["T"], cfg["F"]
+    lv = oracle.hashgrid_levels(L, T, cfg["N_min"], cfg["N_max"])
+    N = 3000
+    pos = rng.uniform(-1, 1, (N, 3)).astype(np.float32)
+    pos[:3] = [[-1, -1, -1], [1, 1, 1], [0.999999, -0.999999, 0.5]]
+    g = rng.normal(size=(N, L * F)).astype(np.float32)
+    got = _c_table_gradient(lv, T, F, pos, g)
+    absw = _c_table_gradient(lv, T, F, pos, np.abs(g))
+    # the C weights are f32 products, the float64 scatter's are float64 products of the
+    # same f32 fractions: they agree to a few f32 ulps of each contribution
+    want = oracle.hashgrid_backward64(pos, g, 1.0, lv, T, F)
+    assert (np.abs(got - want) <= 1e-6 * absw + 1e-300).all()
+    assert (got[absw == 0] == 0).all()
+    for seed in range(3):
+        theta = np.random.default_rng(seed).normal(size=(lv["n_rows"], F))
+        lhs = float((oracle.hashgrid_forward64(pos, theta, 1.0, lv, T) * g).sum())
+        rhs = float((theta * got).sum())
+        scale = float((np.abs(theta) * absw).sum())
+        assert abs(lhs - rhs) <= 1e-6 * scale, (lhs, rhs, scale)
+
+
 # ---------------------------------------------------------------------------
 # marcher
 # ---------------------------------------------------------------------------
@@ -632,6 +676,47 @@ def test_sh4_orthonormality():
     Y = oracle.sh4_64(d)
     gram = 4 * np.pi * (Y.T @ Y) / len(d)
     np.testing.assert_allclose(gram, np.eye(16), atol=0.02)
+
+
+def _real_sh_scipy(d):
+    """Real spherical harmonics of degree < 4 from scipy's complex ones (which carry
+    the Condon-Shortley phase): column l^2 + l + m holds sqrt(2) Im Y_l^|m| for m < 0,
+    Y_l^0 for m = 0 and sqrt(2) Re Y_l^m for m > 0.  This is the sign convention of
+    models/encoders.py:384-405 (column 1 = -sqrt(3/4pi) y, column 3 = -sqrt(3/4pi) x)."""
+    from scipy.special import sph_harm_y
+    theta = np.arccos(np.clip(d[:, 2], -1.0, 1.0))  # polar angle from +z
+    phi = np.arctan2(d[:, 1], d[:, 0])  # azimuth from +x
+    out = np.zeros((len(d), 16))
+    for l in range(4):
+        for m in range(-l, l + 1):
+            y = sph_harm_y(l, abs(m), theta, phi)
+            if m < 0:
+                v = math.sqrt(2) * y.imag
+            elif m == 0:
+                v = y.real
+            else:
+                v = math.sqrt(2) * y.real
+            out[:, l * l + l + m] = v
+    return out
+
+
+def test_sh4_matches_scipy_real_harmonics():
+    """All 16 SH(4) terms against scipy's spherical harmonics at random directions and
+    at the six axis directions and the eight cube diagonals: a flipped sign, a swapped
+    pair of terms or a wrong coefficient fails here (orthonormality alone does not
+    see signs or permutations)."""
+    rng = np.random.default_rng(11)
+    d = rng.normal(size=(2000, 3))
+    axes = np.concatenate([np.eye(3), -np.eye(3)])
+    diag = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64)
+    d = np.concatenate([d, axes, diag])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    np.testing.assert_allclose(oracle.sh4_64(d), _real_sh_scipy(d), rtol=0, atol=1e-12)
+    # the reference's own annotations of the first terms, as closed forms
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    c1 = math.sqrt(3) / (2 * math.sqrt(math.pi))
+    Y = oracle.sh4_64(d)
+    np.testing.assert_allclose(Y[:, 1:4], np.stack([-c1 * y, c1 * z, -c1 * x], 1), atol=1e-15)
 
 
 # ---- learned proposal (defined by this build; pinned by calculus and closed forms) ----
