@@ -301,6 +301,20 @@ void nerf_mlp_forward(ngp_stream_t stream, void **buffers, const char *opaque, s
  *      workspace f32[n_workspace_blocks, NGP_MLP_N_PARAMS] */
 void nerf_mlp_backward(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
+/* No counterpart in the reference: nerf_mlp_backward for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same descriptor, buffers and results as nerf_mlp_backward
+ * without the workspace buffer:
+ * in : encodings, dirs, params, dL_ddrgbs;  out: dL_dencodings, dL_dparams.
+ * The workspace is a fixed device array of NGP_MLP_NO_ALLOC_MAX_BLOCKS rows of
+ * NGP_MLP_N_PARAMS floats (18.4 MiB per GPU, held until the process ends), so
+ * n_workspace_blocks must be at most NGP_MLP_NO_ALLOC_MAX_BLOCKS (else the call
+ * records an error and launches nothing).  Nothing is allocated or freed by the
+ * call.  Every call on a GPU shares that GPU's array: calls that overlap on one
+ * GPU (on different streams, or from several threads without ordering) race. */
+#define NGP_MLP_NO_ALLOC_MAX_BLOCKS 512u
+void nerf_mlp_backward_no_alloc(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
 typedef struct NerfFieldChunksDescriptor {
     HashGridDescriptor grid;  /* the NeRF position encoder (L = 16, F = 2); n_points and
                                  enc_layout are not used */

@@ -38,6 +38,7 @@ TARGETS = (
     "sky_sphere_bg_backward",
     "nerf_mlp_forward",
     "nerf_mlp_backward",
+    "nerf_mlp_backward_no_alloc",
     "train_make_rays",
     "train_gt_blend",
     "train_bg_grad",

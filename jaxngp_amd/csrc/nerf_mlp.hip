@@ -24,6 +24,8 @@
 // registers across all tiles a wave visits; waves then add into LDS, each
 // workgroup writes one row of the workspace, and a second kernel sums the rows
 // in a fixed order.
+#include <mutex>
+
 #include "ngp_common.h"
 #include "hashgrid_device.h"
 
@@ -961,37 +963,115 @@ extern "C" void nerf_mlp_forward(ngp_stream_t stream_, void **buffers, const cha
     check_launch(target);
 }
 
-extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const char *opaque, size_t opaque_len) {
-    const char *target = "nerf_mlp_backward";
-    NerfMlpDescriptor d;
-    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+namespace {
+
+// Checks the descriptor and the six data buffers (encodings, dirs, params,
+// dL_ddrgbs, dL_dencodings, dL_dparams) shared by both backward entry points.
+bool check_mlp_backward(const NerfMlpDescriptor &d, void **buffers, const char *target) {
     if (d.n_workspace_blocks == 0 || d.enc_layout > NGP_ENC_LEVEL_MAJOR || d.param_dtype > NGP_DTYPE_F16) {
         set_error(kBadDescriptor, "%s: n_workspace_blocks must be positive, enc_layout and param_dtype 0 or 1",
                   target);
-        return;
+        return false;
     }
-    if (!check_buffer_list(buffers, {2, 5, 6}, target)) return;
-    if (d.n_samples > 0 && !check_buffers(buffers, 7, target)) return;
+    if (!check_buffer_list(buffers, {2, 5}, target)) return false;
+    if (d.n_samples > 0 && !check_buffers(buffers, 6, target)) return false;
     if (d.n_samples > 0 && (!aligned16(buffers[0]) || !aligned16(buffers[3]) || !aligned16(buffers[4]))) {
         set_error(kBadArgument, "%s: encodings, dL_ddrgbs and dL_dencodings must be 16-byte aligned", target);
-        return;
+        return false;
     }
-    hipStream_t stream = (hipStream_t)stream_;
+    return true;
+}
+
+// The backward kernel (one workspace row per workgroup, every element of the row
+// written) and the fixed-order row sum into dL_dparams.
+void launch_mlp_backward(const NerfMlpDescriptor &d, hipStream_t stream, void **buffers, float *workspace,
+                         const char *target) {
     const bool lm = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
     if (d.param_dtype == NGP_DTYPE_F16) {
         hipLaunchKernelGGL(mlp_backward_h_kernel, dim3(d.n_workspace_blocks), dim3(kThreads), 0, stream, d.n_samples,
                            lm, (const float *)buffers[0], (const float *)buffers[1], (const _Float16 *)buffers[2],
-                           (const float *)buffers[3], (float *)buffers[4], (float *)buffers[6]);
+                           (const float *)buffers[3], (float *)buffers[4], workspace);
     } else {
         hipLaunchKernelGGL(mlp_backward_kernel, dim3(d.n_workspace_blocks), dim3(kThreads), 0, stream, d.n_samples, lm,
                            (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
-                           (const float *)buffers[3], (float *)buffers[4], (float *)buffers[6]);
+                           (const float *)buffers[3], (float *)buffers[4], workspace);
     }
     if (!check_launch(target)) return;
     hipLaunchKernelGGL(sum_rows_kernel, dim3(div_up(P_END, kWave)), dim3(kSumWaves * kWave), 0, stream,
-                       d.n_workspace_blocks,
-                       (const float *)buffers[6], (float *)buffers[5]);
+                       d.n_workspace_blocks, (const float *)workspace, (float *)buffers[5]);
     check_launch(target);
+}
+
+// Workspace of nerf_mlp_backward_no_alloc: the weight-gradient rows of up to
+// NGP_MLP_NO_ALLOC_MAX_BLOCKS workgroups (512 x 9408 f32 = 18.4 MiB per GPU, held
+// until the process ends).  Host code must not pass `scratch` itself to a kernel:
+// on the host the name is a stand-in, not the device address.  The address of each
+// device's copy comes from hipGetSymbolAddress, looked up once per device.
+static __device__ __align__(256) float scratch[(size_t)NGP_MLP_NO_ALLOC_MAX_BLOCKS * P_END];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// The address of `scratch` on the device that owns `stream` (the current device for
+// the null stream), looked up with that device current on its first use.
+float *scratch_for(hipStream_t stream, const char *target) {
+    int dev = -1;
+    hipError_t e = stream == nullptr ? hipGetDevice(&dev) : hipStreamGetDevice(stream, &dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error(kBadArgument, "%s: cannot determine the stream's device (%s, device %d)", target,
+                  hipGetErrorString(e), dev);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] != nullptr) return g_scratch_ptr[dev];
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) {
+        set_error(kLaunchFailed, "%s: hipGetDevice failed", target);
+        return nullptr;
+    }
+    if (prev != dev && hipSetDevice(dev) != hipSuccess) {
+        set_error(kLaunchFailed, "%s: hipSetDevice(%d) failed", target, dev);
+        return nullptr;
+    }
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != dev) (void)hipSetDevice(prev);
+    if (e != hipSuccess || ptr == nullptr) {
+        set_error(kLaunchFailed, "%s: hipGetSymbolAddress(scratch) failed on device %d: %s", target, dev,
+                  hipGetErrorString(e));
+        return nullptr;
+    }
+    g_scratch_ptr[dev] = (float *)ptr;
+    return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" void nerf_mlp_backward(ngp_stream_t stream_, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "nerf_mlp_backward";
+    NerfMlpDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (!check_mlp_backward(d, buffers, target)) return;
+    if (!check_buffer_list(buffers, {6}, target)) return;
+    launch_mlp_backward(d, (hipStream_t)stream_, buffers, (float *)buffers[6], target);
+}
+
+extern "C" void nerf_mlp_backward_no_alloc(ngp_stream_t stream_, void **buffers, const char *opaque,
+                                           size_t opaque_len) {
+    const char *target = "nerf_mlp_backward_no_alloc";
+    NerfMlpDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    if (!check_mlp_backward(d, buffers, target)) return;
+    if (d.n_workspace_blocks > NGP_MLP_NO_ALLOC_MAX_BLOCKS) {
+        set_error(kBadDescriptor, "%s: n_workspace_blocks=%u exceeds the fixed scratch of %u workgroup rows", target,
+                  d.n_workspace_blocks, (unsigned)NGP_MLP_NO_ALLOC_MAX_BLOCKS);
+        return;
+    }
+    hipStream_t stream = (hipStream_t)stream_;
+    float *ws = scratch_for(stream, target);
+    if (ws == nullptr) return;
+    launch_mlp_backward(d, stream, buffers, ws, target);
 }
 
 extern "C" void nerf_field_chunks(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {

@@ -175,6 +175,18 @@ def test_sky_sphere_host_checks():
     assert _lib.last_error() == (0, "")
 
 
+def test_no_alloc_mlp_backward_host_checks():
+    """The no-allocation MLP backward refuses more workgroup rows than its fixed
+    scratch holds, and NULL buffers, before touching a device."""
+    with pytest.raises(_lib.NGPError, match="exceeds the fixed scratch"):
+        _lib.call_raw("nerf_mlp_backward_no_alloc", [16] * 6, _lib.nerf_mlp_descriptor(0, 513))
+    with pytest.raises(_lib.NGPError, match="NULL"):
+        _lib.call_raw("nerf_mlp_backward_no_alloc", [0] * 6, _lib.nerf_mlp_descriptor(64, 8))
+    with pytest.raises(_lib.NGPError, match="n_workspace_blocks must be positive"):
+        _lib.call_raw("nerf_mlp_backward_no_alloc", [16] * 6, _lib.nerf_mlp_descriptor(64, 0))
+    assert _lib.last_error() == (0, "")
+
+
 def test_error_is_seen_from_another_thread():
     """A host framework may run targets on its own executor threads (XLA does) while
     another thread polls ngp_get_last_error: the error state is process-wide."""
