@@ -71,6 +71,139 @@ def cpu_baseline(seconds: float = 12.0) -> dict:
                       f"occupancy grid: oracle C marcher/encoder/integrator (OpenMP) + numpy f32 MLP"}
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_launches(fn, reps: int, stream) -> float:
+    """Median ms of `fn()` over `reps` launches, each bracketed by HIP events on
+    `stream` (the stream the targets are launched on)."""
+    import torch
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for _ in range(3):
+        fn()
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    t = sorted(a.elapsed_time(b) for a, b in ev)
+    return t[len(t) // 2]
+
+
+def bench_encoder(dev, reps: int = 50) -> dict:
+    """The hash encoder alone on its own workload: L=16, F=2, T=2^19, N_min=16,
+    N_max=2048, f32 table U(-1e-4, 1e-4) (seed 1), 2^18 points xyz ~ U[0,1)^3
+    (seed 0) passed as pos = 2 xyz - 1 with bound 1 to the drop-in point-major
+    encoder, cotangent N(0,1) [2^18, 32] (seed 2): forward and backward (table
+    gradient) per launch, against 1,164 algorithmic bytes per point each."""
+    import torch
+    from jaxngp_amd import _lib
+    from jaxngp_amd.encoders import HashGridEncoder
+    N = 1 << 18
+    enc = HashGridEncoder(L=16, T=1 << 19, F=2, N_min=16, N_max=2048, device=dev,
+                          generator=torch.Generator(device="cpu").manual_seed(1))
+    xyz = torch.from_numpy(np.random.default_rng(0).uniform(0, 1, (N, 3)).astype(np.float32)).to(dev)
+    pos = (2 * xyz - 1).contiguous()
+    g = torch.from_numpy(np.random.default_rng(2).normal(size=(N, 32)).astype(np.float32)).to(dev)
+    table = enc.table.detach()
+    out = torch.empty(N, 32, device=dev)
+    dtable = torch.empty_like(table)
+    desc = enc.descriptor(N, 1.0)
+    ws = torch.empty(_lib.hashgrid_backward_workspace_bytes(desc), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+    fwd = _lib.Prepared("hashgrid_encode", [pos, table, out], desc)
+    bwd = _lib.Prepared("hashgrid_encode_backward", [pos, g, dtable, ws], desc)
+    f_ms = time_launches(fwd, reps, stream)
+    b_ms = time_launches(bwd, reps, stream)
+    _lib.check()
+    algo = N * ENC_BYTES_PER_POINT
+    res = {"points": N, "table": "f32 [6098120, 2]", "layout": "point-major [N, 32]",
+           "forward_ms": f_ms, "backward_ms": b_ms,
+           "forward_GBps": algo / (f_ms * 1e-3) / 1e9, "backward_GBps": algo / (b_ms * 1e-3) / 1e9,
+           "algorithmic_bytes_per_launch": algo}
+    res["forward_frac"] = res["forward_GBps"] / HBM_PEAK_GBS
+    res["backward_frac"] = res["backward_GBps"] / HBM_PEAK_GBS
+    log(f"encoder alone: forward {f_ms * 1e3:.1f} us ({res['forward_GBps']:.0f} GB/s), backward "
+        f"{b_ms * 1e3:.1f} us ({res['backward_GBps']:.0f} GB/s)")
+    return res
+
+
+def bench_integrate(dev, reps: int = 50) -> dict:
+    """integrate_rays forward and backward alone: 4096 rays of S / 4096 samples each
+    (S = 2^18 and 2^20), densities softplus(N(0,2)), rgb and bg U[0,1], ds =
+    2 sqrt(3) / 1024, z = 3 + k ds, near 0.3 (seed 4), against 24 / 44 algorithmic
+    bytes per sample and 40 / 68 per ray."""
+    import torch
+    from jaxngp_amd import _lib
+    out = {}
+    rng = np.random.default_rng(4)
+    stream = torch.cuda.current_stream()
+    n = 4096
+    for S in (1 << 18, 1 << 20):
+        per = S // n
+        ns = np.full(n, per, np.uint32)
+        start = (np.arange(n, dtype=np.uint64) * per).astype(np.uint32)
+        ds = np.float32(2 * 3 ** 0.5 / 1024)
+        dss = np.full(S, ds, np.float32)
+        z = (3 + np.tile(np.arange(per, dtype=np.float32), n) * ds).astype(np.float32)
+        sig = np.log1p(np.exp(rng.normal(0, 2, S))).astype(np.float32)
+        drgbs = np.concatenate([sig[:, None], rng.uniform(0, 1, (S, 3)).astype(np.float32)], 1)
+        bgs = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        st, nst, bg, dsd, zd, dr = T(start), T(ns), T(bgs), T(dss), T(z), T(drgbs)
+        meas = torch.empty(1, dtype=torch.uint32, device=dev)
+        rgbd, op = torch.empty(n, 4, device=dev), torch.empty(n, device=dev)
+        g = T(rng.normal(size=(n, 4)).astype(np.float32))
+        dbg, dz, dd = torch.empty(n, 3, device=dev), torch.empty(S, device=dev), torch.empty(S, 4, device=dev)
+        fwd = _lib.Prepared("integrate_rays", [st, nst, bg, dsd, zd, dr, meas, rgbd, op],
+                            _lib.integrating_descriptor(n, S))
+        bwd = _lib.Prepared("integrate_rays_backward", [st, nst, bg, dsd, zd, dr, rgbd, op, g, dbg, dz, dd],
+                            _lib.integrating_backward_descriptor(n, S, 0.3))
+        f_ms = time_launches(fwd, reps, stream)
+        b_ms = time_launches(bwd, reps, stream)
+        _lib.check()
+        fa = S * INT_FWD_BYTES_PER_SAMPLE + n * INT_FWD_BYTES_PER_RAY
+        ba = S * INT_BWD_BYTES_PER_SAMPLE + n * INT_BWD_BYTES_PER_RAY
+        key = f"samples_2^{S.bit_length() - 1}"
+        out[key] = {"rays": n, "samples_per_ray": per, "forward_ms": f_ms, "backward_ms": b_ms,
+                    "fwd_bwd_ms": f_ms + b_ms, "forward_frac": fa / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "backward_frac": ba / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"integrate alone, S={S}: forward {f_ms * 1e3:.1f} us, backward {b_ms * 1e3:.1f} us")
+    return out
+
+
+def steady_state(tr, steps: int = 200) -> dict:
+    """Plain timing (no events) of `steps` training steps of an already trained
+    field: the grid update runs every 16 steps there (after step 256)."""
+    import torch
+    counts = torch.zeros(steps, 4, dtype=torch.int32, device=tr.dev)
+    ctr = tr.counters.view(torch.int32)
+    first = tr.step_count
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        tr.step()
+        counts[i].copy_(ctr)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    c = counts.long()
+    n_samples = int((c[:, 0] - c[:, 1]).sum().item())
+    n_valid = int(c[:, 2].sum().item())
+    res = {"after_steps": first, "steps": steps, "ms_per_step": el / steps * 1e3, "value": n_samples / el,
+           "unit": "samples/s", "rays_per_s": n_valid / el, "samples_per_step": n_samples / steps}
+    log(f"[{tr.cfg.precision}] steady state after {first} steps: {res['ms_per_step']:.3f} ms/step, "
+        f"{res['value'] / 1e6:.1f} M samples/s")
+    return res
+
+
 CALIBRATION_STEPS = 30
 ROOFLINE_CANDIDATES = ("hashgrid_encode", "hashgrid_encode_backward", "march_rays", "integrate_rays",
                        "integrate_rays_backward", "nerf_mlp_forward", "nerf_mlp_backward")
@@ -199,7 +332,7 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
         render = lambda pose, st=None: plain(pose)
     for i in range(3):
         render(pose_t[i % n_frames])
-    times, psnrs, plain_psnrs, fallback, psnrs09, fallback09 = [], [], [], [], [], []
+    times, psnrs, plain_psnrs, fallback, psnrs09, fallback09, times09 = [], [], [], [], [], [], []
     for i in range(n_frames):
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -213,7 +346,11 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
             fallback.append(st["fallback_rays"] / (st["fallback_rays"] + st["proposal_rays"]))
             plain_psnrs.append(sc.psnr_u8(plain(pose_t[i])[1], ref))
             st = {}
+            torch.cuda.synchronize()
+            t = time.perf_counter()
             rgb09 = render(pose_t[i], st, mass_threshold=0.9, placement=_lib.PROPOSAL_SUPPORT)[1]
+            torch.cuda.synchronize()
+            times09.append((time.perf_counter() - t) * 1e3)
             psnrs09.append(sc.psnr_u8(rgb09, ref))
             fallback09.append(st["fallback_rays"] / (st["fallback_rays"] + st["proposal_rays"]))
     times.sort()
@@ -226,7 +363,7 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
                    psnr_plain_render_db=float(np.mean(plain_psnrs)),
                    delta_psnr_db=float(np.mean(psnrs) - np.mean(plain_psnrs)),
                    worst_view_delta_psnr_db=float(min(a - b for a, b in zip(psnrs, plain_psnrs))),
-                   at_mass_threshold_0_9={"placement": "support",
+                   at_mass_threshold_0_9={"placement": "support", "ms_per_frame": sorted(times09)[len(times09) // 2],
                                           "delta_psnr_db": float(np.mean(psnrs09) - np.mean(plain_psnrs)),
                                           "fallback_ray_fraction": float(np.mean(fallback09))})
     log(f"inference{' (proposal)' if proposal else ''}: {res['ms_per_frame']:.2f} ms/frame (median of {n_frames}), "
@@ -250,6 +387,10 @@ def main() -> None:
                     help="train (untimed, after the timed steps) to this many steps in all before rendering frames")
     ap.add_argument("--no-fp16", action="store_true", help="skip the f16-weight precision path")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal path")
+    ap.add_argument("--no-t22", action="store_true", help="skip the precision path at 2^22 entries per level")
+    ap.add_argument("--no-marcher-proposal", action="store_true",
+                    help="skip the proposal trained beside the marcher-trained field")
+    ap.add_argument("--no-legs", action="store_true", help="skip the encoder-alone and integrate-alone timings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy 5 times (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -284,6 +425,7 @@ def main() -> None:
     res = run_training(data, TrainConfig(), args.steps, args.warmup, dev)
     tr, per, dom, dom_ms, elapsed, n_samples, n_valid, ms = (
         res[k] for k in ("trainer", "per", "dom", "dom_ms", "elapsed", "samples", "valid", "ms"))
+    loss_fp32 = tr.metrics()["loss"]  # after warmup + timed steps, as the other paths' losses
     cfg = tr.cfg
     S = cfg.total_samples
     n_rays = tr.n_rays
@@ -341,33 +483,72 @@ def main() -> None:
                         f"region, the others from a {CALIBRATION_STEPS}-step untimed calibration pass before it",
         "roofline": roof,
     }
+    out["loss"] = loss_fp32
+    out["loss_at_step"] = tr.step_count
+    if not args.no_legs:
+        out["encoder_alone"] = bench_encoder(dev)
+        out["integrate_alone"] = bench_integrate(dev)
     if args.frames > 0:
         train_to(tr, args.render_steps)
+        out["steady_state"] = steady_state(tr)
         out["inference"] = bench_inference(tr, data, args.frames, dev)
     if not args.no_proposal:
-        # the learned-proposal path on the precision path: trained online with the field,
-        # rendering 48 proposal-placed samples per ray (occupancy-grid fallback)
-        rp = run_training(data, TrainConfig(precision="fp16", proposal=True), args.steps, args.warmup, dev)
-        tp = rp["trainer"]
-        out["proposal_fp16"] = {"value": rp["samples"] / rp["elapsed"], "unit": "samples/s",
-                                "ms_per_step": rp["ms"], "loss": tp.metrics()["loss"],
-                                "proposal_loss": tp.metrics()["proposal_loss"]}
-        if args.frames > 0:
-            train_to(tp, args.render_steps)
-            out["proposal_fp16"]["inference"] = bench_inference(tp, data, args.frames, dev, proposal=True)
+        # the learned-proposal path on the precision path: the field trains on 48
+        # proposal-placed samples per ray (the proposal on the field's weights at them)
+        # and renders them (occupancy-grid fallback); `proposal_fp16_marcher_trained`:
+        # the same proposal trained beside the marcher-trained field
+        for key, guided in (("proposal_fp16", True), ("proposal_fp16_marcher_trained", False)):
+            if not guided and args.no_marcher_proposal:
+                continue
+            rp = run_training(data, TrainConfig(precision="fp16", proposal=True, proposal_guided=guided), args.steps,
+                              args.warmup, dev)
+            tp = rp["trainer"]
+            out[key] = {"value": rp["samples"] / rp["elapsed"], "unit": "samples/s",
+                        "ms_per_step": rp["ms"], "rays_per_s": rp["valid"] / rp["elapsed"],
+                        "n_rays": tp.n_rays, "samples_per_step": rp["samples"] / args.steps,
+                        "loss": tp.metrics()["loss"], "proposal_loss": tp.metrics()["proposal_loss"],
+                        "loss_at_step": tp.step_count}
+            if args.frames > 0:
+                train_to(tp, args.render_steps)
+                out[key]["steady_state"] = steady_state(tp)
+                out[key]["inference"] = bench_inference(tp, data, args.frames, dev, proposal=True)
+                if "inference" in out:
+                    out[key]["inference"]["delta_psnr_vs_plain_fp32_db"] = (
+                        out[key]["inference"]["psnr_db"] - out["inference"]["psnr_db"])
+            del rp, tp
     if not args.no_fp16:
         # the precision path on the same scene: f16 table and MLP weights, loss scaling
         r16 = run_training(data, TrainConfig(precision="fp16"), args.steps, args.warmup, dev)
         t16 = r16["trainer"]
         out["fp16"] = {"value": r16["samples"] / r16["elapsed"], "unit": "samples/s", "ms_per_step": r16["ms"],
-                       "kernel_ms_per_launch": r16["per"],
-                       "loss": t16.metrics()["loss"], "loss_fp32": tr.metrics()["loss"]}
+                       "rays_per_s": r16["valid"] / r16["elapsed"], "kernel_ms_per_launch": r16["per"],
+                       "loss": t16.metrics()["loss"], "loss_fp32": loss_fp32, "loss_at_step": t16.step_count}
         if args.frames > 0:
             train_to(t16, args.render_steps)
+            out["fp16"]["steady_state"] = steady_state(t16)
             out["fp16"]["inference"] = bench_inference(t16, data, args.frames, dev)
+            if "inference" in out:
+                out["fp16"]["delta_psnr_vs_fp32_db"] = out["fp16"]["inference"]["psnr_db"] - out["inference"]["psnr_db"]
+        del r16, t16
+        if not args.no_t22:
+            # the precision path with 2^22 entries per level (39,474,024 rows, f16)
+            r22 = run_training(data, TrainConfig(precision="fp16", log2_hashmap_size=22), args.steps, args.warmup,
+                               dev)
+            t22 = r22["trainer"]
+            rec = {"value": r22["samples"] / r22["elapsed"], "unit": "samples/s", "ms_per_step": r22["ms"],
+                   "table_rows": t22.model.position_encoder.levels.n_rows, "loss": t22.metrics()["loss"],
+                   "loss_at_step": t22.step_count}
+            if args.frames > 0:
+                train_to(t22, args.render_steps)
+                rec["inference"] = bench_inference(t22, data, args.frames, dev)
+                if "inference" in out:
+                    rec["delta_psnr_vs_fp32_db"] = rec["inference"]["psnr_db"] - out["inference"]["psnr_db"]
+            out["fp16_T2^22"] = rec
+            del r22, t22
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
     print(json.dumps(out), flush=True)
 
 

@@ -336,6 +336,44 @@ typedef struct NerfFieldChunksDescriptor {
  * out: drgbs f32[n_slots, cap, 4] */
 void nerf_field_chunks(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
+typedef struct OccupancyGridUpdateDescriptor {
+    HashGridDescriptor grid; /* the NeRF position encoder (L = 16, F = 2); n_points and
+                                enc_layout are not used, grid.bound is the scene bound */
+    uint32_t G;              /* grid resolution (power of two, <= 1024) */
+    uint32_t K;              /* cascades: density f32[K G^3], bitfield u8[K G^3 / 8] */
+    uint32_t cascade;        /* the cascade this call updates */
+    uint32_t n_alive;        /* alive cells of the cascade */
+    uint32_t update_all;     /* 1: every alive cell once; 0: 2 * max(1, max(1, n_alive / 2) / 2)
+                                random cells, the first half uniform over the alive cells,
+                                the second half uniform over the occupied ones */
+    uint32_t param_dtype;    /* MLP weights: NGP_DTYPE_F32 or NGP_DTYPE_F16 */
+    uint32_t seed_lo, seed_hi, step; /* the draws' key */
+} OccupancyGridUpdateDescriptor;
+
+/* NeRFState.update_ogrid_density (utils/types.py:1149-1225) for one cascade: the
+ * alive cells' densities are multiplied by 0.95, then each chosen cell's density
+ * becomes max(its decayed density, the field's density at a random point of the
+ * cell).  The point of cell m (Morton index within the cascade) is
+ * (morton3d_invert(m) / (G - 1) * 2 - 1) * (mip - mip / G) + (2 u - 1) mip / G per
+ * axis, mip = min(bound, 2^cascade), u uniform in [0, 1) (types.py:1193-1207); the
+ * density is trunc_exp of the density MLP's first output on the position encoding
+ * (models/nerfs.py:66-77; only 32 -> 64 -> 16 runs).  A cell chosen several times
+ * keeps the largest of its densities (the reference's scatter keeps one of them,
+ * order unspecified).  The occupied cells are those set in the bitfield (the last
+ * threshold; they are alive by construction); with none, the second half is
+ * uniform over the alive cells as well.  Random numbers: r_s = splitmix64 draw of
+ * (seed, step, key = cascade << 32 | slot, stream s); a uniform choice among n is
+ * (r >> 32) * n >> 32; the first half (and an empty occupied set) chooses alive
+ * cells with r_0, the second half occupied cells with r_1 (in Morton order), and
+ * the point's u for x, y, z are bits 43-63, 22-42 and 1-21 of r_2 over 2^21.
+ * in : alive u32[n_alive] (Morton indices within the cascade, ascending),
+ *      occupancy_bitfield u8[K G^3 / 8], table f32|f16[offsets[16], 2],
+ *      params f32|f16[NGP_MLP_N_PARAMS]
+ * inout: density f32[K G^3]
+ * out: scratch u32[G^3 / 64 + 1] (occupied-cell prefix counts; not used, and may
+ *      be NULL, when update_all) */
+void occupancy_grid_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
 typedef struct NgpPoolScatterDescriptor {
     uint32_t n;       /* pool slots */
     uint32_t n_total; /* rows of dst */
@@ -443,6 +481,30 @@ typedef struct ProposalRenderDescriptor {
  *      occupancy_bitfield u8[K*G^3/8]
  * out: as proposal_sample */
 void proposal_render_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+typedef struct ProposalTrainDescriptor {
+    ProposalRenderDescriptor render; /* the proposal, quantile placement, the mass threshold
+                                        (0 in training: every ray gets its samples) and
+                                        the marching statics; render.sample.n_rays rays */
+    uint32_t total_samples;          /* S, as march_rays */
+} ProposalTrainDescriptor;
+
+/* Training samples placed by the proposal instead of the occupancy-grid marcher
+ * (no counterpart in the reference), in march_rays' buffer layout so the rest of
+ * the step runs unchanged: per ray, the proposal's 64 bin densities give n_per_ray
+ * samples at the (j + noise) / n_per_ray quantiles of its predicted mass
+ * (stratified), those in cells the occupancy grid marks empty are dropped, and the
+ * rays' kept samples are packed in ray order; a ray is valid iff t_end > t_start
+ * and its samples fit the S slots after those of the rays before it.
+ * next_sample_write_location = the samples every ray asked for,
+ * number_of_exceeded_samples = those of the rays that did not fit.
+ * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n],
+ *      noises f32[n], table f32|f16[offsets[5], 2], params f32|f16[NGP_PROPOSAL_N_PARAMS],
+ *      occupancy_bitfield u8[K*G^3/8]
+ * out: as march_rays (next_sample_write_location u32[1], number_of_exceeded_samples
+ *      u32[1], ray_is_valid bool[n], rays_n_samples u32[n], rays_sample_startidx
+ *      u32[n], idcs u32[S], xyzs f32[S,3], dirs f32[S,3], dss f32[S], z_vals f32[S]) */
+void proposal_train_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* ---------------------------------------------------------------------------
  * Training-step harness (the device-side counterpart of app/nerf/_utils.py:80-170

@@ -47,6 +47,7 @@ TARGETS = (
     "loss_scale_check",
     "loss_scale_update",
     "nerf_field_chunks",
+    "occupancy_grid_update",
     "pool_scatter",
     "proposal_points",
     "proposal_mlp_forward",
@@ -54,6 +55,7 @@ TARGETS = (
     "proposal_loss",
     "proposal_sample",
     "proposal_render_samples",
+    "proposal_train_samples",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -325,6 +327,11 @@ def proposal_render_descriptor(grid_desc: bytes, sample_desc: bytes, param_dtype
     return grid_desc + sample_desc + struct.pack("<I", param_dtype)
 
 
+def proposal_train_descriptor(render_desc: bytes, total_samples) -> bytes:
+    """ProposalTrainDescriptor: ProposalRenderDescriptor, u32 total_samples."""
+    return render_desc + struct.pack("<I", total_samples)
+
+
 def proposal_loss_descriptor(n_rays, eps=1e-7) -> bytes:
     return struct.pack("<If", n_rays, eps)
 
@@ -336,6 +343,13 @@ def pool_scatter_descriptor(n, n_total, width) -> bytes:
 def field_chunks_descriptor(grid_desc: bytes, n_slots, cap, n_total_rays, param_dtype=DTYPE_F32) -> bytes:
     """NerfFieldChunksDescriptor: a HashGridDescriptor followed by four u32."""
     return grid_desc + struct.pack("<4I", n_slots, cap, n_total_rays, param_dtype)
+
+
+def occupancy_update_descriptor(grid_desc: bytes, G, K, cascade, n_alive, update_all, param_dtype, seed, step) -> bytes:
+    """OccupancyGridUpdateDescriptor: a HashGridDescriptor followed by nine u32."""
+    seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    return grid_desc + struct.pack("<9I", G, K, cascade, n_alive, int(bool(update_all)), param_dtype,
+                                   seed & 0xFFFFFFFF, seed >> 32, int(step) & 0xFFFFFFFF)
 
 
 def loss_scale_check_descriptor(n) -> bytes:

@@ -15,25 +15,12 @@
 #include <hip/hip_fp16.h>
 
 #include "ngp_common.h"
+#include "rng_device.h"
 
 namespace ngp {
 namespace {
 
 constexpr int kBlock = 256;
-
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-    // splitmix64 finaliser (Steele, Lea, Flood 2014)
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
-
-__device__ __forceinline__ uint64_t draw(uint64_t seed, uint64_t step, uint64_t i, uint64_t stream) {
-    return mix64(seed ^ mix64(step * 0x9e3779b97f4a7c15ull ^ mix64(i * 4 + stream)));
-}
-
-// uniform in [0, 1) with 24 random bits
-__device__ __forceinline__ float unit24(uint64_t r) { return (float)(uint32_t)(r >> 40) * (1.0f / 16777216.0f); }
 
 __global__ void __launch_bounds__(kBlock)
     make_rays_kernel(NgpTrainRaysDescriptor d, const float *__restrict__ transforms, const uint8_t *__restrict__ rgba,

@@ -262,16 +262,64 @@ __device__ __forceinline__ void seg_step(float *v, uint32_t &fl) {
     fl |= fin;
 }
 
-// Visits the run-merged contributions of work item `item` and calls
-// emit(row, v) on each run's last lane.  The same inputs always give the same
-// multiset of emitted rows, so counting and scattering agree.
+// The run-merged contributions of one point per lane at level l: the wavefront's
+// 64 lanes hold 64 consecutive points; for each corner, lanes whose row equals the
+// previous lane's join its run, and emit(row, v) is called on each run's last lane
+// with the run's summed values.  Inactive lanes (past the last point) get keys no
+// active lane has, so they form no runs.
+template <int F, bool kValues, typename Emit>
+__device__ __forceinline__ void point_runs(const HashGridDescriptor &d, uint32_t l, float scale, uint32_t res,
+                                           bool dense, bool pow2, const float *xyz, const float *gin, bool active,
+                                           Emit emit) {
+    const int lane = lane_id();
+    const float two_b = 2.0f * d.bound;
+    const float px = (xyz[0] + d.bound) / two_b;
+    const float py = (xyz[1] + d.bound) / two_b;
+    const float pz = (xyz[2] + d.bound) / two_b;
+    uint32_t idx[8];
+    float w[8];
+    level_vertices(px, py, pz, scale, res, dense, d.T, pow2, d.offsets[l], d.n_rows - 1u, idx, w);
+    float g[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) g[f] = kValues && active ? gin[f] : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const uint32_t key = active ? idx[c] : 0xffffffffu - (uint32_t)lane;
+        float v[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) v[f] = w[c] * g[f];
+        const uint32_t prev = dpp_u32<kWaveShr1, 0xf>(key, ~key);  // lane 0: no predecessor
+        const bool head = prev != key;
+        bool tail = true;
+        if (__ballot(!head) != 0) {
+            if (kValues) {
+                // segmented inclusive sums of runs: (v, flag) pairs combined by
+                // (a, fa) + (b, fb) = (fb ? b : a + b, fa | fb), shifts within
+                // 16-lane rows, then row 15 -> rows 1 and 3, lane 31 -> rows 2 and 3
+                uint32_t fl = head ? 1u : 0u;
+                seg_step<kRowShr1, 0xf, F>(v, fl);
+                seg_step<kRowShr2, 0xf, F>(v, fl);
+                seg_step<kRowShr4, 0xf, F>(v, fl);
+                seg_step<kRowShr8, 0xf, F>(v, fl);
+                seg_step<kRowBcast15, 0xa, F>(v, fl);
+                seg_step<kRowBcast31, 0xc, F>(v, fl);
+            }
+            const uint32_t next_head = dpp_u32<kWaveShl1, 0xf>(head ? 1u : 0u, 1u);  // lane 63: end of wave
+            tail = next_head != 0u;
+        }
+        if (tail && active) emit(key, v, c);
+    }
+}
+
+// Visits the run-merged contributions of work item `item` (level l, kChunk
+// consecutive points) and calls emit(row, v, corner) on each run's last lane.
+// The same inputs always give the same multiset of emitted rows, so counting and
+// scattering agree.
 template <int F, bool kValues, typename Emit>
 __device__ __forceinline__ void item_contributions(const HashGridDescriptor &d, const BwdPlan &p, uint32_t item,
                                                    const float *__restrict__ pos, const float *__restrict__ dL_denc,
                                                    Emit emit) {
     const uint32_t l = item / p.n_chunks, k = item % p.n_chunks;
-    const int lane = lane_id();
-    const float two_b = 2.0f * d.bound;
     const bool pow2 = (d.T & (d.T - 1u)) == 0;
     const bool level_major = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
     const float scale = d.scales[l];
@@ -300,44 +348,7 @@ __device__ __forceinline__ void item_contributions(const HashGridDescriptor &d, 
 #pragma unroll
         for (int sub = 0; sub < kGroup; ++sub) {
             const uint32_t n = k * kChunk + (g0 + sub) * kItemThreads + threadIdx.x;
-            const bool active = n < d.n_points;
-            const float px = (xyz[sub][0] + d.bound) / two_b;
-            const float py = (xyz[sub][1] + d.bound) / two_b;
-            const float pz = (xyz[sub][2] + d.bound) / two_b;
-            uint32_t idx[8];
-            float w[8];
-            level_vertices(px, py, pz, scale, res, dense, d.T, pow2, d.offsets[l], d.n_rows - 1u, idx, w);
-            float g[F];
-#pragma unroll
-            for (int f = 0; f < F; ++f) g[f] = kValues && active ? gin[sub][f] : 0.0f;
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                // inactive lanes get keys no active lane has, so they form no runs
-                const uint32_t key = active ? idx[c] : 0xffffffffu - (uint32_t)lane;
-                float v[F];
-#pragma unroll
-                for (int f = 0; f < F; ++f) v[f] = w[c] * g[f];
-                const uint32_t prev = dpp_u32<kWaveShr1, 0xf>(key, ~key);  // lane 0: no predecessor
-                const bool head = prev != key;
-                bool tail = true;
-                if (__ballot(!head) != 0) {
-                    if (kValues) {
-                        // segmented inclusive sums of runs: (v, flag) pairs combined by
-                        // (a, fa) + (b, fb) = (fb ? b : a + b, fa | fb), shifts within
-                        // 16-lane rows, then row 15 -> rows 1 and 3, lane 31 -> rows 2 and 3
-                        uint32_t fl = head ? 1u : 0u;
-                        seg_step<kRowShr1, 0xf, F>(v, fl);
-                        seg_step<kRowShr2, 0xf, F>(v, fl);
-                        seg_step<kRowShr4, 0xf, F>(v, fl);
-                        seg_step<kRowShr8, 0xf, F>(v, fl);
-                        seg_step<kRowBcast15, 0xa, F>(v, fl);
-                        seg_step<kRowBcast31, 0xc, F>(v, fl);
-                    }
-                    const uint32_t next_head = dpp_u32<kWaveShl1, 0xf>(head ? 1u : 0u, 1u);  // lane 63: end of wave
-                    tail = next_head != 0u;
-                }
-                if (tail && active) emit(key, v);
-            }
+            point_runs<F, kValues>(d, l, scale, res, dense, pow2, xyz[sub], gin[sub], n < d.n_points, emit);
         }
     }
 }
@@ -351,7 +362,7 @@ __global__ void __launch_bounds__(kItemThreads)
     __syncthreads();
     const uint32_t item = blockIdx.x;
     item_contributions<F, false>(d, p, item, pos, nullptr,
-                                 [&](uint32_t row, const float *) { atomicAdd(&hist[row / p.R], 1u); });
+                                 [&](uint32_t row, const float *, int) { atomicAdd(&hist[row / p.R], 1u); });
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) counts[(size_t)s * p.n_items + item] = hist[s];
 }
@@ -452,7 +463,7 @@ __global__ void __launch_bounds__(kItemThreads)
     const uint32_t item = blockIdx.x;
     for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) cursor[s] = offsets[(size_t)s * p.n_items + item];
     __syncthreads();
-    item_contributions<F, true>(d, p, item, pos, dL_denc, [&](uint32_t row, const float *v) {
+    item_contributions<F, true>(d, p, item, pos, dL_denc, [&](uint32_t row, const float *v, int) {
         const uint32_t s = row / p.R;
         const uint32_t at = atomicAdd(&cursor[s], 1u);
         uint32_t w[EW];
@@ -470,6 +481,120 @@ __global__ void __launch_bounds__(kItemThreads)
             for (int q = 0; q < EW / 4; ++q) e[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
         }
     });
+}
+
+// The last row level l's vertices can address.  A dense level's vertex
+// coordinates reach res (a point on the upper face), so its rows run up to
+// res + res^2 + res^3 past the level's offset, into the next level's rows, as the
+// reference's indices do; hashed rows stay below offset + T.  Clamped to the table.
+__host__ __device__ inline uint32_t level_last_row(const HashGridDescriptor &d, uint32_t l) {
+    const uint64_t res = d.resolutions[l];
+    uint64_t local = (uint64_t)d.T - 1u;
+    if (l < d.first_hash_level) local = std::min<uint64_t>(local, res + res * res + res * res * res);
+    return (uint32_t)std::min<uint64_t>((uint64_t)d.offsets[l] + local, (uint64_t)d.n_rows - 1u);
+}
+
+// 3 (F <= 2).  The same entries, staged in LDS first: each phase of kStagePhase
+// points (one per thread) counts its entries per slice, scans the counts over the
+// level's slices and places every entry in LDS in slice order, then writes each
+// slice's run of the item with consecutive stores.  A wavefront's stores then
+// cover a few contiguous runs instead of up to 64 scattered rows.  Every slice's
+// run gets exactly the entries bwd_scatter_kernel writes there (the same runs, in
+// another order within the run).
+constexpr int kStageThreads = 512;
+constexpr int kStagePhase = kStageThreads;  // points per phase
+constexpr int kStagePhases = kChunk / kStagePhase;
+constexpr uint32_t kStageSlices = 1024;     // slices one level may span
+constexpr int kStageWords = 4;              // destination index + entry (EW <= 3), 16 bytes
+constexpr size_t kStageBytes = (size_t)kStagePhase * 8 * kStageWords * 4;
+static_assert(kChunk % kStagePhase == 0 && kStageSlices == 2 * kStageThreads, "staged scatter shape");
+
+template <int F, bool kHalf>
+__global__ void __launch_bounds__(kStageThreads)
+    bwd_scatter_staged_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos,
+                              const float *__restrict__ dL_denc, const uint32_t *__restrict__ offsets,
+                              uint32_t *__restrict__ entries) {
+    constexpr int EW = entry_words<F, kHalf>();
+    static_assert(EW < kStageWords, "entry fits the staging slot");
+    __shared__ uint32_t cursor[kStageSlices];  // next global index of each slice's run
+    __shared__ uint32_t a[kStageSlices];       // phase count, then cursor - first staging slot
+    __shared__ uint32_t place[kStageSlices];   // next staging slot of each slice
+    extern __shared__ uint4 stage[];           // [kStagePhase * 8] (destination, entry words)
+    const uint32_t item = blockIdx.x;
+    const uint32_t l = item / p.n_chunks, k = item % p.n_chunks;
+    const uint32_t s_lo = d.offsets[l] / p.R;
+    const uint32_t span = level_last_row(d, l) / p.R - s_lo + 1u;  // <= kStageSlices (host check)
+    for (uint32_t s = threadIdx.x; s < kStageSlices; s += kStageThreads) {
+        cursor[s] = s < span ? offsets[(size_t)(s_lo + s) * p.n_items + item] : 0u;
+        a[s] = 0u;
+    }
+    __syncthreads();
+    const bool pow2 = (d.T & (d.T - 1u)) == 0;
+    const bool level_major = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
+    const float scale = d.scales[l];
+    const uint32_t res = d.resolutions[l];
+    const bool dense = l < d.first_hash_level;
+    for (int ph = 0; ph < kStagePhases; ++ph) {
+        const uint32_t n = k * kChunk + ph * kStagePhase + threadIdx.x;
+        const bool active = n < d.n_points;
+        const uint32_t np = active ? n : d.n_points - 1;
+        float xyz[3], gin[F];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) xyz[q] = pos[3 * (size_t)np + q];
+        const size_t gb = level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
+#pragma unroll
+        for (int f = 0; f < F; ++f) gin[f] = dL_denc[gb + f];
+        uint32_t rows[8];
+        float vals[8][F];
+        uint32_t mask = 0;
+        point_runs<F, true>(d, l, scale, res, dense, pow2, xyz, gin, active, [&](uint32_t row, const float *v, int c) {
+            rows[c] = row;
+#pragma unroll
+            for (int f = 0; f < F; ++f) vals[c][f] = v[f];
+            mask |= 1u << c;
+        });
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if ((mask >> c) & 1u) atomicAdd(&a[rows[c] / p.R - s_lo], 1u);
+        __syncthreads();
+        // exclusive scan of the counts over the level's slices, two per thread
+        const uint32_t c0 = a[2 * threadIdx.x], c1 = a[2 * threadIdx.x + 1];
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<kStageThreads>(c0 + c1, &total);
+        place[2 * threadIdx.x] = ex;
+        place[2 * threadIdx.x + 1] = ex + c0;
+        a[2 * threadIdx.x] = cursor[2 * threadIdx.x] - ex;
+        a[2 * threadIdx.x + 1] = cursor[2 * threadIdx.x + 1] - (ex + c0);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if ((mask >> c) & 1u) {
+                const uint32_t sl = rows[c] / p.R;
+                const uint32_t s = sl - s_lo;
+                const uint32_t q = atomicAdd(&place[s], 1u);
+                uint32_t w[kStageWords] = {0u, 0u, 0u, 0u};
+                w[0] = rows[c] - sl * p.R;
+                pack_values<F, kHalf>(w, vals[c]);
+                stage[q] = make_uint4(a[s] + q, w[0], w[1], w[2]);
+            }
+        }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < total; q += kStageThreads) {
+            const uint4 e = stage[q];
+            uint32_t *dst = entries + (size_t)e.x * EW;
+            if constexpr (EW == 2) {
+                *reinterpret_cast<uint2 *>(dst) = make_uint2(e.y, e.z);
+            } else {
+                *reinterpret_cast<uint3 *>(dst) = make_uint3(e.y, e.z, e.w);
+            }
+        }
+        __syncthreads();
+        for (uint32_t s = threadIdx.x; s < kStageSlices; s += kStageThreads) {
+            cursor[s] = a[s] + place[s];
+            a[s] = 0u;
+        }
+        __syncthreads();
+    }
 }
 
 template <int EW>
@@ -585,6 +710,16 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The staged scatter takes entries of at most 3 words (F <= 2) and levels whose
+// rows span at most kStageSlices slices.
+inline bool staged_scatter_fits(const HashGridDescriptor &d, const BwdPlan &p) {
+    if (d.F > 2) return false;
+    for (uint32_t l = 0; l < d.L; ++l) {
+        if (level_last_row(d, l) / p.R - d.offsets[l] / p.R + 1u > kStageSlices) return false;
+    }
+    return true;
+}
+
 template <int F, bool kHalf>
 bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const float *pos, const float *g, float *dtable,
                      uint8_t *ws, const char *target) {
@@ -610,8 +745,24 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
                        sums, meta);
     hipLaunchKernelGGL(plan_parts_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, counts, meta, parts,
                        part_slice);
-    hipLaunchKernelGGL((bwd_scatter_kernel<F, kHalf>), dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, g, counts,
-                       entries);
+    bool staged = false;
+    if constexpr (entry_words<F, kHalf>() < kStageWords) staged = staged_scatter_fits(d, p);
+    if (staged) {
+        if constexpr (entry_words<F, kHalf>() < kStageWords) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_scatter_staged_kernel<F, kHalf>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStageBytes);
+        if (e != hipSuccess) {
+            set_error(kLaunchFailed, "%s: cannot reserve %zu bytes of LDS: %s", target, kStageBytes,
+                      hipGetErrorString(e));
+            return false;
+        }
+        hipLaunchKernelGGL((bwd_scatter_staged_kernel<F, kHalf>), dim3(p.n_items), dim3(kStageThreads), kStageBytes,
+                           stream, d, p, pos, g, counts, entries);
+        }
+    } else {
+        hipLaunchKernelGGL((bwd_scatter_kernel<F, kHalf>), dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos,
+                           g, counts, entries);
+    }
     hipLaunchKernelGGL((bwd_reduce_kernel<F, kHalf>), dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts, meta,
                        parts, part_slice, entries, partials, dtable);
     hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices, kCombineSplit), dim3(256), 0, stream, d, p, parts,

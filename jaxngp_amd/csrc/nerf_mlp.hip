@@ -28,6 +28,8 @@
 
 #include "ngp_common.h"
 #include "hashgrid_device.h"
+#include "ngp_device.h"
+#include "rng_device.h"
 
 namespace ngp {
 namespace {
@@ -923,6 +925,191 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Occupancy-grid update (utils/types.py:1149-1225).  A wavefront takes 16 chosen
+// cells; lane (i, g) derives cell i's random point, computes the four levels of
+// its encoding that its MFMA operand needs (as in field_chunks_kernel), and the
+// density MLP (32 -> 64 -> 16, nerfs.py:66-77) runs on MFMA; lane (i, 0) then
+// applies trunc_exp and the max.  Cell choice and points come from draws keyed by
+// (seed, step, cascade << 32 | slot).
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kDecayBits = 0x3f733333u;  // 0.95f (types.py:1163)
+
+struct OgridArgs {
+    uint32_t G, cascade, n_alive, n_update, half;  // half: slots of the uniform first half
+    bool update_all;
+    uint64_t seed;
+    uint32_t step;
+    float bound, mip, hcw;
+};
+
+// jnp.maximum: NaN if either operand is NaN
+__device__ __forceinline__ float max_nan(float a, float b) { return (a != a || b != b) ? a + b : fmaxf(a, b); }
+
+// Position of bit k (0-based, k < popcount(w)) among the set bits of w.
+__device__ __forceinline__ uint32_t select_bit(uint64_t w, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (int width = 32; width > 0; width >>= 1) {
+        const uint64_t low = w & ((1ull << width) - 1ull);
+        const uint32_t c = (uint32_t)__popcll(low);
+        if (k >= c) {
+            k -= c;
+            pos += width;
+            w >>= width;
+        } else {
+            w = low;
+        }
+    }
+    return pos;
+}
+
+// Morton index (within the cascade) of update slot j.
+__device__ __forceinline__ uint32_t ogrid_cell(const OgridArgs &a, uint32_t j, const uint32_t *__restrict__ alive,
+                                               const uint32_t *__restrict__ prefix, const uint64_t *__restrict__ words) {
+    if (a.update_all) return alive[j];
+    const uint64_t key = ((uint64_t)a.cascade << 32) | j;
+    if (j >= a.half) {
+        const uint32_t W = a.G * a.G * a.G / 64u;  // 64-cell words per cascade (G >= 4)
+        const uint32_t n_occ = prefix[W];
+        if (n_occ > 0) {
+            const uint32_t r = below(draw(a.seed, a.step, key, 1), n_occ);
+            uint32_t lo = 0, hi = W;  // largest w with prefix[w] <= r
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (prefix[mid] <= r) lo = mid;
+                else hi = mid;
+            }
+            return lo * 64u + select_bit(words[(size_t)a.cascade * W + lo], r - prefix[lo]);
+        }
+    }
+    return alive[below(draw(a.seed, a.step, key, 0), a.n_alive)];
+}
+
+// Random point of cell m (types.py:1193-1207): the cell centre mapped into
+// [-mip + hcw, mip - hcw], plus a uniform offset in [-hcw, hcw) per axis from three
+// 21-bit uniforms of one draw.
+__device__ __forceinline__ void ogrid_point(const OgridArgs &a, uint32_t m, uint32_t j, float &x, float &y,
+                                            float &z) {
+    const uint64_t r = draw(a.seed, a.step, ((uint64_t)a.cascade << 32) | j, 2);
+    const float u[3] = {(float)((uint32_t)(r >> 43) & 0x1fffffu) * (1.0f / 2097152.0f),
+                        (float)((uint32_t)(r >> 22) & 0x1fffffu) * (1.0f / 2097152.0f),
+                        (float)((uint32_t)(r >> 1) & 0x1fffffu) * (1.0f / 2097152.0f)};
+    const uint32_t v[3] = {morton_compact(m), morton_compact(m >> 1), morton_compact(m >> 2)};
+    const float g1 = (float)(a.G - 1u), span = a.mip - a.hcw;
+    float p[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float c = ((float)v[k] / g1) * 2.0f - 1.0f;
+        p[k] = c * span + (u[k] * 2.0f - 1.0f) * a.hcw;
+    }
+    x = p[0];
+    y = p[1];
+    z = p[2];
+}
+
+template <typename TT, bool kHalf>
+__global__ void __launch_bounds__(kThreads)
+    ogrid_update_kernel(HashGridDescriptor d, OgridArgs a, const uint32_t *__restrict__ alive,
+                        const uint32_t *__restrict__ prefix, const uint64_t *__restrict__ words,
+                        const TT *__restrict__ table, const void *__restrict__ params, float *__restrict__ density) {
+    __shared__ __align__(16) float lds_f[kHalf ? 1 : L_END];
+    __shared__ __align__(16) _Float16 lds_h[kHalf ? H_FEND : 1];
+    __shared__ LevelTables lt;
+    if (kHalf) stage_weights_h<false>(lds_h, static_cast<const _Float16 *>(params));
+    else stage_weights(lds_f, static_cast<const float *>(params));
+    if (threadIdx.x < 16) {
+        lt.scale[threadIdx.x] = d.scales[threadIdx.x];
+        lt.res[threadIdx.x] = d.resolutions[threadIdx.x];
+        lt.offset[threadIdx.x] = d.offsets[threadIdx.x];
+    }
+    __syncthreads();
+    const int lane = lane_id();
+    const int i = lane & 15, g = lane >> 4;
+    const uint32_t n_tiles = (a.n_update + kTile - 1) / kTile;
+    const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / kWave;
+    const uint32_t n_waves = gridDim.x * kWaves;
+    const float two_b = 2.0f * a.bound;
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    const uint32_t G3 = a.G * a.G * a.G;
+    float *dens = density + (size_t)a.cascade * G3;
+    for (uint32_t t = wave; t < n_tiles; t += n_waves) {
+        const uint32_t j = t * kTile + i;
+        bool live = j < a.n_update;
+        uint32_t m = 0;
+        v4f x[2] = {zero, zero};
+        if (live) {
+            m = ogrid_cell(a, j, alive, prefix, words);
+            live = m < G3;  // an alive index past the cascade is dropped, as JAX drops it
+        }
+        if (live) {
+            float wx, wy, wz;
+            ogrid_point(a, m, j, wx, wy, wz);
+            const float px = (wx + a.bound) / two_b, py = (wy + a.bound) / two_b, pz = (wz + a.bound) / two_b;
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const float2 lo = encode_level(d, lt, table, px, py, pz, 8 * b + 2 * g);
+                const float2 hi = encode_level(d, lt, table, px, py, pz, 8 * b + 2 * g + 1);
+                x[b] = v4f{lo.x, lo.y, hi.x, hi.y};
+            }
+        }
+        float o0;
+        if (kHalf) {
+            FwdH f;
+            const h8 xin = pack_h8(x[0], x[1]);
+            layer_fwd_h<1, 4, FT1>(lds_h + H_F1, &xin, f.h1, i, g);
+            relu<4>(f.h1);
+            const h8 h1in[2] = {pack_h8(f.h1[0], f.h1[1]), pack_h8(f.h1[2], f.h1[3])};
+            layer_fwd_h<2, 1, FT2>(lds_h + H_F2, h1in, &f.c[0], i, g);
+            o0 = f.c[0][0];
+        } else {
+            Fwd f;
+            layer_fwd<32, 64, S_W1>(lds_f + L_W1, x, f.h1, i, g);
+            relu<4>(f.h1);
+            layer_fwd<64, 16, S_W2>(lds_f + L_W2, f.h1, &f.c[0], i, g);
+            o0 = f.c[0][0];
+        }
+        if (g == 0 && live) {
+            const float sigma = det_expf(o0);
+            if (a.update_all) {
+                // each alive cell is in exactly one slot: decay and max in place
+                dens[m] = max_nan(dens[m] * __uint_as_float(kDecayBits), sigma);
+            } else {
+                // decayed by ogrid_decay_kernel; densities of alive cells are >= +0 (or
+                // NaN), so the unsigned order of their bits is the float order, NaN on top
+                atomicMax(reinterpret_cast<unsigned int *>(dens + m), __float_as_uint(sigma));
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) ogrid_decay_kernel(uint32_t n_alive, uint32_t G3, const uint32_t *__restrict__ alive,
+                                                          float *__restrict__ dens) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= n_alive) return;
+    const uint32_t m = alive[j];
+    if (m < G3) dens[m] = dens[m] * __uint_as_float(kDecayBits);
+}
+
+// prefix[w] = occupied cells in the cascade's 64-cell words [0, w), w = 0..W
+// (one workgroup; each thread counts a run of consecutive words).
+constexpr int kPrefixThreads = 1024;
+__global__ void __launch_bounds__(kPrefixThreads)
+    ogrid_prefix_kernel(uint32_t W, const uint64_t *__restrict__ words, uint32_t *__restrict__ prefix) {
+    const uint32_t per = (W + kPrefixThreads - 1) / kPrefixThreads;
+    const uint32_t w0 = min(W, threadIdx.x * per), w1 = min(W, w0 + per);
+    uint32_t own = 0;
+    for (uint32_t w = w0; w < w1; ++w) own += (uint32_t)__popcll(words[w]);
+    uint32_t total = 0;
+    uint32_t run = block_exclusive_scan<kPrefixThreads>(own, &total);
+    for (uint32_t w = w0; w < w1; ++w) {
+        prefix[w] = run;
+        run += (uint32_t)__popcll(words[w]);
+    }
+    if (threadIdx.x == 0) prefix[W] = total;
+}
+
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
 uint32_t forward_grid(uint32_t n) {
@@ -1074,23 +1261,35 @@ extern "C" void nerf_mlp_backward_no_alloc(ngp_stream_t stream_, void **buffers,
     launch_mlp_backward(d, stream, buffers, ws, target);
 }
 
+namespace {
+// The NeRF position encoder's descriptor: L = 16, F = 2, a consistent level table.
+bool check_nerf_grid(const HashGridDescriptor &g, uint32_t param_dtype, const char *target) {
+    if (g.L != 16 || g.F != 2 || g.T == 0 || g.n_rows == 0 || g.first_hash_level > g.L || g.offsets[0] != 0 ||
+        g.offsets[g.L] != g.n_rows || !(g.bound > 0.0f) || g.table_dtype > NGP_DTYPE_F16 ||
+        param_dtype > NGP_DTYPE_F16) {
+        set_error(kBadDescriptor, "%s: needs the NeRF grid (L=16, F=2) with a consistent level table and "
+                  "f32/f16 dtypes", target);
+        return false;
+    }
+    for (uint32_t l = 0; l < g.L; ++l) {
+        if (g.offsets[l + 1] < g.offsets[l]) {
+            set_error(kBadDescriptor, "%s: offsets must be non-decreasing", target);
+            return false;
+        }
+    }
+    return true;
+}
+}  // namespace
+
 extern "C" void nerf_field_chunks(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
     const char *target = "nerf_field_chunks";
     NerfFieldChunksDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
     const HashGridDescriptor &g = d.grid;
-    if (g.L != 16 || g.F != 2 || g.T == 0 || g.n_rows == 0 || g.first_hash_level > g.L || g.offsets[0] != 0 ||
-        g.offsets[g.L] != g.n_rows || !(g.bound > 0.0f) || g.table_dtype > NGP_DTYPE_F16 ||
-        d.param_dtype > NGP_DTYPE_F16 || d.cap == 0) {
-        set_error(kBadDescriptor, "%s: needs the NeRF grid (L=16, F=2) with a consistent level table, cap > 0 and "
-                  "f32/f16 dtypes", target);
+    if (!check_nerf_grid(g, d.param_dtype, target)) return;
+    if (d.cap == 0) {
+        set_error(kBadDescriptor, "%s: cap must be positive", target);
         return;
-    }
-    for (uint32_t l = 0; l < g.L; ++l) {
-        if (g.offsets[l + 1] < g.offsets[l]) {
-            set_error(kBadDescriptor, "%s: offsets must be non-decreasing", target);
-            return;
-        }
     }
     if (d.n_slots == 0) return;
     if (!check_buffers(buffers, 7, target)) return;
@@ -1120,5 +1319,75 @@ extern "C" void nerf_field_chunks(ngp_stream_t stream, void **buffers, const cha
     else
         hipLaunchKernelGGL((field_chunks_kernel<float, false>), dim3(grid), dim3(kThreads), 0, s, g, d.n_slots, d.cap,
                            d.n_total_rays, xyzs, ns, idx, rd, (const float *)buffers[4], params, out);
+    check_launch(target);
+}
+
+extern "C" void occupancy_grid_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "occupancy_grid_update";
+    OccupancyGridUpdateDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    const HashGridDescriptor &g = d.grid;
+    if (!check_nerf_grid(g, d.param_dtype, target)) return;
+    if (d.G < 4 || d.G > 1024 || (d.G & (d.G - 1u)) != 0 || d.K == 0 || d.cascade >= d.K || d.update_all > 1 ||
+        (uint64_t)d.n_alive > (uint64_t)d.G * d.G * d.G) {
+        set_error(kBadDescriptor, "%s: G must be a power of two in [4, 1024], cascade < K, update_all 0 or 1, "
+                  "n_alive <= G^3", target);
+        return;
+    }
+    if (d.n_alive == 0) return;  // no alive cell: nothing to decay or update
+    if (!check_buffers(buffers, 5, target)) return;
+    if (!d.update_all && !check_buffer_list(buffers, {5}, target)) return;
+    if ((reinterpret_cast<uintptr_t>(buffers[1]) & 7u) != 0) {
+        set_error(kBadArgument, "%s: the occupancy bitfield must be 8-byte aligned", target);
+        return;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    OgridArgs a;
+    a.G = d.G;
+    a.cascade = d.cascade;
+    a.n_alive = d.n_alive;
+    a.update_all = d.update_all != 0;
+    a.seed = ((uint64_t)d.seed_hi << 32) | d.seed_lo;
+    a.step = d.step;
+    a.bound = g.bound;
+    a.mip = std::fmin(g.bound, std::ldexp(1.0f, (int)std::min<uint32_t>(d.cascade, 126u)));
+    a.hcw = a.mip / (float)d.G;
+    const uint32_t *alive = (const uint32_t *)buffers[0];
+    const uint64_t *words = (const uint64_t *)buffers[1];
+    float *density = (float *)buffers[4];
+    uint32_t *prefix = (uint32_t *)buffers[5];
+    const uint32_t W = d.G * d.G * d.G / 64u;  // 64-cell words per cascade
+    if (a.update_all) {
+        a.n_update = d.n_alive;
+        a.half = 0;
+        prefix = nullptr;
+    } else {
+        const uint32_t M = std::max<uint32_t>(1, d.n_alive / 2), h = std::max<uint32_t>(1, M / 2);
+        a.n_update = 2 * h;
+        a.half = h;
+        float *dens = density + (size_t)d.cascade * d.G * d.G * d.G;
+        hipLaunchKernelGGL(ogrid_decay_kernel, dim3(div_up(d.n_alive, 256)), dim3(256), 0, s, d.n_alive,
+                           d.G * d.G * d.G, alive, dens);
+        if (!check_launch(target)) return;
+        hipLaunchKernelGGL(ogrid_prefix_kernel, dim3(1), dim3(kPrefixThreads), 0, s, W, words + (size_t)d.cascade * W,
+                           prefix);
+        if (!check_launch(target)) return;
+    }
+    const uint32_t tiles = div_up(a.n_update, kTile);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(tiles, kWaves), 2048));
+    const void *params = buffers[3];
+    const bool t16 = g.table_dtype == NGP_DTYPE_F16, w16 = d.param_dtype == NGP_DTYPE_F16;
+    if (t16 && w16)
+        hipLaunchKernelGGL((ogrid_update_kernel<_Float16, true>), dim3(grid), dim3(kThreads), 0, s, g, a, alive,
+                           prefix, words, (const _Float16 *)buffers[2], params, density);
+    else if (t16)
+        hipLaunchKernelGGL((ogrid_update_kernel<_Float16, false>), dim3(grid), dim3(kThreads), 0, s, g, a, alive,
+                           prefix, words, (const _Float16 *)buffers[2], params, density);
+    else if (w16)
+        hipLaunchKernelGGL((ogrid_update_kernel<float, true>), dim3(grid), dim3(kThreads), 0, s, g, a, alive, prefix,
+                           words, (const float *)buffers[2], params, density);
+    else
+        hipLaunchKernelGGL((ogrid_update_kernel<float, false>), dim3(grid), dim3(kThreads), 0, s, g, a, alive, prefix,
+                           words, (const float *)buffers[2], params, density);
     check_launch(target);
 }

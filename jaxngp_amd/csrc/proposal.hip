@@ -242,30 +242,36 @@ struct SampleArgs {
     MarchParams mp;
 };
 
-// Places ray r's samples from lane b's bin density sigma_b (one wavefront per ray,
-// lane b = bin b; lanes j < n_per_ray place sample j).  Samples whose cell the
-// occupancy grid marks empty are dropped (they would add nothing: march_rays never
-// samples such cells); the rest are packed in order at the front of the ray's slots,
-// counts[r] of them, and the remaining slots are zero.
-__device__ __forceinline__ void place_samples(const SampleArgs &sa, uint32_t r, float sigma_b, float t0, float t1,
-                                              const float *__restrict__ rays_o, const float *__restrict__ rays_d,
-                                              const uint8_t *__restrict__ bits, uint32_t *__restrict__ startidx,
-                                              uint32_t *__restrict__ counts, bool *__restrict__ fallback,
-                                              float *__restrict__ xyzs, float *__restrict__ dss,
-                                              float *__restrict__ zs) {
+// Ray r's sample plan from lane b's bin density sigma_b (one wavefront per ray,
+// lane b = bin b; lane j < n_per_ray plans sample j): its t and ds, its position,
+// and whether the occupancy grid marks its cell occupied (march_rays never samples
+// empty cells).  Quantile placement puts sample j at the (j + jitter) / n_per_ray
+// quantile of the piecewise constant density w_b / (M D): jitter 1/2 renders,
+// a per-ray uniform jitter stratifies training samples.
+struct SamplePlan {
+    float t, ds, p[3];
+    bool keep;  // this lane's sample exists and its cell is occupied
+    bool ok;    // the ray has a segment and enough predicted mass
+    bool seg;   // t1 > t0
+};
+
+__device__ __forceinline__ SamplePlan plan_samples(const SampleArgs &sa, uint32_t r, float sigma_b, float t0, float t1,
+                                                   float jitter, const float *__restrict__ rays_o,
+                                                   const float *__restrict__ rays_d, const uint8_t *__restrict__ bits) {
     const int lane = lane_id();
     const uint32_t N = sa.n_per_ray;
-    const bool seg = t1 > t0;
+    SamplePlan pl;
+    pl.seg = t1 > t0;
     const float D = (t1 - t0) / (float)kBins;
-    const float a = seg ? 1.0f - det_expf(-sigma_b * D) : 0.0f;
+    const float a = pl.seg ? 1.0f - det_expf(-sigma_b * D) : 0.0f;
     const float incl = wave_prod_scan(1.0f - a);
     float Tb = __shfl_up(incl, 1, kWave);
     if (lane == 0) Tb = 1.0f;
     const float w = Tb * a;
     const float M = wave_sumf(w);
-    const bool ok = seg && M >= sa.mass_threshold;
+    pl.ok = pl.seg && M >= sa.mass_threshold && M > 0.0f;
     float t = 0.0f, ds = 0.0f;
-    if (ok && sa.mode == NGP_PROPOSAL_SUPPORT) {
+    if (pl.ok && sa.mode == NGP_PROPOSAL_SUPPORT) {
         // bins [lo, hi]: first with cumulative share > eps, first with >= 1 - eps
         const float inc = wave_prefix_sum(w / M);
         const uint64_t above = __ballot(inc > sa.eps), full = __ballot(inc >= 1.0f - sa.eps);
@@ -275,10 +281,10 @@ __device__ __forceinline__ void place_samples(const SampleArgs &sa, uint32_t r, 
         const float t_lo = t0 + (float)lo * D, t_hi = t0 + (float)(hi + 1) * D;
         ds = (t_hi - t_lo) / (float)N;
         t = t_lo + ((float)lane + 0.5f) * ds;
-    } else if (ok) {
+    } else if (pl.ok) {
         const float p = w / M;
         const float C = wave_prefix_sum(p) - p;  // exclusive
-        const float u = ((float)lane + 0.5f) / (float)N;
+        const float u = ((float)lane + jitter) / (float)N;
         // last bin with C_b <= u (C is non-decreasing, C_0 = 0)
         int b = 0;
 #pragma unroll
@@ -293,29 +299,45 @@ __device__ __forceinline__ void place_samples(const SampleArgs &sa, uint32_t r, 
         const float tn = __shfl_down(t, 1, kWave);
         ds = (uint32_t)lane + 1 < N ? tn - t : fminf(pb > 0.0f ? D / ((float)N * pb) : D, t1 - t);
     }
-    float p[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[c] = ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
-    bool keep = false;
-    if (ok && (uint32_t)lane < N) {
+    for (int c = 0; c < 3; ++c) pl.p[c] = pl.ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
+    pl.keep = false;
+    if (pl.ok && (uint32_t)lane < N) {
         BitCache cache = make_bit_cache(bits, false);
         float gx, gy, gz, mb;
-        keep = lookup_cell(sa.mp, bits, cache, p[0], p[1], p[2], step_size(sa.mp, t), gx, gy, gz, mb);
+        pl.keep = lookup_cell(sa.mp, bits, cache, pl.p[0], pl.p[1], pl.p[2], step_size(sa.mp, t), gx, gy, gz, mb);
     }
-    const uint64_t kept = __ballot(keep);
+    pl.t = t;
+    pl.ds = ds;
+    return pl;
+}
+
+// Places ray r's samples: the kept ones packed in order at the front of the ray's
+// slots, counts[r] of them, the remaining slots zero; a ray below the mass
+// threshold gets none and fallback = true.
+__device__ __forceinline__ void place_samples(const SampleArgs &sa, uint32_t r, float sigma_b, float t0, float t1,
+                                              const float *__restrict__ rays_o, const float *__restrict__ rays_d,
+                                              const uint8_t *__restrict__ bits, uint32_t *__restrict__ startidx,
+                                              uint32_t *__restrict__ counts, bool *__restrict__ fallback,
+                                              float *__restrict__ xyzs, float *__restrict__ dss,
+                                              float *__restrict__ zs) {
+    const int lane = lane_id();
+    const uint32_t N = sa.n_per_ray;
+    const SamplePlan pl = plan_samples(sa, r, sigma_b, t0, t1, 0.5f, rays_o, rays_d, bits);
+    const uint64_t kept = __ballot(pl.keep);
     const uint32_t n_kept = (uint32_t)__popcll(kept);
     const size_t base = (size_t)r * N;
     if (lane == 0) {
         startidx[r] = (uint32_t)base;
         counts[r] = n_kept;
-        fallback[r] = seg && !ok;
+        fallback[r] = pl.seg && !pl.ok;
     }
-    if (keep) {
+    if (pl.keep) {
         const size_t j = base + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull));
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xyzs[3 * j + c] = p[c];
-        dss[j] = ds;
-        zs[j] = t;
+        for (int c = 0; c < 3; ++c) xyzs[3 * j + c] = pl.p[c];
+        dss[j] = pl.ds;
+        zs[j] = pl.t;
     }
     if ((uint32_t)lane >= n_kept && (uint32_t)lane < N) {
         const size_t j = base + lane;
@@ -340,29 +362,13 @@ __global__ void __launch_bounds__(kBlock)
                   counts, fallback, xyzs, dss, zs);
 }
 
-// Fused: lane b evaluates the proposal (encoder + MLP) at its bin centre, then the
-// ray's samples are placed as in proposal_sample.  No bin points, encodings or
-// densities go through memory.
-template <typename TT, typename PT>
-__global__ void __launch_bounds__(kBlock)
-    proposal_render_kernel(uint32_t n_rays, HashGridDescriptor g, SampleArgs sa, const float *__restrict__ rays_o,
-                           const float *__restrict__ rays_d, const float *__restrict__ t0s,
-                           const float *__restrict__ t1s, const TT *__restrict__ table, const PT *__restrict__ params,
-                           const uint8_t *__restrict__ bits, uint32_t *__restrict__ startidx,
-                           uint32_t *__restrict__ counts, bool *__restrict__ fallback, float *__restrict__ xyzs,
-                           float *__restrict__ dss, float *__restrict__ zs) {
-    __shared__ float w[kParams];
-    __shared__ LevelTables lt;
-    if (threadIdx.x < kIn / 2) {
-        lt.scale[threadIdx.x] = g.scales[threadIdx.x];
-        lt.res[threadIdx.x] = g.resolutions[threadIdx.x];
-        lt.offset[threadIdx.x] = g.offsets[threadIdx.x];
-    }
-    stage_params(w, params);  // ends with __syncthreads
-    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
-    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
-    if (r >= n_rays) return;
-    const float t0 = t0s[r], t1 = t1s[r];
+// The proposal's density at ray r's bin lane: the 5-level encoding of the bin
+// centre and the MLP, in registers (weights w and level tables lt staged in LDS).
+template <typename TT>
+__device__ __forceinline__ float bin_sigma(const HashGridDescriptor &g, const LevelTables &lt, const float *w,
+                                           const TT *__restrict__ table, const float *__restrict__ rays_o,
+                                           const float *__restrict__ rays_d, uint32_t r, float t0, float t1) {
+    const int lane = lane_id();
     const float D = (t1 - t0) / (float)kBins;
     const float tc = t0 + ((float)lane + 0.5f) * D;
     const float two_b = 2.0f * g.bound;
@@ -388,8 +394,144 @@ __global__ void __launch_bounds__(kBlock)
         x[2 * l + 1] = a1;
     }
     float h[kHid];
-    const float sigma_b = t1 > t0 ? det_expf(hidden_and_logit(w, x, h)) : 0.0f;
+    return t1 > t0 ? det_expf(hidden_and_logit(w, x, h)) : 0.0f;
+}
+
+template <typename PT>
+__device__ __forceinline__ void stage_proposal(const HashGridDescriptor &g, float *w, LevelTables &lt,
+                                               const PT *__restrict__ params) {
+    if (threadIdx.x < kIn / 2) {
+        lt.scale[threadIdx.x] = g.scales[threadIdx.x];
+        lt.res[threadIdx.x] = g.resolutions[threadIdx.x];
+        lt.offset[threadIdx.x] = g.offsets[threadIdx.x];
+    }
+    stage_params(w, params);  // ends with __syncthreads
+}
+
+// Fused: lane b evaluates the proposal (encoder + MLP) at its bin centre, then the
+// ray's samples are placed as in proposal_sample.  No bin points, encodings or
+// densities go through memory.
+template <typename TT, typename PT>
+__global__ void __launch_bounds__(kBlock)
+    proposal_render_kernel(uint32_t n_rays, HashGridDescriptor g, SampleArgs sa, const float *__restrict__ rays_o,
+                           const float *__restrict__ rays_d, const float *__restrict__ t0s,
+                           const float *__restrict__ t1s, const TT *__restrict__ table, const PT *__restrict__ params,
+                           const uint8_t *__restrict__ bits, uint32_t *__restrict__ startidx,
+                           uint32_t *__restrict__ counts, bool *__restrict__ fallback, float *__restrict__ xyzs,
+                           float *__restrict__ dss, float *__restrict__ zs) {
+    __shared__ float w[kParams];
+    __shared__ LevelTables lt;
+    stage_proposal(g, w, lt, params);
+    const int wave = (int)threadIdx.x / kWave;
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
+    if (r >= n_rays) return;
+    const float t0 = t0s[r], t1 = t1s[r];
+    const float sigma_b = bin_sigma(g, lt, w, table, rays_o, rays_d, r, t0, t1);
     place_samples(sa, r, sigma_b, t0, t1, rays_o, rays_d, bits, startidx, counts, fallback, xyzs, dss, zs);
+}
+
+// ---------------------------------------------------------------------------
+// Training samples placed by the proposal, in march_rays' output layout (rays'
+// samples contiguous in ray order, invalid rays and the unused tail empty), so the
+// rest of the training step runs unchanged on them.  Stratified: sample j of ray r
+// sits at the (j + noise_r) / n_per_ray quantile.  Three passes, no scratch:
+//   1. count: the kept samples per ray -> rays_n_samples;
+//   2. scan (one workgroup): start = exclusive prefix in ray order; a ray is valid
+//      iff t_end > t_start and its samples fit (start + n <= total_samples), as
+//      march_rays decides; next_sample_write_location = every ray's count,
+//      number_of_exceeded_samples = the counts of the rays that did not fit;
+//   3. write: each valid ray recomputes its plan and writes its kept samples at
+//      its start; the samples past the valid ones are zeroed.
+// ---------------------------------------------------------------------------
+
+template <typename TT, typename PT, bool kWrite>
+__global__ void __launch_bounds__(kBlock)
+    proposal_train_kernel(uint32_t n_rays, HashGridDescriptor g, SampleArgs sa, const float *__restrict__ rays_o,
+                          const float *__restrict__ rays_d, const float *__restrict__ t0s,
+                          const float *__restrict__ t1s, const float *__restrict__ noises,
+                          const TT *__restrict__ table, const PT *__restrict__ params, const uint8_t *__restrict__ bits,
+                          uint32_t *__restrict__ rays_n_samples, const uint32_t *__restrict__ startidx,
+                          const bool *__restrict__ valid, uint32_t *__restrict__ idcs, float *__restrict__ xyzs,
+                          float *__restrict__ dirs, float *__restrict__ dss, float *__restrict__ zs) {
+    __shared__ float w[kParams];
+    __shared__ LevelTables lt;
+    stage_proposal(g, w, lt, params);
+    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
+    if (r >= n_rays) return;
+    if (kWrite && (!valid[r] || rays_n_samples[r] == 0)) return;
+    const float t0 = t0s[r], t1 = t1s[r];
+    const float sigma_b = bin_sigma(g, lt, w, table, rays_o, rays_d, r, t0, t1);
+    const SamplePlan pl = plan_samples(sa, r, sigma_b, t0, t1, noises[r], rays_o, rays_d, bits);
+    const uint64_t kept = __ballot(pl.keep);
+    if (!kWrite) {
+        if (lane == 0) rays_n_samples[r] = (uint32_t)__popcll(kept);
+        return;
+    }
+    const uint32_t rank = (uint32_t)__popcll(kept & ((1ull << lane) - 1ull));
+    if (pl.keep && rank < rays_n_samples[r]) {  // the count pass's plan: never past the ray's range
+        const size_t j = (size_t)startidx[r] + rank;
+        idcs[j] = r;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xyzs[3 * j + c] = pl.p[c];
+            dirs[3 * j + c] = rays_d[3 * (size_t)r + c];
+        }
+        dss[j] = pl.ds;
+        zs[j] = pl.t;
+    }
+}
+
+constexpr int kTrainScanThreads = 1024;
+__global__ void __launch_bounds__(kTrainScanThreads)
+    proposal_train_scan_kernel(uint32_t n_rays, uint32_t total_samples, const float *__restrict__ t0s,
+                               const float *__restrict__ t1s, uint32_t *__restrict__ next_loc,
+                               uint32_t *__restrict__ exceeded, bool *__restrict__ valid,
+                               uint32_t *__restrict__ rays_n_samples, uint32_t *__restrict__ startidx) {
+    uint64_t carry = 0, over = 0;
+    for (uint32_t c = 0; c < n_rays; c += kTrainScanThreads) {
+        const uint32_t r = c + threadIdx.x;
+        const uint32_t n = r < n_rays ? rays_n_samples[r] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kTrainScanThreads>(n, &tot);
+        uint32_t miss = 0;
+        if (r < n_rays) {
+            const uint64_t start = carry + ex;
+            const bool ok = t1s[r] > t0s[r] && start < total_samples && start + n <= total_samples;
+            valid[r] = ok;
+            rays_n_samples[r] = ok ? n : 0u;
+            startidx[r] = ok && n > 0 ? (uint32_t)start : 0u;
+            miss = ok ? 0u : n;
+        }
+        uint32_t miss_tot;
+        block_exclusive_scan<kTrainScanThreads>(miss, &miss_tot);
+        carry += tot;
+        over += miss_tot;
+    }
+    if (threadIdx.x == 0) {
+        *next_loc = (uint32_t)min<uint64_t>(carry, 0xffffffffull);
+        *exceeded = (uint32_t)min<uint64_t>(over, 0xffffffffull);
+    }
+}
+
+// Zeroes samples [next_loc - exceeded, total_samples).
+__global__ void __launch_bounds__(kBlock)
+    proposal_train_tail_kernel(uint32_t total_samples, const uint32_t *__restrict__ next_loc,
+                               const uint32_t *__restrict__ exceeded, uint32_t *__restrict__ idcs,
+                               float *__restrict__ xyzs, float *__restrict__ dirs, float *__restrict__ dss,
+                               float *__restrict__ zs) {
+    const uint32_t measured = *next_loc - *exceeded;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t j = (size_t)measured + blockIdx.x * kBlock + threadIdx.x; j < total_samples; j += stride) {
+        idcs[j] = 0u;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xyzs[3 * j + c] = 0.0f;
+            dirs[3 * j + c] = 0.0f;
+        }
+        dss[j] = 0.0f;
+        zs[j] = 0.0f;
+    }
 }
 
 __global__ void zero_f32_kernel(float *p) {
@@ -523,22 +665,21 @@ extern "C" void proposal_sample(ngp_stream_t stream, void **buffers, const char 
     check_launch(target);
 }
 
-extern "C" void proposal_render_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
-    const char *target = "proposal_render_samples";
-    ProposalRenderDescriptor d;
-    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+// Checks a ProposalRenderDescriptor (the proposal grid, placement and marching
+// statics) and fills the sampler's arguments.
+static bool check_render_descriptor(const ProposalRenderDescriptor &d, SampleArgs &sa, const char *target) {
     const HashGridDescriptor &g = d.grid;
     const ProposalSampleDescriptor &q = d.sample;
     if (g.L != (uint32_t)(kIn / 2) || g.F != 2 || g.T == 0 || g.n_rows == 0 || g.first_hash_level > g.L ||
         g.offsets[0] != 0 || g.offsets[g.L] != g.n_rows || !(g.bound > 0.0f) || g.table_dtype > NGP_DTYPE_F16 ||
         d.param_dtype > NGP_DTYPE_F16) {
         set_error(kBadDescriptor, "%s: needs the proposal grid (L = 5, F = 2) with a consistent level table", target);
-        return;
+        return false;
     }
     for (uint32_t l = 0; l < g.L; ++l) {
         if (g.offsets[l + 1] < g.offsets[l]) {
             set_error(kBadDescriptor, "%s: offsets must be non-decreasing", target);
-            return;
+            return false;
         }
     }
     const uint32_t G = q.G;
@@ -547,16 +688,26 @@ extern "C" void proposal_render_samples(ngp_stream_t stream, void **buffers, con
         !(q.eps >= 0.0f && q.eps < 0.5f) || q.diagonal_n_steps == 0 || q.K == 0 || !g_ok || !(q.bound > 0.0f) ||
         !(q.stepsize_portion >= 0.0f)) {
         set_error(kBadDescriptor, "%s: invalid sample placement or marching statics", target);
-        return;
+        return false;
     }
-    if (q.n_rays == 0) return;
-    if (!check_buffers(buffers, 13, target)) return;
-    SampleArgs sa;
     sa.n_per_ray = q.n_per_ray;
     sa.mode = q.mode;
     sa.mass_threshold = q.mass_threshold;
     sa.eps = q.eps;
     sa.mp = make_march_params(q.diagonal_n_steps, q.K, q.G, q.bound, q.stepsize_portion);
+    return true;
+}
+
+extern "C" void proposal_render_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_render_samples";
+    ProposalRenderDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    const HashGridDescriptor &g = d.grid;
+    const ProposalSampleDescriptor &q = d.sample;
+    SampleArgs sa;
+    if (!check_render_descriptor(d, sa, target)) return;
+    if (q.n_rays == 0) return;
+    if (!check_buffers(buffers, 13, target)) return;
     const dim3 grid(div_up(q.n_rays, kBlock / kWave));
     hipStream_t s = (hipStream_t)stream;
     const float *ro = (const float *)buffers[0], *rd = (const float *)buffers[1];
@@ -582,5 +733,59 @@ extern "C" void proposal_render_samples(ngp_stream_t stream, void **buffers, con
         hipLaunchKernelGGL((proposal_render_kernel<float, float>), grid, dim3(kBlock), 0, s, q.n_rays, g, sa, ro, rd,
                            t0, t1, (const float *)buffers[4], (const float *)buffers[5], bits, start, cnt, fb, xyzs,
                            dss, zs);
+    check_launch(target);
+}
+
+extern "C" void proposal_train_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len) {
+    const char *target = "proposal_train_samples";
+    ProposalTrainDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return;
+    const HashGridDescriptor &g = d.render.grid;
+    const uint32_t n = d.render.sample.n_rays;
+    SampleArgs sa;
+    if (!check_render_descriptor(d.render, sa, target)) return;
+    if (d.render.sample.mode != NGP_PROPOSAL_QUANTILES || d.total_samples == 0) {
+        set_error(kBadDescriptor, "%s: needs quantile placement and total_samples > 0", target);
+        return;
+    }
+    if (!check_buffers(buffers, 18, target)) return;
+    hipStream_t s = (hipStream_t)stream;
+    const float *ro = (const float *)buffers[0], *rd = (const float *)buffers[1];
+    const float *t0 = (const float *)buffers[2], *t1 = (const float *)buffers[3], *noise = (const float *)buffers[4];
+    const uint8_t *bits = (const uint8_t *)buffers[7];
+    uint32_t *next_loc = (uint32_t *)buffers[8], *exceeded = (uint32_t *)buffers[9];
+    bool *valid = (bool *)buffers[10];
+    uint32_t *ns = (uint32_t *)buffers[11], *start = (uint32_t *)buffers[12], *idcs = (uint32_t *)buffers[13];
+    float *xyzs = (float *)buffers[14], *dirs = (float *)buffers[15], *dss = (float *)buffers[16];
+    float *zs = (float *)buffers[17];
+    const bool t16 = g.table_dtype == NGP_DTYPE_F16, p16 = d.render.param_dtype == NGP_DTYPE_F16;
+    const dim3 grid(std::max<uint32_t>(1, div_up(n, kBlock / kWave)));
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            hipLaunchKernelGGL(proposal_train_scan_kernel, dim3(1), dim3(kTrainScanThreads), 0, s, n, d.total_samples,
+                               t0, t1, next_loc, exceeded, valid, ns, start);
+            if (!check_launch(target)) return;
+        }
+        if (n == 0) continue;
+#define NGP_TRAIN_LAUNCH(TT, PT, W)                                                                                 \
+    hipLaunchKernelGGL((proposal_train_kernel<TT, PT, W>), grid, dim3(kBlock), 0, s, n, g, sa, ro, rd, t0, t1, noise, \
+                       (const TT *)buffers[5], (const PT *)buffers[6], bits, ns, start, valid, idcs, xyzs, dirs, dss, zs)
+        if (pass == 0) {
+            if (t16 && p16) NGP_TRAIN_LAUNCH(__half, __half, false);
+            else if (t16) NGP_TRAIN_LAUNCH(__half, float, false);
+            else if (p16) NGP_TRAIN_LAUNCH(float, __half, false);
+            else NGP_TRAIN_LAUNCH(float, float, false);
+        } else {
+            if (t16 && p16) NGP_TRAIN_LAUNCH(__half, __half, true);
+            else if (t16) NGP_TRAIN_LAUNCH(__half, float, true);
+            else if (p16) NGP_TRAIN_LAUNCH(float, __half, true);
+            else NGP_TRAIN_LAUNCH(float, float, true);
+        }
+#undef NGP_TRAIN_LAUNCH
+        if (!check_launch(target)) return;
+    }
+    const uint32_t tail_grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(d.total_samples, kBlock), 1024));
+    hipLaunchKernelGGL(proposal_train_tail_kernel, dim3(tail_grid), dim3(kBlock), 0, s, d.total_samples, next_loc,
+                       exceeded, idcs, xyzs, dirs, dss, zs);
     check_launch(target);
 }

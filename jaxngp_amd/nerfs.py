@@ -97,13 +97,15 @@ def nerf_mlp(enc: torch.Tensor, dirs: torch.Tensor, params: torch.Tensor) -> tor
 
 class NeRF(torch.nn.Module):
     """models/nerfs.py:26-86 with make_nerf_ngp's configuration (position encoder
-    L=16, T=2^19, F=2, N_min=16, N_max=2048*bound)."""
+    L=16, T=2^19, F=2, N_min=16, N_max=2048*bound).  `log2_T` sets another table
+    size per level (the precision workload also runs T = 2^22)."""
 
-    def __init__(self, bound: float = 1.0, device=None, seed: int = 0, table_dtype: torch.dtype = torch.float32):
+    def __init__(self, bound: float = 1.0, device=None, seed: int = 0, table_dtype: torch.dtype = torch.float32,
+                 log2_T: int = 19):
         super().__init__()
         self.bound = float(bound)
         gen = torch.Generator(device="cpu").manual_seed(seed)
-        self.position_encoder = HashGridEncoder(L=16, T=2 ** 19, F=2, N_min=2 ** 4, N_max=int(2 ** 11 * bound),
+        self.position_encoder = HashGridEncoder(L=16, T=2 ** log2_T, F=2, N_min=2 ** 4, N_max=int(2 ** 11 * bound),
                                                 tv_scale=0.0, param_dtype=table_dtype, generator=gen)
         flat = glorot_uniform_(torch.empty(N_PARAMS, dtype=torch.float32), generator=gen)
         self.mlp_params = torch.nn.Parameter(flat)

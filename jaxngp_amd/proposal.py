@@ -5,7 +5,11 @@ density at 64 equal bins of each ray's segment [t_start, t_end).  It is trained
 online against the NeRF's own sample weights from the occupancy-grid marcher
 (histogram-bound loss, ``proposal_loss``), and at render time places 48 samples per
 ray at the quantiles of its bin weights (``proposal_sample``); rays whose predicted
-mass is below 0.9 fall back to the occupancy-grid marcher.
+mass is below the renderer's fallback threshold (``render_image_proposal``'s
+``mass_threshold``, 0.98 by default) are rendered by the occupancy-grid marcher.
+With ``TrainConfig(proposal_guided=True)`` the field itself trains on samples the
+proposal places (``proposal_train_samples``), and the proposal on the field's
+weights at them.
 """
 from __future__ import annotations
 

@@ -28,6 +28,9 @@ from .nerfs import N_PARAMS, NeRF, mlp_workspace_blocks
 from .scene import SceneData
 
 SQRT3 = 3 ** 0.5
+# distinguishes the occupancy update's random draws from the ray generation's,
+# which share the trainer's seed and step
+OGRID_SEED_TAG = 0x6F63637570616E63
 
 
 @dataclass
@@ -53,9 +56,19 @@ class TrainConfig:
     proposal: bool = False
     proposal_rays: int = 1024
     proposal_every: int = 4  # proposal update on every n-th step
+    # train the field on samples the proposal places (proposal_train_samples:
+    # `proposal_samples_per_ray` per ray at stratified quantiles of its predicted
+    # mass, minus empty occupancy cells) instead of the occupancy-grid marcher's;
+    # the proposal then learns from the field's weights at its own samples.  Needs
+    # `proposal`.  n_rays defaults to total_samples / 32 in this mode
+    proposal_guided: bool = False
+    proposal_samples_per_ray: int = 48
     # sky-sphere background model trained with the field, for scenes marked "bg"
     # (app/nerf/_utils.py:128-133; random_bg is then ignored, as there)
     background_model: bool = False
+    # entries per hash-grid level (2^19 in make_nerf_ngp; the precision workload
+    # also trains 2^22)
+    log2_hashmap_size: int = 19
     # run the optimizer update on a side stream, overlapping the next step's ray
     # generation and march (False: on the main stream, in order)
     overlap_update: bool = True
@@ -81,6 +94,10 @@ class OccupancyGrid:
         self.bitfield = torch.full((n // 8,), 255, dtype=torch.uint8, device=device)
         self.alive = torch.arange(n, dtype=torch.int64, device=device)
         self.alive_offsets = [i * G ** 3 for i in range(K + 1)]
+        # per cascade: its alive cells' Morton indices within the cascade (u32, ascending)
+        self.alive_cas = [torch.arange(G ** 3, dtype=torch.int32, device=device) for _ in range(K)]
+        # occupied-cell prefix counts of the sampled update (u32[G^3 / 64 + 1])
+        self.prefix = torch.zeros(G ** 3 // 64 + 1, dtype=torch.int32, device=device)
 
     def mean_density_up_to_cascade(self, cas: int) -> torch.Tensor:
         return self.density[self.alive[: self.alive_offsets[cas]]].mean()
@@ -89,15 +106,6 @@ class OccupancyGrid:
 def density_threshold_from_min_step_size(diagonal_n_steps: int, bound: float) -> float:
     """utils/types.py:1367-1369."""
     return 0.01 * diagonal_n_steps / (2 * min(bound, 1) * SQRT3)
-
-
-def cell_centers(idx: torch.Tensor, G: int, cas: int, bound: float) -> torch.Tensor:
-    """World-space centres of Morton cells `idx` of cascade `cas` (types.py:1193-1197)."""
-    from .volrend import morton3d_invert
-    xyz = morton3d_invert(idx.to(torch.int32).view(torch.uint32)).view(torch.int32).float()
-    xyz = xyz / (G - 1) * 2 - 1
-    mip = min(bound, 2 ** cas)
-    return xyz * (mip - mip / G)
 
 
 class Trainer:
@@ -116,12 +124,14 @@ class Trainer:
         self.fp16 = cfg.precision == "fp16"
         if self.fp16:
             table_dtype = torch.float16
-        self._model = NeRF(bound=self.bound, device=self.dev, seed=cfg.seed % 2 ** 31, table_dtype=table_dtype)
+        self._model = NeRF(bound=self.bound, device=self.dev, seed=cfg.seed % 2 ** 31, table_dtype=table_dtype,
+                           log2_T=cfg.log2_hashmap_size)
         self.grid = OccupancyGrid(self.K, cfg.grid_res, self.dev)
         self.step_count = 0
-        self.rng = torch.Generator(device=self.dev).manual_seed(cfg.seed % 2 ** 31)
         S = cfg.total_samples
-        n = cfg.n_rays or S
+        if cfg.proposal_guided and not cfg.proposal:
+            raise ValueError("proposal_guided needs proposal=True")
+        n = cfg.n_rays or (S // 32 if cfg.proposal_guided else S)
         self.n_rays = n
         f32, u32 = torch.float32, torch.uint32
         z = lambda *shape, dtype=f32: torch.zeros(*shape, dtype=dtype, device=self.dev)
@@ -260,44 +270,29 @@ class Trainer:
         g.alive = all_idx[alive]
         counts = [int(alive[c * G3:(c + 1) * G3].sum()) for c in range(K)]
         g.alive_offsets = np.cumsum([0] + counts).tolist()
+        g.alive_cas = [(g.alive[g.alive_offsets[c]:g.alive_offsets[c + 1]] - c * G3).to(torch.int32).contiguous()
+                       for c in range(K)]
 
-    def update_ogrid_density(self, cas: int, update_all: bool, max_inference: int):
-        """types.py:1149-1225: decay alive cells by 0.95, then max with the field's
-        density at a random point of each selected cell."""
-        g, G = self.grid, self.cfg.grid_res
-        G3 = G ** 3
-        sl = slice(cas * G3, (cas + 1) * G3)
-        aligned = g.alive[g.alive_offsets[cas]:g.alive_offsets[cas + 1]] % G3
-        n_grids = aligned.shape[0]
-        if n_grids == 0:
+    def update_ogrid_density(self, cas: int, update_all: bool, max_inference: int | None = None):
+        """types.py:1149-1225 for cascade `cas`, in one fused launch
+        (occupancy_grid_update): decay the alive cells by 0.95, then max each chosen
+        cell's density with the field's density at a random point of the cell (every
+        alive cell when `update_all`, else half uniform over the alive cells and half
+        uniform over the occupied ones).  `max_inference` (the reference's chunk size
+        for its field evaluations) is accepted and not needed: the kernel evaluates
+        every chosen cell in one launch."""
+        g = self.grid
+        alive = g.alive_cas[cas]
+        if alive.numel() == 0:
             return
-        dens = g.density[sl]
-        dens[aligned] = dens[aligned] * 0.95
-        if update_all:
-            upd = aligned
-        else:
-            M = max(1, n_grids // 2)
-            h = max(1, M // 2)
-            first = aligned[torch.randint(0, n_grids, (h,), device=self.dev, generator=self.rng)]
-            # uniform choice among occupied cells (sampling proportional to the 0/1
-            # occupancy, as types.py:1180-1190), uniform among all when none is
-            # occupied; on the device, without a host sync
-            cs = torch.cumsum(g.occ_mask[sl][aligned].to(torch.int64), 0)
-            total = cs[-1]
-            u = torch.rand(h, device=self.dev, generator=self.rng, dtype=torch.float64)
-            k_occ = torch.searchsorted(cs, (u * total).floor().to(torch.int64), right=True).clamp_max(n_grids - 1)
-            k_any = (u * n_grids).floor().to(torch.int64).clamp_max(n_grids - 1)
-            second = aligned[torch.where(total > 0, k_occ, k_any)]
-            upd = torch.cat([first, second])
-        mip = min(self.bound, 2 ** cas)
-        hcw = mip / G
-        coords = cell_centers(upd, G, cas, self.bound)
-        coords = coords + (torch.rand(coords.shape, device=self.dev, generator=self.rng) * 2 - 1) * hcw
-        new = torch.empty(upd.shape[0], dtype=torch.float32, device=self.dev)
-        with torch.no_grad():
-            for a in range(0, upd.shape[0], max_inference):
-                new[a:a + max_inference] = self.model(coords[a:a + max_inference].contiguous(), None)[0].reshape(-1)
-        dens[upd] = torch.maximum(dens[upd], new)
+        m = self.model  # orders the main stream after the last optimizer update
+        enc = m.position_encoder
+        w16 = self.fp16
+        params = m.mlp_half if w16 else m.mlp_params.detach()
+        desc = _lib.occupancy_update_descriptor(
+            enc.descriptor(0, self.bound), g.G, g.K, cas, alive.numel(), update_all,
+            _lib.DTYPE_F16 if w16 else _lib.DTYPE_F32, self.cfg.seed ^ OGRID_SEED_TAG, self.step_count)
+        _lib.call("occupancy_grid_update", [alive, g.bitfield, enc.table.detach(), params, g.density, g.prefix], desc)
 
     def threshold_ogrid(self):
         """types.py:1227-1239."""
@@ -317,7 +312,7 @@ class Trainer:
         s = self.step_count
         if s > 0 and s % self.update_ogrid_interval == 0:
             for cas in range(self.K):
-                self.update_ogrid_density(cas, update_all=s < 256, max_inference=self.cfg.total_samples)
+                self.update_ogrid_density(cas, update_all=s < 256)
             self.threshold_ogrid()
 
     # -- one step -----------------------------------------------------------------------
@@ -343,13 +338,30 @@ class Trainer:
             mlp_w, pd, flags, state = params, _lib.DTYPE_F32, 0, []
         none = torch.empty(0, device=self.dev)  # a NULL mirror slot
         side = self.side_stream.cuda_stream  # the optimizer's stream (see step)
+        march_out = [b["next_loc"], b["exceeded"], b["valid"], b["ns"], b["start"], b["idcs"], b["xyzs"], b["dirs"],
+                     b["dss"], b["zs"]]
+        if c.proposal_guided:
+            # samples placed by the proposal (its parameters are updated on the main
+            # stream, so this launch is ordered after their update)
+            ppe = self.prop.position_encoder
+            pw, ppd = self.prop.weights()
+            rdesc = _lib.proposal_render_descriptor(
+                ppe.descriptor(0, self.bound),
+                _lib.proposal_sample_descriptor(n, c.proposal_samples_per_ray, 0.0, _lib.PROPOSAL_QUANTILES, 1e-3,
+                                                c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion), ppd)
+            march = P("proposal_train_samples", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], ppe.table.detach(), pw,
+                                                 self.grid.bitfield] + march_out,
+                      _lib.proposal_train_descriptor(rdesc, S))
+            bits_slot = 7
+        else:
+            march = P("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield] + march_out,
+                      _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion))
+            bits_slot = 5
         calls = dict(
             rays=P("train_make_rays", [data.transforms, data.rgba, b["o"], b["d"], b["t0"], b["t1"], b["noise"],
                                        b["bgs"], b["gt"], b["pix"]], b""),
-            march=P("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield, b["next_loc"],
-                                   b["exceeded"], b["valid"], b["ns"], b["start"], b["idcs"], b["xyzs"], b["dirs"],
-                                   b["dss"], b["zs"]],
-                    _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion)),
+            march=march,
+            bits_slot=bits_slot,
             encode=P("hashgrid_encode", [b["xyzs"], table, b["enc"]], hdesc),
             mlp=P("nerf_mlp_forward", [b["enc"], b["dirs"], mlp_w, b["drgbs"]], _lib.nerf_mlp_descriptor(S, 0, LM, pd)),
             integrate=P("integrate_rays", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"], b["measured"],
@@ -433,7 +445,7 @@ class Trainer:
         if self._calls is None:
             self._calls = self._prepare()
         k = self._calls
-        k["march"].set_buffer(5, self.grid.bitfield)
+        k["march"].set_buffer(k["bits_slot"], self.grid.bitfield)
         cam, data = self.data.camera, self.data
         bg = self._bg_model is not None
         flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg and not bg else 0)

@@ -499,6 +499,81 @@ def cpu_train_step(rays_o, rays_d, t_starts, t_ends, noises, bits, bgs, gt, tabl
 
 
 # ---------------------------------------------------------------------------
+# occupancy-grid update (utils/types.py:1149-1225).  The reference draws its cells
+# and points with JAX's PRNG; the build draws them from a counter-based splitmix64
+# stream (the same draws the kernel makes), restated here, so the chosen cells and
+# points are compared exactly and the densities against float64.
+# ---------------------------------------------------------------------------
+
+OGRID_DECAY = np.float32(0.95)  # types.py:1163
+
+
+def mix64(z):
+    """splitmix64 finaliser (Steele, Lea, Flood 2014) on uint64 arrays."""
+    z = np.asarray(z, np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def draw(seed, step, i, stream):
+    """The kernels' random draw of (seed, step, item i, stream)."""
+    i = np.asarray(i, np.uint64)
+    with np.errstate(over="ignore"):
+        a = mix64(i * np.uint64(4) + np.uint64(stream))
+        b = mix64((np.uint64(step) * np.uint64(0x9E3779B97F4A7C15)) ^ a)
+    return mix64(np.uint64(int(seed) & 0xFFFFFFFFFFFFFFFF) ^ b)
+
+
+def below(r, n):
+    """Uniform index in [0, n) from a draw: (r >> 32) * n >> 32."""
+    return ((np.asarray(r, np.uint64) >> np.uint64(32)) * np.uint64(n)) >> np.uint64(32)
+
+
+def ogrid_choice(alive, occupied, G, cascade, update_all, seed, step, bound):
+    """Cells (Morton index within the cascade, one per update slot) and their random
+    points, f32 [n, 3], of one occupancy-grid update (types.py:1165-1207).
+    `alive`: the cascade's alive cells, ascending; `occupied`: bool [G^3] of the
+    cascade (bits of the last threshold).  update_all: every alive cell once; else
+    max(1, M // 2) cells uniform over the alive ones and as many uniform over the
+    occupied ones (types.py:1172-1191), M = max(1, n_alive // 2), and uniform over
+    the alive ones when none is occupied."""
+    alive = np.asarray(alive, np.int64)
+    n_alive = alive.shape[0]
+    key0 = np.uint64(cascade) << np.uint64(32)
+    if update_all:
+        cells = alive.copy()
+    else:
+        M = max(1, n_alive // 2)
+        h = max(1, M // 2)
+        j = np.arange(2 * h, dtype=np.uint64)
+        cells = alive[below(draw(seed, step, key0 | j, 0), n_alive).astype(np.int64)]
+        occ = np.flatnonzero(np.asarray(occupied, bool))
+        if occ.size:
+            cells[h:] = occ[below(draw(seed, step, key0 | j[h:], 1), occ.size).astype(np.int64)]
+    n = cells.shape[0]
+    j = np.arange(n, dtype=np.uint64)
+    r = draw(seed, step, key0 | j, 2)
+    u = np.stack([((r >> np.uint64(s)) & np.uint64(0x1FFFFF)).astype(np.float32) for s in (43, 22, 1)], 1)
+    u = u * np.float32(1.0 / 2097152.0)
+    v = morton3d_invert(cells.astype(np.uint32)).astype(np.float32)
+    mip = np.float32(min(float(bound), 2.0 ** cascade))
+    hcw = np.float32(mip / np.float32(G))
+    c = (v / np.float32(G - 1)) * np.float32(2) - np.float32(1)
+    pts = c * (mip - hcw) + (u * np.float32(2) - np.float32(1)) * hcw
+    return cells, pts.astype(np.float32)
+
+
+def ogrid_density64(points, table, params, bound, lv, T):
+    """The field's density at `points` in float64: trunc_exp of the density MLP's
+    first output on the position encoding (models/nerfs.py:66-77)."""
+    enc = hashgrid_forward64(points, table, bound, lv, T)
+    W1, W2 = unpack_mlp(params)[:2]
+    return np.exp(np.maximum(enc @ W1, 0.0) @ W2[:, 0])
+
+
+# ---------------------------------------------------------------------------
 # Learned proposal (no counterpart in the reference; defined by this build and
 # pinned by finite differences and closed forms in tests/test_oracle_pins.py)
 # ---------------------------------------------------------------------------
@@ -621,16 +696,19 @@ def proposal_sample_support64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9,
     return counts, fallback, t, ds
 
 
-def proposal_sample64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9):
+def proposal_sample64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9, jitter=0.5):
     """Samples placed by the proposal, per ray: bin weights w_b = T_b (1 - exp(-sigma_b D))
     over 64 equal bins of [t0, t1); mass M = sum w_b; rays with t1 <= t0 get no samples,
-    rays with M < mass_threshold get none either and are marked for the occupancy-grid
-    fallback.  Otherwise sample j sits at the (j + 1/2) / n quantile of the piecewise
-    constant density w_b / (M D): t_j = t0 + (b + (u - C_b) / p_b) D with p_b = w_b / M,
-    C_b = sum_{b' < b} p_b', b the last bin with C_b <= u; ds_j = t_{j+1} - t_j and, for
-    the last sample, min(D / (n p_b), t1 - t_j).
+    rays with M < mass_threshold (or M = 0) get none either and are marked for the
+    occupancy-grid fallback.  Otherwise sample j sits at the (j + jitter) / n quantile
+    of the piecewise constant density w_b / (M D) (jitter 1/2 when rendering, the ray's
+    noise in [0, 1) for stratified training samples): t_j = t0 + (b + (u - C_b) / p_b) D
+    with p_b = w_b / M, C_b = sum_{b' < b} p_b', b the last bin with C_b <= u;
+    ds_j = t_{j+1} - t_j and, for the last sample, min(D / (n p_b), t1 - t_j).
+    `jitter` is a scalar or one value per ray.
     Returns (counts [n_rays], fallback [n_rays], t [n_rays, n], ds [n_rays, n])."""
     n_rays = len(t0)
+    jit = np.broadcast_to(np.asarray(jitter, np.float64), (n_rays,))
     sig = np.asarray(sigma_p, np.float64).reshape(n_rays, PROP_BINS)
     counts = np.zeros(n_rays, np.int64)
     fallback = np.zeros(n_rays, bool)
@@ -645,19 +723,19 @@ def proposal_sample64(t0, t1, sigma_p, n_samples=48, mass_threshold=0.9):
         T = np.concatenate([[1.0], np.cumprod(1.0 - a)[:-1]])
         w = T * a
         M = w.sum()
-        if not M >= mass_threshold:
+        if not (M >= mass_threshold and M > 0):
             fallback[r] = True
             continue
         p = w / M
         C = np.concatenate([[0.0], np.cumsum(p)[:-1]])
         for j in range(n_samples):
-            u = (j + 0.5) / n_samples
+            u = (j + jit[r]) / n_samples
             b = int(np.nonzero(C <= u)[0].max())
             frac = min(max((u - C[b]) / p[b], 0.0), 1.0) if p[b] > 0 else 0.5
             t[r, j] = a0 + (b + frac) * D
         for j in range(n_samples - 1):
             ds[r, j] = t[r, j + 1] - t[r, j]
-        bl = int(np.nonzero(C <= (n_samples - 0.5) / n_samples)[0].max())
+        bl = int(np.nonzero(C <= (n_samples - 1 + jit[r]) / n_samples)[0].max())
         ds[r, -1] = min(D / (n_samples * p[bl]) if p[bl] > 0 else D, a1 - t[r, -1])
         counts[r] = n_samples
     return counts, fallback, t, ds

@@ -60,6 +60,9 @@ CONFIGS = {
     "small_f4": dict(L=8, T=2 ** 12, F=4, N_min=4, N_max=64),
     "f1_dense_only": dict(L=3, T=2 ** 20, F=1, N_min=8, N_max=32),
     "f8": dict(L=4, T=2 ** 14, F=8, N_min=16, N_max=256),
+    # dense levels of up to 4.1 M rows: a vertex on a level's upper face addresses up
+    # to res^2 + res rows (~26 k) past the level, several 8192-row slices into the next
+    "dense_overflow": dict(L=4, T=2 ** 22, F=2, N_min=64, N_max=160),
 }
 
 
@@ -207,7 +210,7 @@ def test_backward_non_finite_reaches_its_rows(dev):
     assert (np.abs(d[ok].astype(np.float64) - want[ok]) <= 1e-5 * absw[ok] + 1e-30).all()
 
 
-def test_fp16_table_backward():
+def test_fp16_table_backward(dev):
     """f16 tables (the precision path) sort f16-rounded contributions: the gradient
     is within 1e-3 of the float64 scatter relative to each row's absolute
     contributions (f16 keeps 11 bits) plus 1e-6 absolute (contributions below f16's
@@ -215,7 +218,6 @@ def test_fp16_table_backward():
     above it), untouched rows stay 0, and a contribution past the f16 range reaches
     its rows as inf (what the loss-scale check looks for)."""
     from jaxngp_amd import _lib
-    dev = torch.device("cuda:0")
     rng = np.random.default_rng(31)
     enc = HashGridEncoder(**CONFIGS["l16_t2e19"], param_dtype=torch.float16, generator=torch.Generator().manual_seed(4)).to(dev)
     N = 30000
@@ -235,3 +237,35 @@ def test_fp16_table_backward():
     g[5, 7] = 1e6  # past 65504 once weighted by anything above 0.066
     _lib.call("hashgrid_encode_backward", [pos_t, torch.from_numpy(g).to(dev), d, ws], desc)
     assert torch.isinf(d).any() and torch.isfinite(d).float().mean() > 0.999
+
+
+def test_fp16_table_2pow22(dev):
+    """The precision workload's larger table: T = 2^22 entries per level (39,474,024
+    rows, levels 0-7 dense), f16.  Forward bit-exact against the oracle's f32
+    restatement on the f16 values; the table gradient within the f16 bound of the
+    float64 scatter (as test_fp16_table_backward), untouched rows 0."""
+    from jaxngp_amd import _lib
+    rng = np.random.default_rng(41)
+    cfg = dict(L=16, T=2 ** 22, F=2, N_min=16, N_max=2048)
+    enc = HashGridEncoder(**cfg, param_dtype=torch.float16, generator=torch.Generator().manual_seed(5)).to(dev)
+    assert enc.levels.n_rows == 39474024 and enc.levels.first_hash_level == 8
+    with torch.no_grad():
+        enc.table.copy_(torch.from_numpy(rng.uniform(-1e-2, 1e-2, tuple(enc.table.shape)).astype(np.float16)))
+    N = 2 ** 17
+    pos = rng.uniform(-1, 1, (N, 3)).astype(np.float32)
+    pos_t = torch.from_numpy(pos).to(dev)
+    print(f"hashgrid f16 T=2^22: N={N}", flush=True)
+    out, _ = enc(pos_t, 1.0)
+    table32 = enc.table.detach().float().cpu().numpy()
+    np.testing.assert_array_equal(out.detach().cpu().numpy(), _c_forward(enc, pos, table32))
+    del table32
+    g = rng.normal(size=(N, 32)).astype(np.float32)
+    desc = enc.descriptor(N, 1.0)
+    ws = torch.empty(_lib.hashgrid_backward_workspace_bytes(desc), dtype=torch.uint8, device=dev)
+    d = torch.full(tuple(enc.table.shape), 5.0, dtype=torch.float32, device=dev)
+    _lib.call("hashgrid_encode_backward", [pos_t, torch.from_numpy(g).to(dev), d, ws], desc)
+    dt = d.cpu().numpy().astype(np.float64)
+    want = _c_backward(enc, pos, g)
+    absw = _c_backward(enc, pos, np.abs(g))
+    assert (dt[absw == 0] == 0).all()
+    assert (np.abs(dt - want) <= 1e-3 * absw + 1e-6).all()

@@ -233,3 +233,113 @@ def test_proposal_render_samples_fused(dev):
     for a, b in zip(*outs):
         assert torch.equal(a, b)
     assert outs[0][1].sum() > 0 and outs[0][2].sum() > 0
+
+
+def _train_sample_inputs(dev, seed, n, table_dtype):
+    from jaxngp_amd.proposal import ProposalNet
+    from jaxngp_amd.renderer import make_near_far_from_bound
+    rng = np.random.default_rng(seed)
+    prop = ProposalNet(1.0, dev, seed=seed, table_dtype=table_dtype)
+    with torch.no_grad():
+        prop.position_encoder.table.uniform_(-1.0, 1.0)
+        prop.mlp_params.mul_(3.0)
+    if table_dtype == torch.float16:
+        prop.mlp_half = prop.mlp_params.detach().half()
+    o = (rng.normal(size=(n, 3)) * 0.3 - np.array([0, 0, 3.0])).astype(np.float32)
+    d = rng.normal(size=(n, 3)) * 0.25 + np.array([0, 0, 1.0])
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    d[:7] = [0, 1, 0]  # rays that miss the box: t_end <= t_start
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    ot, dt = T(o), T(d)
+    t0, t1 = make_near_far_from_bound(1.0, ot, dt)
+    bits, _ = blob_bitfield(rng, n_blobs=40)
+    return rng, prop, ot, dt, t0, t1, T(bits)
+
+
+def _train_samples(prop, ot, dt, t0, t1, noise, bt, N, S):
+    dev = ot.device
+    n = ot.shape[0]
+    u32 = torch.uint32
+    out = dict(next_loc=torch.empty(1, dtype=u32, device=dev), exceeded=torch.empty(1, dtype=u32, device=dev),
+               valid=torch.empty(n, dtype=torch.bool, device=dev), ns=torch.empty(n, dtype=u32, device=dev),
+               start=torch.empty(n, dtype=u32, device=dev), idcs=torch.full((S,), 7, dtype=u32, device=dev),
+               xyzs=torch.full((S, 3), 7.0, device=dev), dirs=torch.full((S, 3), 7.0, device=dev),
+               dss=torch.full((S,), 7.0, device=dev), zs=torch.full((S,), 7.0, device=dev))
+    w, pd = prop.weights()
+    rdesc = _lib.proposal_render_descriptor(prop.position_encoder.descriptor(0, 1.0),
+                                            _lib.proposal_sample_descriptor(n, N, 0.0), pd)
+    _lib.call("proposal_train_samples", [ot, dt, t0, t1, noise, prop.position_encoder.table.detach(), w, bt] +
+              list(out.values()), _lib.proposal_train_descriptor(rdesc, S))
+    torch.cuda.synchronize()
+    return {k: v.cpu() for k, v in out.items()}
+
+
+@pytest.mark.parametrize("table_dtype,fill", [(torch.float32, 0.6), (torch.float16, 1.1)])
+def test_proposal_train_samples_are_packed_render_samples(dev, table_dtype, fill):
+    """With the jitter at 1/2 for every ray and no mass threshold, the training
+    sampler places exactly proposal_render_samples' kept samples, packed in ray order
+    in march_rays' layout: start = exclusive prefix of the counts, a ray is valid iff
+    t_end > t_start and its samples fit, idcs = the ray, dirs = its direction, the
+    tail zero, next_sample_write_location = every ray's count and
+    number_of_exceeded_samples = those of the rays that did not fit.  The slots hold
+    `fill` times the rays' samples (0.6: the later rays do not fit)."""
+    n, N = 3000, 48
+    rng, prop, ot, dt, t0, t1, bt = _train_sample_inputs(dev, 21, n, table_dtype)
+    start = torch.empty(n, dtype=torch.uint32, device=dev)
+    cnt, fb = torch.empty_like(start), torch.empty(n, dtype=torch.bool, device=dev)
+    xyz, ds, z = torch.empty(n, N, 3, device=dev), torch.empty(n, N, device=dev), torch.empty(n, N, device=dev)
+    w, pd = prop.weights()
+    _lib.call("proposal_render_samples", [ot, dt, t0, t1, prop.position_encoder.table.detach(), w, bt, start, cnt, fb,
+                                          xyz, ds, z],
+              _lib.proposal_render_descriptor(prop.position_encoder.descriptor(0, 1.0),
+                                              _lib.proposal_sample_descriptor(n, N, 0.0), pd))
+    c = cnt.view(torch.int32).cpu().numpy().astype(np.int64)
+    S = int(fill * c.sum())
+    print(f"proposal_train_samples: rays={n} S={S} table={table_dtype}", flush=True)
+    got = _train_samples(prop, ot, dt, t0, t1, torch.full((n,), 0.5, device=dev), bt, N, S)
+    seg = (t1 > t0).cpu().numpy()
+    assert (~seg).sum() >= 7 and c[seg].sum() > 4 * n
+    pre = np.concatenate([[0], np.cumsum(c)[:-1]])
+    valid = seg & (pre < S) & (pre + c <= S)
+    assert valid.sum() > n // 4 and (valid[seg].all() if fill > 1 else not valid[seg].all())
+    np.testing.assert_array_equal(got["valid"].numpy(), valid)
+    np.testing.assert_array_equal(got["ns"].view(torch.int32).numpy(), np.where(valid, c, 0))
+    np.testing.assert_array_equal(got["start"].view(torch.int32).numpy(), np.where(valid & (c > 0), pre, 0))
+    assert int(got["next_loc"].view(torch.int32)) == c.sum()
+    assert int(got["exceeded"].view(torch.int32)) == c[~valid].sum()
+    m = int(c[valid].sum())
+    rays = np.repeat(np.arange(n), np.where(valid, c, 0))
+    slot = np.concatenate([np.arange(k) for k in np.where(valid, c, 0)])
+    np.testing.assert_array_equal(got["idcs"].view(torch.int32).numpy()[:m], rays)
+    np.testing.assert_array_equal(got["xyzs"].numpy()[:m], xyz.cpu().numpy()[rays, slot])
+    np.testing.assert_array_equal(got["dirs"].numpy()[:m], dt.cpu().numpy()[rays])
+    np.testing.assert_array_equal(got["dss"].numpy()[:m], ds.cpu().numpy()[rays, slot])
+    np.testing.assert_array_equal(got["zs"].numpy()[:m], z.cpu().numpy()[rays, slot])
+    for k in ("idcs", "xyzs", "dirs", "dss", "zs"):
+        assert (got[k][m:] == 0).all(), k
+
+
+def test_proposal_train_samples_stratified(dev):
+    """Per-ray jitter: sample j sits at the (j + noise) / n quantile of the ray's
+    predicted mass (the oracle's float64 placement with the bin densities of the
+    unfused chain), before empty cells are dropped (all cells occupied here)."""
+    n, N, S = 2000, 48, 1 << 17
+    rng, prop, ot, dt, t0, t1, _ = _train_sample_inputs(dev, 22, n, torch.float32)
+    full = torch.full((128 ** 3 // 8,), 255, dtype=torch.uint8, device=dev)
+    noise = rng.uniform(0, 1, n).astype(np.float32)
+    got = _train_samples(prop, ot, dt, t0, t1, torch.from_numpy(noise).to(dev), full, N, S)
+    sig = prop.bin_density(ot, dt, t0, t1).cpu().numpy()
+    T0, T1 = t0.cpu().numpy(), t1.cpu().numpy()
+    want_c, _, want_t, want_ds = oracle.proposal_sample64(T0, T1, sig, N, 0.0, jitter=noise)
+    ns = got["ns"].view(torch.int32).numpy()
+    valid = got["valid"].numpy()
+    np.testing.assert_array_equal(ns[valid], want_c[valid])
+    assert valid.sum() > n // 2
+    st = got["start"].view(torch.int32).numpy()
+    zt = got["zs"].numpy()
+    for r in np.nonzero(valid & (ns > 0))[0]:
+        span = T1[r] - T0[r]
+        assert (np.abs(zt[st[r]:st[r] + ns[r]] - want_t[r]) <= 2e-5 * span + 1e-6).all(), r
+    # jitter moves samples: not the render placement
+    rend = oracle.proposal_sample64(T0, T1, sig, N, 0.0)[2]
+    assert np.abs(want_t[valid] - rend[valid]).max() > 1e-3

@@ -256,6 +256,52 @@ def test_integrate_rays_2pow20_samples_properties(dev):
     np.testing.assert_allclose(N(rgbd)[sub], rgbd64, rtol=1e-4, atol=2e-5)
 
 
+def _c2_inputs(rng, n_rays, counts):
+    """The differentiation workload: rays of `counts` samples, densities softplus(N(0,2)),
+    rgb and bg U[0,1], ds = 2 sqrt(3) / 1024, z = 3 + k ds."""
+    ns = np.asarray(counts, np.uint32)
+    start = np.zeros(n_rays, np.uint32)
+    start[1:] = np.cumsum(ns)[:-1]
+    S = int(ns.sum())
+    ds = np.float32(2 * 1.7320508075688772 / 1024)
+    dss = np.full(S, ds, np.float32)
+    k = np.arange(S) - np.repeat(start.astype(np.int64), ns)
+    z = (np.float32(3) + k.astype(np.float32) * ds).astype(np.float32)
+    sig = np.log1p(np.exp(rng.normal(0, 2, S))).astype(np.float32)
+    drgbs = np.concatenate([sig[:, None], rng.uniform(0, 1, (S, 3)).astype(np.float32)], 1)
+    bgs = rng.uniform(0, 1, (n_rays, 3)).astype(np.float32)
+    return start, ns, bgs, dss, z, drgbs
+
+
+@pytest.mark.parametrize("n_rays,max_samples,full", [(4096, 1024, False), (1024, 1024, True)])
+def test_integrate_rays_backward_differentiation_workload(dev, n_rays, max_samples, full):
+    """4096 rays x U{1..1024} samples (seed 4; 2,119,089 samples), and 1024 full rays
+    (2^20 samples): forward and the reference VJP (dbg, dz, ddrgbs) bit for bit
+    against the oracle."""
+    rng = np.random.default_rng(4)
+    counts = np.full(n_rays, max_samples) if full else rng.integers(1, max_samples + 1, n_rays)
+    start, ns, bgs, dss, z, drgbs = _c2_inputs(rng, n_rays, counts)
+    print(f"integrate c2: rays={n_rays} samples={int(ns.sum())}", flush=True)
+    m_o, rgbd_o, op_o = oracle.integrate_rays(start, ns, bgs, dss, z, drgbs)
+    drgbs_t = T(drgbs, dev).requires_grad_(True)
+    bgs_t = T(bgs, dev).requires_grad_(True)
+    z_t = T(z, dev).requires_grad_(True)
+    m, rgbd, op = volrend.integrate_rays(0.3, U(start, dev), U(ns, dev), bgs_t, T(dss, dev), z_t, drgbs_t)
+    assert int(N(m.view(torch.int32))) == int(m_o)
+    np.testing.assert_array_equal(N(rgbd.detach()), rgbd_o)
+    np.testing.assert_array_equal(N(op.detach()), op_o)
+    # at these densities no ray reaches T <= 1e-4 within 1024 samples (opacity <= 0.981):
+    # every ray keeps its background term; test_integrate_rays_bit_exact_and_backward
+    # covers terminated rays
+    assert (op_o < 1 - 1e-4).all() and op_o.max() > 0.9
+    g = rng.normal(size=(n_rays, 4)).astype(np.float32)
+    rgbd.backward(T(g, dev))
+    dbg_o, dz_o, dd_o = oracle.integrate_rays_backward(0.3, start, ns, bgs, dss, z, drgbs, rgbd_o, op_o, g)
+    np.testing.assert_array_equal(N(bgs_t.grad), dbg_o)
+    np.testing.assert_array_equal(N(z_t.grad), dz_o)
+    np.testing.assert_array_equal(N(drgbs_t.grad), dd_o)
+
+
 def test_integrate_rays_inference_bit_exact(dev):
     rng = np.random.default_rng(11)
     n_total, n, cap = 3000, 512, 8

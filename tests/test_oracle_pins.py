@@ -814,6 +814,30 @@ def test_proposal_sample_quantiles():
     assert np.abs(ds[0, :-1] - 2.0 / n).max() <= 1e-9
 
 
+def test_proposal_sample_stratified_jitter():
+    """Stratified training samples: with equal bin weights, sample j of a ray with
+    jitter u sits at t0 + (j + u) / n (t1 - t0); jitter 1/2 is the render placement;
+    a zero-mass ray with threshold 0 gets no samples (its fallback flag set) instead
+    of dividing by zero."""
+    n = 48
+    w = np.full(64, 1.0 / 64)
+    T = 1.0 - np.concatenate([[0.0], np.cumsum(w)[:-1]])
+    a = np.minimum(w / T, 1.0)
+    D = 2.0 / 64
+    sig_u = -np.log1p(-np.clip(a, 0, 1 - 1e-15)) / D
+    jit = np.array([0.0, 0.25, 0.5, 0.999])
+    k = len(jit)
+    counts, fb, t, ds = oracle.proposal_sample64(np.full(k, 2.0), np.full(k, 4.0), np.tile(sig_u, (k, 1)), n, 0.0,
+                                                 jitter=jit)
+    assert list(counts) == [n] * k and not fb.any()
+    want = 2.0 + (np.arange(n)[None, :] + jit[:, None]) / n * 2.0
+    assert np.abs(t - want).max() <= 1e-9
+    assert np.abs(ds[:, :-1] - 2.0 / n).max() <= 1e-9
+    np.testing.assert_array_equal(t[2], oracle.proposal_sample64(np.array([2.0]), np.array([4.0]), sig_u[None], n)[2][0])
+    c0, fb0, _, _ = oracle.proposal_sample64(np.array([2.0]), np.array([4.0]), np.zeros((1, 64)), n, 0.0)
+    assert c0[0] == 0 and fb0[0]
+
+
 def test_proposal_sample_support():
     """Support placement: equal bin weights cover the whole segment, so the samples are
     evenly spaced over [t0, t1); an opaque first bin confines them to that bin."""
