@@ -108,8 +108,8 @@ def bench_encoder(dev, reps: int = 50) -> dict:
     from jaxngp_amd import _lib
     from jaxngp_amd.encoders import HashGridEncoder
     N = 1 << 18
-    enc = HashGridEncoder(L=16, T=1 << 19, F=2, N_min=16, N_max=2048, device=dev,
-                          generator=torch.Generator(device="cpu").manual_seed(1))
+    enc = HashGridEncoder(L=16, T=1 << 19, F=2, N_min=16, N_max=2048,
+                          generator=torch.Generator(device="cpu").manual_seed(1)).to(dev)
     xyz = torch.from_numpy(np.random.default_rng(0).uniform(0, 1, (N, 3)).astype(np.float32)).to(dev)
     pos = (2 * xyz - 1).contiguous()
     g = torch.from_numpy(np.random.default_rng(2).normal(size=(N, 32)).astype(np.float32)).to(dev)

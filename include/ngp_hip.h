@@ -487,13 +487,18 @@ typedef struct ProposalTrainDescriptor {
                                         (0 in training: every ray gets its samples) and
                                         the marching statics; render.sample.n_rays rays */
     uint32_t total_samples;          /* S, as march_rays */
+    uint32_t n_uniform;              /* U: samples added per ray, stratified uniformly
+                                        over [t_start, t_end); n_per_ray + U <= 64 */
 } ProposalTrainDescriptor;
 
 /* Training samples placed by the proposal instead of the occupancy-grid marcher
  * (no counterpart in the reference), in march_rays' buffer layout so the rest of
  * the step runs unchanged: per ray, the proposal's 64 bin densities give n_per_ray
  * samples at the (j + noise) / n_per_ray quantiles of its predicted mass
- * (stratified), those in cells the occupancy grid marks empty are dropped, and the
+ * (stratified), plus n_uniform samples at t_start + (i + noise) (t_end - t_start) / U
+ * (so the field also sees where the proposal predicts no mass), sorted, each with
+ * ds = the distance to the next sample (the last: to t_end, at most the uniform
+ * spacing); those in cells the occupancy grid marks empty are dropped, and the
  * rays' kept samples are packed in ray order; a ray is valid iff t_end > t_start
  * and its samples fit the S slots after those of the rays before it.
  * next_sample_write_location = the samples every ray asked for,

@@ -327,9 +327,9 @@ def proposal_render_descriptor(grid_desc: bytes, sample_desc: bytes, param_dtype
     return grid_desc + sample_desc + struct.pack("<I", param_dtype)
 
 
-def proposal_train_descriptor(render_desc: bytes, total_samples) -> bytes:
-    """ProposalTrainDescriptor: ProposalRenderDescriptor, u32 total_samples."""
-    return render_desc + struct.pack("<I", total_samples)
+def proposal_train_descriptor(render_desc: bytes, total_samples, n_uniform=0) -> bytes:
+    """ProposalTrainDescriptor: ProposalRenderDescriptor, u32 total_samples, u32 n_uniform."""
+    return render_desc + struct.pack("<2I", total_samples, n_uniform)
 
 
 def proposal_loss_descriptor(n_rays, eps=1e-7) -> bytes:

@@ -215,10 +215,18 @@ struct BwdPlan {
     uint64_t counts_off, sums_off, meta_off, parts_off, part_slice_off, partials_off, entries_off, bytes;
 };
 
+// Rows of one slice: R rows of F f64 accumulators fill kSliceBytes of LDS.  A power
+// of two, so the kernels split a row into (slice, row in slice) with shifts.
+template <int F>
+constexpr uint32_t slice_rows() {
+    static_assert((kSliceBytes / (8 * F) & (kSliceBytes / (8 * F) - 1)) == 0, "slice rows: a power of two");
+    return kSliceBytes / (8 * F);
+}
+
 inline BwdPlan make_plan(const HashGridDescriptor &d) {
     auto align = [](uint64_t x) { return (x + 255) / 256 * 256; };
     BwdPlan p;
-    p.R = std::max<uint32_t>(64, kSliceBytes / (8 * d.F));  // one f64 accumulator per row and feature
+    p.R = kSliceBytes / (8 * d.F);  // slice_rows<F>()
     p.n_slices = div_up(d.n_rows, p.R);
     p.n_chunks = std::max<uint32_t>(1, div_up(d.n_points, kChunk));
     p.n_items = d.L * p.n_chunks;
@@ -357,12 +365,13 @@ __device__ __forceinline__ void item_contributions(const HashGridDescriptor &d, 
 template <int F>
 __global__ void __launch_bounds__(kItemThreads)
     bwd_count_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos, uint32_t *__restrict__ counts) {
+    constexpr uint32_t R = slice_rows<F>();
     __shared__ uint32_t hist[kMaxSlices];
     for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) hist[s] = 0u;
     __syncthreads();
     const uint32_t item = blockIdx.x;
     item_contributions<F, false>(d, p, item, pos, nullptr,
-                                 [&](uint32_t row, const float *, int) { atomicAdd(&hist[row / p.R], 1u); });
+                                 [&](uint32_t row, const float *, int) { atomicAdd(&hist[row / R], 1u); });
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) counts[(size_t)s * p.n_items + item] = hist[s];
 }
@@ -459,17 +468,18 @@ __global__ void __launch_bounds__(kItemThreads)
     bwd_scatter_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos, const float *__restrict__ dL_denc,
                        const uint32_t *__restrict__ offsets, uint32_t *__restrict__ entries) {
     constexpr int EW = entry_words<F, kHalf>();
+    constexpr uint32_t R = slice_rows<F>();
     __shared__ uint32_t cursor[kMaxSlices];
     const uint32_t item = blockIdx.x;
     for (uint32_t s = threadIdx.x; s < p.n_slices; s += kItemThreads) cursor[s] = offsets[(size_t)s * p.n_items + item];
     __syncthreads();
     item_contributions<F, true>(d, p, item, pos, dL_denc, [&](uint32_t row, const float *v, int) {
-        const uint32_t s = row / p.R;
+        const uint32_t s = row / R;
         const uint32_t at = atomicAdd(&cursor[s], 1u);
         uint32_t w[EW];
 #pragma unroll
         for (int k = 0; k < EW; ++k) w[k] = 0u;
-        w[0] = row - s * p.R;
+        w[0] = row - s * R;
         pack_values<F, kHalf>(w, v);
         uint4 *e = reinterpret_cast<uint4 *>(entries + (size_t)at * EW);
         if constexpr (EW == 2) {
@@ -515,6 +525,7 @@ __global__ void __launch_bounds__(kStageThreads)
                               const float *__restrict__ dL_denc, const uint32_t *__restrict__ offsets,
                               uint32_t *__restrict__ entries) {
     constexpr int EW = entry_words<F, kHalf>();
+    constexpr uint32_t R = slice_rows<F>();
     static_assert(EW < kStageWords, "entry fits the staging slot");
     __shared__ uint32_t cursor[kStageSlices];  // next global index of each slice's run
     __shared__ uint32_t a[kStageSlices];       // phase count, then cursor - first staging slot
@@ -522,8 +533,8 @@ __global__ void __launch_bounds__(kStageThreads)
     extern __shared__ uint4 stage[];           // [kStagePhase * 8] (destination, entry words)
     const uint32_t item = blockIdx.x;
     const uint32_t l = item / p.n_chunks, k = item % p.n_chunks;
-    const uint32_t s_lo = d.offsets[l] / p.R;
-    const uint32_t span = level_last_row(d, l) / p.R - s_lo + 1u;  // <= kStageSlices (host check)
+    const uint32_t s_lo = d.offsets[l] / R;
+    const uint32_t span = level_last_row(d, l) / R - s_lo + 1u;  // <= kStageSlices (host check)
     for (uint32_t s = threadIdx.x; s < kStageSlices; s += kStageThreads) {
         cursor[s] = s < span ? offsets[(size_t)(s_lo + s) * p.n_items + item] : 0u;
         a[s] = 0u;
@@ -534,16 +545,24 @@ __global__ void __launch_bounds__(kStageThreads)
     const float scale = d.scales[l];
     const uint32_t res = d.resolutions[l];
     const bool dense = l < d.first_hash_level;
+    // every phase's inputs are loaded up front, so only the first phase waits on memory
+    float xyz_all[kStagePhases][3], gin_all[kStagePhases][F];
+#pragma unroll
+    for (int ph = 0; ph < kStagePhases; ++ph) {
+        const uint32_t n = k * kChunk + ph * kStagePhase + threadIdx.x;
+        const uint32_t np = n < d.n_points ? n : d.n_points - 1;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) xyz_all[ph][q] = pos[3 * (size_t)np + q];
+        const size_t gb = level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
+#pragma unroll
+        for (int f = 0; f < F; ++f) gin_all[ph][f] = dL_denc[gb + f];
+    }
+#pragma unroll
     for (int ph = 0; ph < kStagePhases; ++ph) {
         const uint32_t n = k * kChunk + ph * kStagePhase + threadIdx.x;
         const bool active = n < d.n_points;
-        const uint32_t np = active ? n : d.n_points - 1;
-        float xyz[3], gin[F];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) xyz[q] = pos[3 * (size_t)np + q];
-        const size_t gb = level_major ? ((size_t)l * d.n_points + np) * F : (size_t)np * d.L * F + (size_t)l * F;
-#pragma unroll
-        for (int f = 0; f < F; ++f) gin[f] = dL_denc[gb + f];
+        const float *xyz = xyz_all[ph];
+        const float *gin = gin_all[ph];
         uint32_t rows[8];
         float vals[8][F];
         uint32_t mask = 0;
@@ -555,7 +574,7 @@ __global__ void __launch_bounds__(kStageThreads)
         });
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-            if ((mask >> c) & 1u) atomicAdd(&a[rows[c] / p.R - s_lo], 1u);
+            if ((mask >> c) & 1u) atomicAdd(&a[rows[c] / R - s_lo], 1u);
         __syncthreads();
         // exclusive scan of the counts over the level's slices, two per thread
         const uint32_t c0 = a[2 * threadIdx.x], c1 = a[2 * threadIdx.x + 1];
@@ -569,11 +588,11 @@ __global__ void __launch_bounds__(kStageThreads)
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             if ((mask >> c) & 1u) {
-                const uint32_t sl = rows[c] / p.R;
+                const uint32_t sl = rows[c] / R;
                 const uint32_t s = sl - s_lo;
                 const uint32_t q = atomicAdd(&place[s], 1u);
                 uint32_t w[kStageWords] = {0u, 0u, 0u, 0u};
-                w[0] = rows[c] - sl * p.R;
+                w[0] = rows[c] - sl * R;
                 pack_values<F, kHalf>(w, vals[c]);
                 stage[q] = make_uint4(a[s] + q, w[0], w[1], w[2]);
             }

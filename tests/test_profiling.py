@@ -61,3 +61,25 @@ def test_counters_average_per_dispatch(tmp_path):
     assert lines[0].split()[-2:] == ["SQ_BUSY_CYCLES", "SQ_VALU_MFMA_BUSY_CYCLES"]
     assert lines[1].split() == ["mlp_backward_kernel", "2", "200", "40"]
     assert len(lines) == 2  # kernels outside the library are left out
+
+
+def test_gaps_counts_overlap_once_and_idle_time(tmp_path):
+    """Per-step idle time from a kernel trace: overlapping dispatches (two streams)
+    count once, and the idle rest is the span minus that."""
+    import csv
+    from jaxngp_amd import profiling
+    rows, t = [], 0
+    for _ in range(4):
+        rows.append(("void ngp::(anonymous namespace)::make_rays_kernel(int)", t, t + 10_000))
+        rows.append(("march_emit_kernel", t + 15_000, t + 65_000))
+        rows.append(("adam_kernel<float>", t + 40_000, t + 80_000))  # overlaps the march by 25 us
+        t += 100_000
+    f = tmp_path / "trace.csv"
+    with open(f, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writerows(rows)
+    out = profiling.gaps(str(f))
+    # span 100 us; busy 10 + (15..80 = 65) = 75 us; idle 25 us
+    assert "median span 100.0 us" in out and "kernels running 75.0 us" in out and "idle 25.0 us" in out
+    assert "3 dispatches per step" in out
