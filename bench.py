@@ -493,29 +493,59 @@ def main() -> None:
         out["steady_state"] = steady_state(tr)
         out["inference"] = bench_inference(tr, data, args.frames, dev)
     if not args.no_proposal:
-        # the learned-proposal path on the precision path: the field trains on 48
-        # proposal-placed samples per ray (the proposal on the field's weights at them)
-        # and renders them (occupancy-grid fallback); `proposal_fp16_marcher_trained`:
-        # the same proposal trained beside the marcher-trained field
-        for key, guided in (("proposal_fp16", True), ("proposal_fp16_marcher_trained", False)):
-            if not guided and args.no_marcher_proposal:
-                continue
-            rp = run_training(data, TrainConfig(precision="fp16", proposal=True, proposal_guided=guided), args.steps,
-                              args.warmup, dev)
+        # the learned-proposal path on the precision path.  `proposal_fp16`: the field
+        # trains on proposal-placed samples (32 at the proposal's quantiles + 32
+        # stratified uniform per ray, occupancy threshold for that sample spacing) and
+        # renders with 48 proposal samples per ray (occupancy-grid fallback);
+        # `proposal_fp16_marcher_trained`: the marcher-trained FP16 field with the same
+        # proposal beside it; `plain_fp32_at_guided_threshold`: the plain FP32 path with
+        # the guided path's occupancy threshold, the control for the threshold
+        from jaxngp_amd.train import Trainer
+        runs = [("proposal_fp16", TrainConfig(precision="fp16", proposal=True, proposal_guided=True))]
+        if not args.no_marcher_proposal:
+            runs.append(("proposal_fp16_marcher_trained", TrainConfig(precision="fp16", proposal=True)))
+        guided_cap = Trainer.threshold_cap_of(runs[0][1], float(data.scene.bound))
+        runs.append(("plain_fp32_at_guided_threshold", TrainConfig(density_threshold=guided_cap)))
+        for key, pcfg in runs:
+            rp = run_training(data, pcfg, args.steps, args.warmup, dev)
             tp = rp["trainer"]
+            m = tp.metrics()
             out[key] = {"value": rp["samples"] / rp["elapsed"], "unit": "samples/s",
                         "ms_per_step": rp["ms"], "rays_per_s": rp["valid"] / rp["elapsed"],
                         "n_rays": tp.n_rays, "samples_per_step": rp["samples"] / args.steps,
-                        "loss": tp.metrics()["loss"], "proposal_loss": tp.metrics()["proposal_loss"],
-                        "loss_at_step": tp.step_count}
+                        "density_threshold_cap": tp.density_threshold_cap,
+                        "loss": m["loss"], "loss_at_step": tp.step_count}
+            if pcfg.proposal:
+                out[key]["proposal_loss"] = m["proposal_loss"]
+            if pcfg.proposal_guided:
+                out[key].update(guided_samples_per_ray=pcfg.proposal_samples_per_ray,
+                                uniform_samples_per_ray=pcfg.proposal_uniform_samples)
             if args.frames > 0:
                 train_to(tp, args.render_steps)
                 out[key]["steady_state"] = steady_state(tp)
-                out[key]["inference"] = bench_inference(tp, data, args.frames, dev, proposal=True)
+                out[key]["inference"] = bench_inference(tp, data, args.frames, dev, proposal=pcfg.proposal)
                 if "inference" in out:
                     out[key]["inference"]["delta_psnr_vs_plain_fp32_db"] = (
                         out[key]["inference"]["psnr_db"] - out["inference"]["psnr_db"])
             del rp, tp
+        if args.frames > 0 and "steady_state" in out:
+            # the target's ratios, steady state (after the render_steps of training)
+            pf, pl = out["proposal_fp16"], out
+            ctl = out["plain_fp32_at_guided_threshold"]
+            out["target"] = {
+                "train_samples_per_s_ratio": pf["steady_state"]["value"] / pl["steady_state"]["value"],
+                "train_rays_per_s_ratio": pf["steady_state"]["rays_per_s"] / pl["steady_state"]["rays_per_s"],
+                "frame_ms_ratio": pf["inference"]["ms_per_frame"] / pl["inference"]["ms_per_frame"],
+                "delta_psnr_db": pf["inference"]["psnr_db"] - pl["inference"]["psnr_db"],
+                "same_field_delta_psnr_db": pf["inference"]["delta_psnr_db"],
+                "vs_plain_at_guided_threshold": {
+                    "train_samples_per_s_ratio": pf["steady_state"]["value"] / ctl["steady_state"]["value"],
+                    "train_rays_per_s_ratio": pf["steady_state"]["rays_per_s"] / ctl["steady_state"]["rays_per_s"],
+                    "frame_ms_ratio": pf["inference"]["ms_per_frame"] / ctl["inference"]["ms_per_frame"],
+                    "delta_psnr_db": pf["inference"]["psnr_db"] - ctl["inference"]["psnr_db"]},
+                "note": "plain = the reference configuration (occupancy threshold cap 0.01 * 1024 / (2 sqrt 3)); "
+                        "the guided path's cap follows the same formula with its 64 samples per ray",
+            }
     if not args.no_fp16:
         # the precision path on the same scene: f16 table and MLP weights, loss scaling
         r16 = run_training(data, TrainConfig(precision="fp16"), args.steps, args.warmup, dev)

@@ -207,12 +207,37 @@ __device__ __forceinline__ float entry_value(const uint32_t *w, int f) {
     }
 }
 
+// The last row level l's vertices can address.  A dense level's vertex
+// coordinates reach res (a point on the upper face), so its rows run up to
+// res + res^2 + res^3 past the level's offset, into the next level's rows, as the
+// reference's indices do; hashed rows stay below offset + T.  Clamped to the table.
+__host__ __device__ inline uint32_t level_last_row(const HashGridDescriptor &d, uint32_t l) {
+    const uint64_t res = d.resolutions[l];
+    uint64_t local = (uint64_t)d.T - 1u;
+    if (l < d.first_hash_level) local = std::min<uint64_t>(local, res + res * res + res * res * res);
+    return (uint32_t)std::min<uint64_t>((uint64_t)d.offsets[l] + local, (uint64_t)d.n_rows - 1u);
+}
+
+// The one-pass table gradient (F <= 2, every level's rows within kStageSlices
+// slices): work item = (level, kBucketPoints consecutive points).
+constexpr int kBucketThreads = 512;
+constexpr int kBucketPhases = 2;
+constexpr uint32_t kBucketPoints = kBucketThreads * kBucketPhases;
+constexpr uint32_t kBucketEntries = kBucketPoints * 8;  // entries one item may write
+constexpr uint32_t kStageSlices = 1024;                 // slices one level may span
+
 struct BwdPlan {
     uint32_t R, n_slices, n_chunks, n_items;
     uint64_t M;             // n_slices * n_items counters
     uint64_t scan_blocks;   // blocks of the counter scan
     uint32_t max_parts;     // upper bound of the reduce grid
     uint64_t counts_off, sums_off, meta_off, parts_off, part_slice_off, partials_off, entries_off, bytes;
+    // one-pass path: items of kBucketPoints points, each level's slices
+    // [s_lo[l], s_hi[l]], per item and slice of its level the run's start and length
+    bool bucket;
+    uint32_t n_bchunks, n_bitems, span_stride;
+    uint32_t s_lo[NGP_HASHGRID_MAX_LEVELS], s_hi[NGP_HASHGRID_MAX_LEVELS];
+    uint64_t run_cnt_off, run_off_off, slice_np_off;
 };
 
 // Rows of one slice: R rows of F f64 accumulators fill kSliceBytes of LDS.  A power
@@ -225,9 +250,36 @@ constexpr uint32_t slice_rows() {
 
 inline BwdPlan make_plan(const HashGridDescriptor &d) {
     auto align = [](uint64_t x) { return (x + 255) / 256 * 256; };
-    BwdPlan p;
+    BwdPlan p{};
     p.R = kSliceBytes / (8 * d.F);  // slice_rows<F>()
     p.n_slices = div_up(d.n_rows, p.R);
+    const int ew = entry_words_host(d.F, d.table_dtype == NGP_DTYPE_F16);
+    p.bucket = d.F <= 2;
+    p.span_stride = 1;
+    for (uint32_t l = 0; l < d.L; ++l) {
+        p.s_lo[l] = d.offsets[l] / p.R;
+        p.s_hi[l] = level_last_row(d, l) / p.R;
+        const uint32_t span = p.s_hi[l] - p.s_lo[l] + 1u;
+        if (span > kStageSlices) p.bucket = false;
+        p.span_stride = std::max(p.span_stride, span);
+    }
+    if (p.bucket) {
+        p.n_bchunks = std::max<uint32_t>(1, div_up(d.n_points, kBucketPoints));
+        p.n_bitems = d.L * p.n_bchunks;
+        const uint64_t runs = (uint64_t)p.n_bitems * p.span_stride;
+        const uint64_t max_entries = (uint64_t)p.n_bitems * kBucketEntries;
+        p.max_parts = (uint32_t)(p.n_slices + max_entries / kPart);
+        p.run_cnt_off = 0;
+        p.run_off_off = align(p.run_cnt_off + 4 * runs);
+        p.slice_np_off = align(p.run_off_off + 4 * runs);
+        p.meta_off = align(p.slice_np_off + 4 * (uint64_t)p.n_slices);
+        p.parts_off = align(p.meta_off + 8);
+        p.part_slice_off = align(p.parts_off + 4 * ((uint64_t)p.n_slices + 1));
+        p.partials_off = align(p.part_slice_off + 4 * (uint64_t)p.max_parts);
+        p.entries_off = align(p.partials_off + (uint64_t)p.max_parts * p.R * d.F * 4);
+        p.bytes = p.entries_off + max_entries * 4 * ew;
+        return p;
+    }
     p.n_chunks = std::max<uint32_t>(1, div_up(d.n_points, kChunk));
     p.n_items = d.L * p.n_chunks;
     p.M = (uint64_t)p.n_slices * p.n_items;
@@ -235,7 +287,6 @@ inline BwdPlan make_plan(const HashGridDescriptor &d) {
     const uint64_t max_entries = (uint64_t)d.n_points * d.L * 8;
     // a slice with c entries takes max(1, ceil(c / kPart)) parts
     p.max_parts = (uint32_t)(p.n_slices + max_entries / kPart);
-    const int ew = entry_words_host(d.F, d.table_dtype == NGP_DTYPE_F16);
     p.counts_off = 0;
     p.sums_off = align(p.counts_off + 4 * p.M);
     p.meta_off = align(p.sums_off + 4 * p.scan_blocks);        // [0] total entries, [1] total parts
@@ -493,63 +544,40 @@ __global__ void __launch_bounds__(kItemThreads)
     });
 }
 
-// The last row level l's vertices can address.  A dense level's vertex
-// coordinates reach res (a point on the upper face), so its rows run up to
-// res + res^2 + res^3 past the level's offset, into the next level's rows, as the
-// reference's indices do; hashed rows stay below offset + T.  Clamped to the table.
-__host__ __device__ inline uint32_t level_last_row(const HashGridDescriptor &d, uint32_t l) {
-    const uint64_t res = d.resolutions[l];
-    uint64_t local = (uint64_t)d.T - 1u;
-    if (l < d.first_hash_level) local = std::min<uint64_t>(local, res + res * res + res * res * res);
-    return (uint32_t)std::min<uint64_t>((uint64_t)d.offsets[l] + local, (uint64_t)d.n_rows - 1u);
-}
 
-// 3 (F <= 2).  The same entries, staged in LDS first: each phase of kStagePhase
-// points (one per thread) counts its entries per slice, scans the counts over the
-// level's slices and places every entry in LDS in slice order, then writes each
-// slice's run of the item with consecutive stores.  A wavefront's stores then
-// cover a few contiguous runs instead of up to 64 scattered rows.  Every slice's
-// run gets exactly the entries bwd_scatter_kernel writes there (the same runs, in
-// another order within the run).
-constexpr int kStageThreads = 512;
-constexpr int kStagePhase = kStageThreads;  // points per phase
-constexpr int kStagePhases = kChunk / kStagePhase;
-constexpr uint32_t kStageSlices = 1024;     // slices one level may span
-constexpr int kStageWords = 4;              // destination index + entry (EW <= 3), 16 bytes
-constexpr size_t kStageBytes = (size_t)kStagePhase * 8 * kStageWords * 4;
-static_assert(kChunk % kStagePhase == 0 && kStageSlices == 2 * kStageThreads, "staged scatter shape");
-
+// One-pass table gradient, part 1 (F <= 2).  Block = item (level l, kBucketPoints
+// consecutive points, one per thread and phase).  The item's run-merged
+// contributions are computed once and kept in registers; their count per slice of
+// the level goes to LDS, an exclusive scan of the counts places each slice's run
+// in the item's own region of the entry buffer (kBucketEntries entries), and the
+// run's start and length are stored for the reducers.  Within a run, entries are in
+// LDS-atomic order; the reducer's sums do not depend on it beyond f64 rounding.
 template <int F, bool kHalf>
-__global__ void __launch_bounds__(kStageThreads)
-    bwd_scatter_staged_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos,
-                              const float *__restrict__ dL_denc, const uint32_t *__restrict__ offsets,
-                              uint32_t *__restrict__ entries) {
+__global__ void __launch_bounds__(kBucketThreads)
+    bwd_bucket_kernel(HashGridDescriptor d, BwdPlan p, const float *__restrict__ pos,
+                      const float *__restrict__ dL_denc, uint32_t *__restrict__ run_cnt,
+                      uint32_t *__restrict__ run_off, uint32_t *__restrict__ entries) {
     constexpr int EW = entry_words<F, kHalf>();
     constexpr uint32_t R = slice_rows<F>();
-    static_assert(EW < kStageWords, "entry fits the staging slot");
-    __shared__ uint32_t cursor[kStageSlices];  // next global index of each slice's run
-    __shared__ uint32_t a[kStageSlices];       // phase count, then cursor - first staging slot
-    __shared__ uint32_t place[kStageSlices];   // next staging slot of each slice
-    extern __shared__ uint4 stage[];           // [kStagePhase * 8] (destination, entry words)
+    static_assert(kStageSlices == 2 * kBucketThreads, "two slice counters per thread");
+    __shared__ uint32_t cnt[kStageSlices];
+    __shared__ uint32_t cur[kStageSlices];
     const uint32_t item = blockIdx.x;
-    const uint32_t l = item / p.n_chunks, k = item % p.n_chunks;
-    const uint32_t s_lo = d.offsets[l] / R;
-    const uint32_t span = level_last_row(d, l) / R - s_lo + 1u;  // <= kStageSlices (host check)
-    for (uint32_t s = threadIdx.x; s < kStageSlices; s += kStageThreads) {
-        cursor[s] = s < span ? offsets[(size_t)(s_lo + s) * p.n_items + item] : 0u;
-        a[s] = 0u;
-    }
-    __syncthreads();
+    const uint32_t l = item / p.n_bchunks, k = item % p.n_bchunks;
+    const uint32_t s_lo = p.s_lo[l];
+    const uint32_t span = p.s_hi[l] - s_lo + 1u;  // <= kStageSlices (host check)
+    cnt[2 * threadIdx.x] = 0u;
+    cnt[2 * threadIdx.x + 1] = 0u;
     const bool pow2 = (d.T & (d.T - 1u)) == 0;
     const bool level_major = d.enc_layout == NGP_ENC_LEVEL_MAJOR;
     const float scale = d.scales[l];
     const uint32_t res = d.resolutions[l];
     const bool dense = l < d.first_hash_level;
-    // every phase's inputs are loaded up front, so only the first phase waits on memory
-    float xyz_all[kStagePhases][3], gin_all[kStagePhases][F];
+    // both phases' inputs are loaded up front, so only the first phase waits on memory
+    float xyz_all[kBucketPhases][3], gin_all[kBucketPhases][F];
 #pragma unroll
-    for (int ph = 0; ph < kStagePhases; ++ph) {
-        const uint32_t n = k * kChunk + ph * kStagePhase + threadIdx.x;
+    for (int ph = 0; ph < kBucketPhases; ++ph) {
+        const uint32_t n = k * kBucketPoints + ph * kBucketThreads + threadIdx.x;
         const uint32_t np = n < d.n_points ? n : d.n_points - 1;
 #pragma unroll
         for (int q = 0; q < 3; ++q) xyz_all[ph][q] = pos[3 * (size_t)np + q];
@@ -557,62 +585,112 @@ __global__ void __launch_bounds__(kStageThreads)
 #pragma unroll
         for (int f = 0; f < F; ++f) gin_all[ph][f] = dL_denc[gb + f];
     }
+    __syncthreads();  // counters cleared
+    uint32_t rows[kBucketPhases][8];
+    float vals[kBucketPhases][8][F];
+    uint32_t mask[kBucketPhases];
 #pragma unroll
-    for (int ph = 0; ph < kStagePhases; ++ph) {
-        const uint32_t n = k * kChunk + ph * kStagePhase + threadIdx.x;
-        const bool active = n < d.n_points;
-        const float *xyz = xyz_all[ph];
-        const float *gin = gin_all[ph];
-        uint32_t rows[8];
-        float vals[8][F];
-        uint32_t mask = 0;
-        point_runs<F, true>(d, l, scale, res, dense, pow2, xyz, gin, active, [&](uint32_t row, const float *v, int c) {
-            rows[c] = row;
+    for (int ph = 0; ph < kBucketPhases; ++ph) {
+        const uint32_t n = k * kBucketPoints + ph * kBucketThreads + threadIdx.x;
+        uint32_t m = 0;
+        point_runs<F, true>(d, l, scale, res, dense, pow2, xyz_all[ph], gin_all[ph], n < d.n_points,
+                            [&](uint32_t row, const float *v, int c) {
+                                rows[ph][c] = row;
 #pragma unroll
-            for (int f = 0; f < F; ++f) vals[c][f] = v[f];
-            mask |= 1u << c;
-        });
+                                for (int f = 0; f < F; ++f) vals[ph][c][f] = v[f];
+                                m |= 1u << c;
+                            });
+        mask[ph] = m;
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-            if ((mask >> c) & 1u) atomicAdd(&a[rows[c] / R - s_lo], 1u);
-        __syncthreads();
-        // exclusive scan of the counts over the level's slices, two per thread
-        const uint32_t c0 = a[2 * threadIdx.x], c1 = a[2 * threadIdx.x + 1];
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan<kStageThreads>(c0 + c1, &total);
-        place[2 * threadIdx.x] = ex;
-        place[2 * threadIdx.x + 1] = ex + c0;
-        a[2 * threadIdx.x] = cursor[2 * threadIdx.x] - ex;
-        a[2 * threadIdx.x + 1] = cursor[2 * threadIdx.x + 1] - (ex + c0);
-        __syncthreads();
+            if ((m >> c) & 1u) atomicAdd(&cnt[rows[ph][c] / R - s_lo], 1u);
+    }
+    __syncthreads();
+    const uint32_t c0 = cnt[2 * threadIdx.x], c1 = cnt[2 * threadIdx.x + 1];
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<kBucketThreads>(c0 + c1, &total);
+    cur[2 * threadIdx.x] = ex;
+    cur[2 * threadIdx.x + 1] = ex + c0;
+    const size_t rb = (size_t)item * p.span_stride;
+    if (2 * threadIdx.x < span) {
+        run_cnt[rb + 2 * threadIdx.x] = c0;
+        run_off[rb + 2 * threadIdx.x] = ex;
+    }
+    if (2 * threadIdx.x + 1 < span) {
+        run_cnt[rb + 2 * threadIdx.x + 1] = c1;
+        run_off[rb + 2 * threadIdx.x + 1] = ex + c0;
+    }
+    __syncthreads();
+    uint32_t *region = entries + (size_t)item * kBucketEntries * EW;
+#pragma unroll
+    for (int ph = 0; ph < kBucketPhases; ++ph) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            if ((mask >> c) & 1u) {
-                const uint32_t sl = rows[c] / R;
-                const uint32_t s = sl - s_lo;
-                const uint32_t q = atomicAdd(&place[s], 1u);
-                uint32_t w[kStageWords] = {0u, 0u, 0u, 0u};
-                w[0] = rows[c] - sl * R;
-                pack_values<F, kHalf>(w, vals[c]);
-                stage[q] = make_uint4(a[s] + q, w[0], w[1], w[2]);
+            if ((mask[ph] >> c) & 1u) {
+                const uint32_t sl = rows[ph][c] / R;
+                const uint32_t at = atomicAdd(&cur[sl - s_lo], 1u);
+                uint32_t w[EW];
+#pragma unroll
+                for (int q = 0; q < EW; ++q) w[q] = 0u;
+                w[0] = rows[ph][c] - sl * R;
+                pack_values<F, kHalf>(w, vals[ph][c]);
+                uint32_t *dst = region + (size_t)at * EW;
+                if constexpr (EW == 2) {
+                    *reinterpret_cast<uint2 *>(dst) = make_uint2(w[0], w[1]);
+                } else {
+                    static_assert(EW == 3, "one-pass entries are 2 or 3 words");
+                    *reinterpret_cast<uint3 *>(dst) = make_uint3(w[0], w[1], w[2]);
+                }
             }
         }
-        __syncthreads();
-        for (uint32_t q = threadIdx.x; q < total; q += kStageThreads) {
-            const uint4 e = stage[q];
-            uint32_t *dst = entries + (size_t)e.x * EW;
-            if constexpr (EW == 2) {
-                *reinterpret_cast<uint2 *>(dst) = make_uint2(e.y, e.z);
-            } else {
-                *reinterpret_cast<uint3 *>(dst) = make_uint3(e.y, e.z, e.w);
-            }
+    }
+}
+
+// One-pass table gradient, part 2: parts of slice s (kPart entries each, at least
+// one), from the run lengths of every item whose level reaches s.
+__global__ void __launch_bounds__(256)
+    bwd_slice_parts_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ run_cnt,
+                           uint32_t *__restrict__ slice_np) {
+    const uint32_t s = blockIdx.x;
+    uint32_t tot = 0;
+    for (uint32_t l = 0; l < d.L; ++l) {
+        if (s < p.s_lo[l] || s > p.s_hi[l]) continue;
+        const uint32_t sl = s - p.s_lo[l];
+        for (uint32_t k = threadIdx.x; k < p.n_bchunks; k += 256)
+            tot += run_cnt[(size_t)(l * p.n_bchunks + k) * p.span_stride + sl];
+    }
+    __shared__ uint32_t ws[256 / kWave];
+    tot = wave_sum(tot);
+    if (lane_id() == 0) ws[threadIdx.x / kWave] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < 256 / kWave; ++w) t += ws[w];
+        slice_np[s] = t == 0 ? 1u : (t + kPart - 1) / kPart;
+    }
+}
+
+// One-pass table gradient, part 3 (one workgroup): prefix of the parts per slice and
+// the slice of every part; parts[n_slices] and meta[1] hold the total.
+__global__ void __launch_bounds__(kScanThreads) bwd_bucket_plan_kernel(BwdPlan p, const uint32_t *__restrict__ slice_np,
+                                                                       uint32_t *__restrict__ meta,
+                                                                       uint32_t *__restrict__ parts,
+                                                                       uint32_t *__restrict__ part_slice) {
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < p.n_slices; c += kScanThreads) {
+        const uint32_t s = c + threadIdx.x;
+        const uint32_t k = s < p.n_slices ? slice_np[s] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan<kScanThreads>(k, &tot);
+        if (s < p.n_slices) {
+            parts[s] = carry + ex;
+            for (uint32_t j = 0; j < k; ++j) part_slice[carry + ex + j] = s;
         }
-        __syncthreads();
-        for (uint32_t s = threadIdx.x; s < kStageSlices; s += kStageThreads) {
-            cursor[s] = a[s] + place[s];
-            a[s] = 0u;
-        }
-        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        parts[p.n_slices] = carry;
+        meta[1] = carry;
     }
 }
 
@@ -692,6 +770,92 @@ __global__ void __launch_bounds__(kReduceThreads)
     for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) out[t] = (float)acc[(t % F) * p.R + t / F];
 }
 
+// One-pass table gradient, part 4: workgroup b sums part b of slice s, as
+// bwd_reduce_kernel does, from the runs of the items whose level reaches s.  The
+// items are visited in (level, chunk) order and a part takes those whose exclusive
+// prefix of run lengths falls in [part * kPart, (part + 1) * kPart): a contiguous
+// range of items per tile of kReduceThreads items.  The selected runs are read as
+// one flat sequence, kU entries per thread in flight; a flat index finds its run by
+// binary search over the runs' starting prefixes in LDS.
+template <int F, bool kHalf>
+__global__ void __launch_bounds__(kReduceThreads)
+    bwd_reduce_runs_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ meta,
+                           const uint32_t *__restrict__ parts, const uint32_t *__restrict__ part_slice,
+                           const uint32_t *__restrict__ run_cnt, const uint32_t *__restrict__ run_off,
+                           const uint32_t *__restrict__ entries, float *__restrict__ partials,
+                           float *__restrict__ dtable) {
+    constexpr int EW = entry_words<F, kHalf>();
+    constexpr uint32_t R = slice_rows<F>();
+    constexpr int kU = 8;
+    extern __shared__ double acc[];
+    // selected runs: entry base (item region + run offset) and starting prefix
+    __shared__ uint64_t sel_base[kReduceThreads];
+    __shared__ uint32_t sel_start[kReduceThreads];
+    __shared__ uint32_t sel_lo, sel_hi;
+    const uint32_t b = blockIdx.x;
+    if (b >= meta[1]) return;
+    const uint32_t s = part_slice[b];
+    const uint32_t part = b - parts[s];
+    const uint32_t n_parts = parts[s + 1] - parts[s];
+    const uint32_t rows = min(d.n_rows - s * R, R);
+    for (uint32_t t = threadIdx.x; t < R * F; t += kReduceThreads) acc[t] = 0.0;
+    __syncthreads();
+    uint32_t carry = 0;  // entries of the items visited before
+    for (uint32_t l = 0; l < d.L; ++l) {
+        if (s < p.s_lo[l] || s > p.s_hi[l]) continue;
+        const uint32_t sl = s - p.s_lo[l];
+        for (uint32_t k0 = 0; k0 < p.n_bchunks; k0 += kReduceThreads) {
+            const uint32_t k = k0 + threadIdx.x;
+            const uint32_t item = l * p.n_bchunks + k;
+            uint32_t c = 0, off = 0;
+            if (k < p.n_bchunks) {
+                c = run_cnt[(size_t)item * p.span_stride + sl];
+                off = run_off[(size_t)item * p.span_stride + sl];
+            }
+            uint32_t tot, n_sel;
+            const uint32_t ex = block_exclusive_scan<kReduceThreads>(c, &tot);
+            const bool sel = c > 0 && (carry + ex) / kPart == part;
+            const uint32_t at = block_exclusive_scan<kReduceThreads>(sel ? 1u : 0u, &n_sel);
+            if (sel) {
+                sel_base[at] = (uint64_t)item * kBucketEntries + off;
+                sel_start[at] = carry + ex;
+                if (at == 0) sel_lo = carry + ex;
+                if (at == n_sel - 1) sel_hi = carry + ex + c;
+            }
+            carry += tot;
+            __syncthreads();
+            if (n_sel > 0) {
+                const uint32_t lo = sel_lo, hi = sel_hi;
+                for (uint32_t e0 = lo; e0 < hi; e0 += kU * kReduceThreads) {
+                    uint32_t w[kU][EW];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const uint32_t e = min(e0 + u * kReduceThreads + threadIdx.x, hi - 1u);
+                        // last selected run starting at or before e
+                        uint32_t j = 0;
+#pragma unroll
+                        for (uint32_t step = kReduceThreads / 2; step > 0; step >>= 1)
+                            if (j + step < n_sel && sel_start[j + step] <= e) j += step;
+                        load_entry<EW>(entries + (sel_base[j] + (e - sel_start[j])) * EW, w[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const bool in = e0 + u * kReduceThreads + threadIdx.x < hi;
+#pragma unroll
+                        for (int f = 0; f < F; ++f) {
+                            const float v = entry_value<F, kHalf>(w[u], f);
+                            if (in && v != 0.0f) atomicAdd(&acc[f * R + w[u][0]], (double)v);
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // the selection is rewritten by the next tile
+        }
+    }
+    float *out = n_parts == 1 ? dtable + (size_t)s * R * F : partials + (size_t)b * R * F;
+    for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) out[t] = (float)acc[(t % F) * R + t / F];
+}
+
 // 5. slices of several parts: rows = sum of the parts' partial rows, in part
 // order.  Block (s, c) covers 1/kCombineSplit of slice s.
 constexpr uint32_t kCombineSplit = 16;
@@ -729,28 +893,41 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// The staged scatter takes entries of at most 3 words (F <= 2) and levels whose
-// rows span at most kStageSlices slices.
-inline bool staged_scatter_fits(const HashGridDescriptor &d, const BwdPlan &p) {
-    if (d.F > 2) return false;
-    for (uint32_t l = 0; l < d.L; ++l) {
-        if (level_last_row(d, l) / p.R - d.offsets[l] / p.R + 1u > kStageSlices) return false;
-    }
-    return true;
-}
-
 template <int F, bool kHalf>
 bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const float *pos, const float *g, float *dtable,
                      uint8_t *ws, const char *target) {
     const BwdPlan p = make_plan(d);
-    uint32_t *counts = reinterpret_cast<uint32_t *>(ws + p.counts_off);
-    uint32_t *sums = reinterpret_cast<uint32_t *>(ws + p.sums_off);
     uint32_t *meta = reinterpret_cast<uint32_t *>(ws + p.meta_off);
     uint32_t *parts = reinterpret_cast<uint32_t *>(ws + p.parts_off);
     uint32_t *part_slice = reinterpret_cast<uint32_t *>(ws + p.part_slice_off);
     float *partials = reinterpret_cast<float *>(ws + p.partials_off);
     uint32_t *entries = reinterpret_cast<uint32_t *>(ws + p.entries_off);
     const size_t lds = (size_t)p.R * F * sizeof(double);
+    if constexpr (F <= 2) {
+        if (p.bucket) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_reduce_runs_kernel<F, kHalf>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) {
+                set_error(kLaunchFailed, "%s: cannot reserve %zu bytes of LDS: %s", target, lds, hipGetErrorString(e));
+                return false;
+            }
+            uint32_t *run_cnt = reinterpret_cast<uint32_t *>(ws + p.run_cnt_off);
+            uint32_t *run_off = reinterpret_cast<uint32_t *>(ws + p.run_off_off);
+            uint32_t *slice_np = reinterpret_cast<uint32_t *>(ws + p.slice_np_off);
+            hipLaunchKernelGGL((bwd_bucket_kernel<F, kHalf>), dim3(p.n_bitems), dim3(kBucketThreads), 0, stream, d, p,
+                               pos, g, run_cnt, run_off, entries);
+            hipLaunchKernelGGL(bwd_slice_parts_kernel, dim3(p.n_slices), dim3(256), 0, stream, d, p, run_cnt, slice_np);
+            hipLaunchKernelGGL(bwd_bucket_plan_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, slice_np, meta, parts,
+                               part_slice);
+            hipLaunchKernelGGL((bwd_reduce_runs_kernel<F, kHalf>), dim3(p.max_parts), dim3(kReduceThreads), lds, stream,
+                               d, p, meta, parts, part_slice, run_cnt, run_off, entries, partials, dtable);
+            hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices, kCombineSplit), dim3(256), 0, stream, d, p,
+                               parts, partials, dtable);
+            return true;
+        }
+    }
+    uint32_t *counts = reinterpret_cast<uint32_t *>(ws + p.counts_off);
+    uint32_t *sums = reinterpret_cast<uint32_t *>(ws + p.sums_off);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_reduce_kernel<F, kHalf>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) {
@@ -764,26 +941,10 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
                        sums, meta);
     hipLaunchKernelGGL(plan_parts_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, counts, meta, parts,
                        part_slice);
-    bool staged = false;
-    if constexpr (entry_words<F, kHalf>() < kStageWords) staged = staged_scatter_fits(d, p);
-    if (staged) {
-        if constexpr (entry_words<F, kHalf>() < kStageWords) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_scatter_staged_kernel<F, kHalf>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStageBytes);
-        if (e != hipSuccess) {
-            set_error(kLaunchFailed, "%s: cannot reserve %zu bytes of LDS: %s", target, kStageBytes,
-                      hipGetErrorString(e));
-            return false;
-        }
-        hipLaunchKernelGGL((bwd_scatter_staged_kernel<F, kHalf>), dim3(p.n_items), dim3(kStageThreads), kStageBytes,
-                           stream, d, p, pos, g, counts, entries);
-        }
-    } else {
-        hipLaunchKernelGGL((bwd_scatter_kernel<F, kHalf>), dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos,
-                           g, counts, entries);
-    }
-    hipLaunchKernelGGL((bwd_reduce_kernel<F, kHalf>), dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts, meta,
-                       parts, part_slice, entries, partials, dtable);
+    hipLaunchKernelGGL((bwd_scatter_kernel<F, kHalf>), dim3(p.n_items), dim3(kItemThreads), 0, stream, d, p, pos, g,
+                       counts, entries);
+    hipLaunchKernelGGL((bwd_reduce_kernel<F, kHalf>), dim3(p.max_parts), dim3(kReduceThreads), lds, stream, d, p, counts,
+                       meta, parts, part_slice, entries, partials, dtable);
     hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices, kCombineSplit), dim3(256), 0, stream, d, p, parts,
                        partials, dtable);
     return true;

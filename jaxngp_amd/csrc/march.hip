@@ -150,6 +150,39 @@ __device__ __forceinline__ bool chain_in_binade(float base, float k, int lane, f
     return true;
 }
 
+// Lane j's element of the window starting at `base` (c_j of the chain from base) and
+// the element after the window (c_64); `stall` where the chain stops growing.
+__device__ __forceinline__ void chain_window(const MarchParams &p, bool const_step, float k_step, float base, int lane,
+                                             float &w, float &next, bool &stall) {
+    w = base;
+    next = base;
+    stall = false;
+    if (const_step && chain_in_binade(base, k_step, lane, w, next)) return;  // 64 elements at once
+    float c = base;
+    if (const_step) {
+#pragma unroll
+        for (int q = 0; q < kWave; ++q) {
+            if (lane == q) w = c;
+            const float nx = c + k_step;
+            stall |= nx == c;
+            c = nx;
+        }
+    } else {
+#pragma unroll 8
+        for (int q = 0; q < kWave; ++q) {
+            if (lane == q) w = c;
+            const float nx = c + step_size(p, c);
+            stall |= nx == c;
+            c = nx;
+        }
+    }
+    next = c;
+}
+
+// Windows of 64 chain elements a marching wavefront looks up together: their
+// bitfield loads are all in flight before the first window is walked.
+constexpr int kMarchWindows = 4;
+
 // Counts the samples of one ray with the whole wavefront, starting at t_start,
 // taking at most cap samples (kWrite: also records their t).  Call with all 64
 // lanes active.
@@ -158,39 +191,121 @@ __device__ uint32_t march_ray_wave(const MarchParams &p, const uint8_t *__restri
                                    float t_start, uint32_t cap, const RaySamples &o) {
     const int lane = lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
-    BitCache cache = make_bit_cache(bits, word_ok);
     // with stepsize_portion 0 every step is the same constant, for every t
     const bool const_step = p.portion == 0.0f;
     const float k_step = step_size(p, 0.0f);
+    const uint64_t *__restrict__ words = reinterpret_cast<const uint64_t *>(bits);
     uint32_t n = 0;
-    float base = t_start;  // sequence element of lane 0
+    float base = t_start;  // sequence element of lane 0 of the next window
     bool pending = false;  // a skip is searching for the first element >= thr
     float thr = 0.0f;
     while (n < cap) {
-        float w = base, c = base;
+        // up to kMarchWindows windows: stop after a window that reaches t_end, and
+        // before one whose chain stalls (finished on one lane below)
+        float w[kMarchWindows];
+        int nw = 0;
         bool stall = false;
-        if (const_step && chain_in_binade(base, k_step, lane, w, c)) {
-            // 64 elements at once
-        } else if (const_step) {
 #pragma unroll
-            for (int q = 0; q < kWave; ++q) {
-                if (lane == q) w = c;
-                const float nx = c + k_step;
-                stall |= nx == c;
-                c = nx;
-            }
-        } else {
-#pragma unroll 8
-            for (int q = 0; q < kWave; ++q) {
-                if (lane == q) w = c;
-                const float nx = c + step_size(p, c);
-                stall |= nx == c;
-                c = nx;
+        for (int q = 0; q < kMarchWindows; ++q) {
+            if (nw == q && !stall) {
+                float wq, next;
+                bool st;
+                chain_window(p, const_step, k_step, base, lane, wq, next, st);
+                if (__ballot(st) != 0ull) {
+                    stall = true;
+                } else {
+                    w[q] = wq;
+                    nw = q + 1;
+                    base = next;
+                    // the window's last element at or past t_end: later windows are dead
+                    if (!(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(wq), kWave - 1)) < r.t1)) break;
+                }
             }
         }
-        if (__ballot(stall) != 0ull) {
+        // cells of every window, the bitfield loads issued together
+        uint32_t idx[kMarchWindows];
+        float gx[kMarchWindows], gy[kMarchWindows], gz[kMarchWindows], mb[kMarchWindows];
+        bool live[kMarchWindows], occ[kMarchWindows];
+#pragma unroll
+        for (int q = 0; q < kMarchWindows; ++q) {
+            live[q] = q < nw && w[q] < r.t1;
+            idx[q] = 0u;
+            if (live[q]) {
+                const float px = r.ox + w[q] * r.dx, py = r.oy + w[q] * r.dy, pz = r.oz + w[q] * r.dz;
+                idx[q] = cell_index(p, px, py, pz, step_size(p, w[q]), gx[q], gy[q], gz[q], mb[q]);
+            }
+        }
+        if (word_ok) {
+            uint64_t wd[kMarchWindows];
+#pragma unroll
+            for (int q = 0; q < kMarchWindows; ++q) wd[q] = live[q] ? words[idx[q] >> 6] : 0ull;
+#pragma unroll
+            for (int q = 0; q < kMarchWindows; ++q) occ[q] = live[q] && ((wd[q] >> (idx[q] & 63u)) & 1ull);
+        } else {
+            uint32_t by[kMarchWindows];
+#pragma unroll
+            for (int q = 0; q < kMarchWindows; ++q) by[q] = live[q] ? bits[idx[q] >> 3] : 0u;
+#pragma unroll
+            for (int q = 0; q < kMarchWindows; ++q) occ[q] = live[q] && ((by[q] >> (idx[q] & 7u)) & 1u);
+        }
+        bool done = false;
+#pragma unroll
+        for (int q = 0; q < kMarchWindows; ++q) {
+            if (q < nw && !done) {
+                float ex = 0.0f;
+                if (live[q] && !occ[q]) {
+                    const float px = r.ox + w[q] * r.dx, py = r.oy + w[q] * r.dy, pz = r.oz + w[q] * r.dz;
+                    ex = cell_exit_t(p, w[q], px, py, pz, gx[q], gy[q], gz[q], mb[q], r.dx, r.dy, r.dz, r.idx, r.idy,
+                                     r.idz);
+                }
+                const uint64_t live_m = __ballot(live[q]);
+                const uint64_t occ_m = __ballot(occ[q]);
+                const uint32_t n0 = n;
+                uint64_t emit = 0ull;
+                int cur = 0;
+                while (cur < kWave) {
+                    if (pending) {
+                        // skip_to stops at the first element with !(t < thr)
+                        const uint64_t m = __ballot(!(w[q] < thr)) & (~0ull << cur);
+                        if (m == 0ull) {
+                            // the target lies past this window; if the window already
+                            // reached t_end, so does the target and the ray is done
+                            done = ((live_m >> (kWave - 1)) & 1ull) == 0ull;
+                            break;
+                        }
+                        cur = __builtin_ctzll(m);
+                        pending = false;
+                    }
+                    if (((live_m >> cur) & 1ull) == 0ull) {
+                        done = true;
+                        break;
+                    }
+                    if ((occ_m >> cur) & 1ull) {
+                        const uint64_t from = occ_m >> cur;
+                        const int run = ~from == 0ull ? kWave - cur : __builtin_ctzll(~from);
+                        const uint32_t take = min((uint32_t)run, cap - n);
+                        emit |= (take >= 64u ? ~0ull : ((1ull << take) - 1ull)) << cur;
+                        n += take;
+                        if (n >= cap) {
+                            done = true;
+                            break;
+                        }
+                        cur += run;
+                    } else {
+                        thr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex), cur));
+                        pending = true;
+                        cur += 1;
+                    }
+                }
+                if (kWrite && ((emit >> lane) & 1ull)) put_t(o, n0 + (uint32_t)__popcll(emit & below), w[q]);
+            }
+        }
+        if (done) break;
+        if (stall) {
+            // the chain stops growing at or after `base`: the serial loop on one lane
             uint32_t nn = n;
             if (lane == 0) {
+                BitCache cache = make_bit_cache(bits, word_ok);
                 float t = base;
                 if (pending) {
                     // rest of skip_to: t is the first step past the skipped element
@@ -207,59 +322,6 @@ __device__ uint32_t march_ray_wave(const MarchParams &p, const uint8_t *__restri
             }
             return (uint32_t)__builtin_amdgcn_readlane((int)nn, 0);
         }
-        const bool live = w < r.t1;
-        const float px = r.ox + w * r.dx, py = r.oy + w * r.dy, pz = r.oz + w * r.dz;
-        const float ds = step_size(p, w);
-        bool occ = false;
-        float ex = 0.0f;
-        if (live) {
-            float gx, gy, gz, mb;
-            occ = lookup_cell(p, bits, cache, px, py, pz, ds, gx, gy, gz, mb);
-            if (!occ) ex = cell_exit_t(p, w, px, py, pz, gx, gy, gz, mb, r.dx, r.dy, r.dz, r.idx, r.idy, r.idz);
-        }
-        const uint64_t live_m = __ballot(live);
-        const uint64_t occ_m = __ballot(occ);
-        const uint32_t n0 = n;
-        uint64_t emit = 0ull;
-        bool done = false;
-        int cur = 0;
-        while (cur < kWave) {
-            if (pending) {
-                // skip_to stops at the first element with !(t < thr)
-                const uint64_t m = __ballot(!(w < thr)) & (~0ull << cur);
-                if (m == 0ull) {
-                    // the target lies past this window; if the window already
-                    // reached t_end, so does the target and the ray is done
-                    done = ((live_m >> (kWave - 1)) & 1ull) == 0ull;
-                    break;
-                }
-                cur = __builtin_ctzll(m);
-                pending = false;
-            }
-            if (((live_m >> cur) & 1ull) == 0ull) {
-                done = true;
-                break;
-            }
-            if ((occ_m >> cur) & 1ull) {
-                const uint64_t from = occ_m >> cur;
-                const int run = ~from == 0ull ? kWave - cur : __builtin_ctzll(~from);
-                const uint32_t take = min((uint32_t)run, cap - n);
-                emit |= (take >= 64u ? ~0ull : ((1ull << take) - 1ull)) << cur;
-                n += take;
-                if (n >= cap) {
-                    done = true;
-                    break;
-                }
-                cur += run;
-            } else {
-                thr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex), cur));
-                pending = true;
-                cur += 1;
-            }
-        }
-        if (kWrite && ((emit >> lane) & 1ull)) put_t(o, n0 + (uint32_t)__popcll(emit & below), w);
-        if (done) break;
-        base = c;
     }
     return n;
 }
@@ -847,7 +909,9 @@ extern "C" void march_rays(ngp_stream_t stream_, void **buffers, const char *opa
                        d.n_rays, d.total_samples, rays_o, rays_d, t_starts, t_ends, n_samples, startidx, zs, idcs, xyzs,
                        dirs, dss);
     if (!check_launch(target)) return;
-    hipLaunchKernelGGL(march_finish_kernel, dim3(div_up(d.n_rays, kBlock)), dim3(kBlock), 0, stream, d.n_rays,
+    // a thread per ray, and enough workgroups for the tail's zeros when rays are few
+    const uint32_t finish_grid = std::max(div_up(d.n_rays, kBlock), std::min(div_up(d.total_samples, kBlock), 2048u));
+    hipLaunchKernelGGL(march_finish_kernel, dim3(finish_grid), dim3(kBlock), 0, stream, d.n_rays,
                        d.total_samples, t_starts, t_ends, next_loc, exceeded, valid, n_samples, startidx, idcs, xyzs,
                        dirs, dss, zs);
     check_launch(target);

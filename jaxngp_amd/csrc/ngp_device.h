@@ -110,11 +110,10 @@ __device__ __forceinline__ bool occupied(BitCache &c, const uint8_t *__restrict_
     return (c.word >> (idx & 63u)) & 1u;
 }
 
-// Occupancy lookup of one position (marching.cu:79-98 and 170-176).  Fills the
-// float grid coordinate and the mip bound used by the empty-cell skip.
-__device__ __forceinline__ bool lookup_cell(const MarchParams &p, const uint8_t *__restrict__ bits, BitCache &cache,
-                                            float px, float py, float pz, float ds, float &gx, float &gy, float &gz,
-                                            float &mip_bound) {
+// Occupancy cell of one position (marching.cu:79-98 and 170-176): its index in the
+// bitfield, the float grid coordinate and the mip bound used by the empty-cell skip.
+__device__ __forceinline__ uint32_t cell_index(const MarchParams &p, float px, float py, float pz, float ds, float &gx,
+                                               float &gy, float &gz, float &mip_bound) {
     uint32_t cascade = 0;
     if (p.K > 1) {
         float m = fmaxf(fabsf(px), fmaxf(fabsf(py), fabsf(pz)));
@@ -123,17 +122,37 @@ __device__ __forceinline__ bool lookup_cell(const MarchParams &p, const uint8_t 
         cascade = c1 > c2 ? c1 : c2;
     }
     mip_bound = fminf(__uint_as_float((cascade + 127u) << 23), p.bound);
+    // x / 2^k and x * 2^-k are the same exact scaling, rounded alike (also below the
+    // normal range), so a power-of-two bound (every cascade but a clamped last one)
+    // takes a multiply instead of the division
+    float qx, qy, qz;
+    if ((__float_as_uint(mip_bound) & 0x007fffffu) == 0u && mip_bound >= 1.1754944e-38f && mip_bound <= 8.5e37f) {
+        const float inv = __uint_as_float((254u << 23) - __float_as_uint(mip_bound));  // 1 / mip_bound, exact
+        qx = px * inv;
+        qy = py * inv;
+        qz = pz * inv;
+    } else {
+        qx = px / mip_bound;
+        qy = py / mip_bound;
+        qz = pz / mip_bound;
+    }
     const float G = (float)p.G;
-    gx = ((px / mip_bound + 1.0f) * 0.5f) * G;
-    gy = ((py / mip_bound + 1.0f) * 0.5f) * G;
-    gz = ((pz / mip_bound + 1.0f) * 0.5f) * G;
+    gx = ((qx + 1.0f) * 0.5f) * G;
+    gy = ((qy + 1.0f) * 0.5f) * G;
+    gz = ((qz + 1.0f) * 0.5f) * G;
     const float gmax = (float)(p.G - 1);
     // clamp in float: identical to clampi((int)floorf(.), 0, G-1) for every input
     uint32_t ix = (uint32_t)fminf(fmaxf(floorf(gx), 0.0f), gmax);
     uint32_t iy = (uint32_t)fminf(fmaxf(floorf(gy), 0.0f), gmax);
     uint32_t iz = (uint32_t)fminf(fmaxf(floorf(gz), 0.0f), gmax);
-    uint32_t idx = cascade * p.G3 + morton3(ix, iy, iz);
-    return occupied(cache, bits, idx);
+    return cascade * p.G3 + morton3(ix, iy, iz);
+}
+
+// Occupancy lookup of one position through the ray's bit cache.
+__device__ __forceinline__ bool lookup_cell(const MarchParams &p, const uint8_t *__restrict__ bits, BitCache &cache,
+                                            float px, float py, float pz, float ds, float &gx, float &gy, float &gz,
+                                            float &mip_bound) {
+    return occupied(cache, bits, cell_index(p, px, py, pz, ds, gx, gy, gz, mip_bound));
 }
 
 // Exit distance of the current cell along the ray (marching.cu:182-185).

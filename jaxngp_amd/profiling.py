@@ -28,12 +28,13 @@ import sys
 
 # the kernels each target launches, and one that runs exactly once per launch
 TARGET_KERNELS = {
-    "march_rays": (["march_count_kernel", "march_scan_kernel", "march_emit_kernel", "march_finish_kernel",
-                    "march_empty_kernel"], "march_scan_kernel"),
+    "march_rays": (["march_count_kernel", "march_scan_kernel", "march_emit_kernel", "march_fill_kernel",
+                    "march_finish_kernel", "march_empty_kernel"], "march_scan_kernel"),
     "hashgrid_encode": (["hashgrid_fwd_lm_kernel", "hashgrid_fwd_kernel"], None),
-    "hashgrid_encode_backward": (["bwd_count_kernel", "scan_sums_kernel", "scan_apply_kernel", "plan_parts_kernel",
-                                  "bwd_scatter_kernel", "bwd_scatter_staged_kernel", "bwd_reduce_kernel",
-                                  "bwd_combine_kernel", "zero_table_kernel"], "plan_parts_kernel"),
+    "hashgrid_encode_backward": (["bwd_bucket_kernel", "bwd_slice_parts_kernel", "bwd_bucket_plan_kernel",
+                                  "bwd_reduce_runs_kernel", "bwd_count_kernel", "scan_sums_kernel",
+                                  "scan_apply_kernel", "plan_parts_kernel", "bwd_scatter_kernel", "bwd_reduce_kernel",
+                                  "bwd_combine_kernel", "zero_table_kernel"], "bwd_combine_kernel"),
     "nerf_mlp_forward": (["mlp_forward_kernel", "mlp_forward_h_kernel"], None),
     "nerf_mlp_backward": (["mlp_backward_kernel", "mlp_backward_h_kernel", "sum_rows_kernel"], "sum_rows_kernel"),
     "integrate_rays": (["integrate_kernel"], "integrate_kernel"),

@@ -63,10 +63,10 @@ class TrainConfig:
     # `proposal`.  n_rays defaults to total_samples / 16 in this mode (rays through
     # empty space keep no samples; the rays that do not fit are dropped)
     proposal_guided: bool = False
-    proposal_samples_per_ray: int = 48
+    proposal_samples_per_ray: int = 32
     # guided mode: samples per ray added at stratified uniform positions, so the
     # field also sees (and clears) what the proposal predicts empty
-    proposal_uniform_samples: int = 16
+    proposal_uniform_samples: int = 32
     # guided mode: the occupancy-grid marcher places the samples of the first
     # `proposal_guided_warmup` steps and of every `proposal_guided_plain_every`-th
     # step after them (0: none)
@@ -78,6 +78,13 @@ class TrainConfig:
     # entries per hash-grid level (2^19 in make_nerf_ngp; the precision workload
     # also trains 2^22)
     log2_hashmap_size: int = 19
+    # occupancy threshold cap: None takes the reference's 0.01 * diagonal_n_steps /
+    # (2 min(bound, 1) sqrt 3) (types.py:1227-1239), the density at which one step
+    # of the smallest size has opacity 1%; in guided mode the same formula with the
+    # samples per ray of the guided sampler (proposal_samples_per_ray +
+    # proposal_uniform_samples) in place of diagonal_n_steps.  The grid thresholds
+    # at the smaller of the cap and the mean density
+    density_threshold: float | None = None
     # run the optimizer update on a side stream, overlapping the next step's ray
     # generation and march (False: on the main stream, in order)
     overlap_update: bool = True
@@ -272,8 +279,7 @@ class Trainer:
         g = self.grid
         g.density = torch.where(alive, g.density, torch.full_like(g.density, -1.0))
         thr = -0.5 if self.step_count == 0 else min(
-            density_threshold_from_min_step_size(self.cfg.diagonal_n_steps, self.bound),
-            float(g.mean_density_up_to_cascade(1)))
+            self.density_threshold_cap, float(g.mean_density_up_to_cascade(1)))
         from .volrend import packbits
         g.occ_mask, g.bitfield = packbits(thr, g.density)
         g.alive = all_idx[alive]
@@ -307,10 +313,23 @@ class Trainer:
         """types.py:1227-1239."""
         g = self.grid
         # clamp against a Python scalar: no host-to-device copy, so no host sync
-        thr = torch.clamp(g.mean_density_up_to_cascade(1),
-                          max=density_threshold_from_min_step_size(self.cfg.diagonal_n_steps, self.bound))
+        thr = torch.clamp(g.mean_density_up_to_cascade(1), max=self.density_threshold_cap)
         from .volrend import packbits
         g.occ_mask, g.bitfield = packbits(thr, g.density)
+
+    @property
+    def density_threshold_cap(self) -> float:
+        return self.threshold_cap_of(self.cfg, self.bound)
+
+    @staticmethod
+    def threshold_cap_of(c: "TrainConfig", bound: float = 1.0) -> float:
+        """The occupancy threshold cap configuration `c` trains with (see
+        TrainConfig.density_threshold)."""
+        if c.density_threshold is not None:
+            return float(c.density_threshold)
+        if c.proposal_guided:
+            return density_threshold_from_min_step_size(c.proposal_samples_per_ray + c.proposal_uniform_samples, bound)
+        return density_threshold_from_min_step_size(c.diagonal_n_steps, bound)
 
     @property
     def update_ogrid_interval(self) -> int:

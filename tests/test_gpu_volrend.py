@@ -82,6 +82,7 @@ def _assert_march_equal(got, want):
     dict(n=777, S=1, K=1, G=128, bound=1.0, portion=0.0),           # tiny buffer
     dict(n=1500, S=2 ** 19, K=2, G=64, bound=2.0, portion=1 / 256),  # cascades + growing steps
     dict(n=1000, S=2 ** 19, K=1, G=32, bound=1.0, portion=0.0),
+    dict(n=1200, S=2 ** 19, K=3, G=64, bound=3.0, portion=1 / 256),  # last cascade's bound 3: not a power of two
 ])
 def test_march_rays_bit_exact(dev, case):
     rng = np.random.default_rng(case["n"])
