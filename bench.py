@@ -185,13 +185,12 @@ def steady_state(tr, steps: int = 200) -> dict:
     field: the grid update runs every 16 steps there (after step 256)."""
     import torch
     counts = torch.zeros(steps, 4, dtype=torch.int32, device=tr.dev)
-    ctr = tr.counters.view(torch.int32)
     first = tr.step_count
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
         tr.step()
-        counts[i].copy_(ctr)
+        counts[i].copy_(tr.counters.view(torch.int32))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     c = counts.long()
@@ -239,18 +238,21 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
         def __getattr__(self, a):
             return getattr(self.call, a)
 
-        def __call__(self, opaque=None):
-            stream = torch.cuda.current_stream()
+        def __call__(self, opaque=None, stream=None):
+            # the events go on the stream the call is launched on: the trainer passes
+            # a stream only inside that stream's context
+            cur = torch.cuda.current_stream()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            self.call(opaque)
-            b.record(stream)
+            a.record(cur)
+            self.call(opaque, stream)
+            b.record(cur)
             self.events.setdefault(self.call.name, []).append((a, b))
 
-    plain = tr._calls
+    plain = tr._calls  # one dict of bound calls per buffer set
 
     def instrument(names, events):
-        tr._calls = {k: (Timed(v, events) if hasattr(v, "name") and v.name in names else v) for k, v in plain.items()}
+        tr._calls = [{k: (Timed(v, events) if hasattr(v, "name") and v.name in names else v) for k, v in d.items()}
+                     for d in plain]
 
     def per_launch(events):
         return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in events.items() if v}
@@ -267,12 +269,11 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
     # per-step counters (next_loc, exceeded, n_valid, measured) logged on the device
     # with one 16-byte copy per step, summed after the timed region
     counts = torch.zeros(steps, 4, dtype=torch.int32, device=dev)
-    ctr = tr.counters.view(torch.int32)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for i in range(steps):
         tr.step()
-        counts[i].copy_(ctr)
+        counts[i].copy_(tr.counters.view(torch.int32))  # the set the step used
     t_host = time.perf_counter() - t_start
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
@@ -303,7 +304,7 @@ def train_to(tr, total_steps: int) -> None:
     torch.cuda.synchronize()
 
 
-def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dict:
+def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False, empty_mass: float = 0.0) -> dict:
     """ms per 800x800 frame of the trained field from held-out seeded poses, with the
     reference's ray-pool renderer (march_steps_cap 8, white background) on a pool of
     2^18 slots, median over the timed frames after three warm-up renders; PSNR of the
@@ -327,7 +328,8 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
     plain = lambda pose: render_image_inference(tr.model, data.camera, pose, tr.grid.bitfield, **kw)
     if proposal:
         render = lambda pose, st=None, **o: render_image_proposal(tr.model, tr.prop, data.camera, pose,
-                                                                  tr.grid.bitfield, stats=st, **o, **kw)
+                                                                  tr.grid.bitfield, stats=st, empty_mass=empty_mass,
+                                                                  **o, **kw)
     else:
         render = lambda pose, st=None: plain(pose)
     for i in range(3):
@@ -358,7 +360,7 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False) -> dic
            "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
            "train_steps": tr.step_count}
     if proposal:
-        res.update(samples_per_ray=48, mass_threshold=0.98, placement="quantiles",
+        res.update(samples_per_ray=48, mass_threshold=0.98, placement="quantiles", empty_mass=empty_mass,
                    fallback_ray_fraction=float(np.mean(fallback)),
                    psnr_plain_render_db=float(np.mean(plain_psnrs)),
                    delta_psnr_db=float(np.mean(psnrs) - np.mean(plain_psnrs)),

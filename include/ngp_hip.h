@@ -448,6 +448,9 @@ typedef struct ProposalSampleDescriptor {
     /* marching statics of the occupancy grid (as MarchingDescriptor) */
     uint32_t diagonal_n_steps, K, G;
     float bound, stepsize_portion;
+    float empty_mass;     /* a ray with M < empty_mass gets no samples and no fallback: it
+                             renders as background (0: off, every ray below
+                             mass_threshold falls back) */
 } ProposalSampleDescriptor;
 
 /* Samples placed by the proposal: per ray, the bin weights w_b of sigma_p give the
@@ -460,7 +463,8 @@ typedef struct ProposalSampleDescriptor {
  * others are packed in order at the front of the ray's slots, rays_n_samples of
  * them, the remaining slots are zero.  A ray
  * below the threshold gets none and fallback = true (the occupancy-grid marcher
- * renders it).  Sample arrays use [n_rays, n_per_ray] slots, startidx = r n_per_ray.
+ * renders it), unless its mass is below empty_mass: then fallback = false and the ray
+ * composites to its background.  Sample arrays use [n_rays, n_per_ray] slots, startidx = r n_per_ray.
  * in : rays_o f32[n,3], rays_d f32[n,3], t_starts f32[n], t_ends f32[n], sigma_p f32[n,64],
  *      occupancy_bitfield u8[K*G^3/8]
  * out: rays_sample_startidx u32[n], rays_n_samples u32[n], fallback bool[n],

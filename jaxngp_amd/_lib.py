@@ -167,9 +167,11 @@ class Prepared:
         self.keep[i] = t
         self.ptrs[i] = t.data_ptr()
 
-    def __call__(self, opaque: bytes | None = None) -> None:
+    def __call__(self, opaque: bytes | None = None, stream: int | None = None) -> None:
+        """Launches with the bound descriptor (or `opaque`) on the bound stream (or
+        `stream`, a raw stream handle)."""
         o = self.opaque if opaque is None else opaque
-        self.fn(self.stream, self.ptrs, o, len(o))
+        self.fn(self.stream if stream is None else ctypes.c_void_p(stream), self.ptrs, o, len(o))
 
 
 def check() -> None:
@@ -317,9 +319,10 @@ PROPOSAL_SUPPORT = 1
 
 
 def proposal_sample_descriptor(n_rays, n_per_ray=48, mass_threshold=0.9, mode=PROPOSAL_QUANTILES, eps=1e-3,
-                               diagonal_n_steps=1024, K=1, G=128, bound=1.0, stepsize_portion=0.0) -> bytes:
-    return struct.pack("<2IfIf3I2f", n_rays, n_per_ray, mass_threshold, mode, eps, diagonal_n_steps, K, G, bound,
-                       stepsize_portion)
+                               diagonal_n_steps=1024, K=1, G=128, bound=1.0, stepsize_portion=0.0,
+                               empty_mass=0.0) -> bytes:
+    return struct.pack("<2IfIf3I3f", n_rays, n_per_ray, mass_threshold, mode, eps, diagonal_n_steps, K, G, bound,
+                       stepsize_portion, empty_mass)
 
 
 def proposal_render_descriptor(grid_desc: bytes, sample_desc: bytes, param_dtype=DTYPE_F32) -> bytes:

@@ -241,6 +241,7 @@ struct SampleArgs {
     float mass_threshold, eps;
     MarchParams mp;
     uint32_t n_uniform = 0;  // training: stratified uniform samples added to each ray
+    float empty_mass = 0.0f;  // below it a ray renders as background, without fallback
 };
 
 // Ascending bitonic sort of one value per lane across the wavefront.
@@ -267,8 +268,9 @@ __device__ __forceinline__ float wave_sort_ascending(float v) {
 struct SamplePlan {
     float t, ds, p[3];
     bool keep;  // this lane's sample exists and its cell is occupied
-    bool ok;    // the ray has a segment and enough predicted mass
-    bool seg;   // t1 > t0
+    bool ok;     // the ray has a segment and enough predicted mass
+    bool seg;    // t1 > t0
+    bool empty;  // predicted mass below empty_mass: background, no fallback
 };
 
 __device__ __forceinline__ SamplePlan plan_samples(const SampleArgs &sa, uint32_t r, float sigma_b, float t0, float t1,
@@ -286,6 +288,7 @@ __device__ __forceinline__ SamplePlan plan_samples(const SampleArgs &sa, uint32_
     const float w = Tb * a;
     const float M = wave_sumf(w);
     pl.ok = pl.seg && M >= sa.mass_threshold && M > 0.0f;
+    pl.empty = pl.seg && !pl.ok && M < sa.empty_mass;
     float t = 0.0f, ds = 0.0f;
     if (pl.ok && sa.mode == NGP_PROPOSAL_SUPPORT) {
         // bins [lo, hi]: first with cumulative share > eps, first with >= 1 - eps
@@ -364,7 +367,7 @@ __device__ __forceinline__ void place_samples(const SampleArgs &sa, uint32_t r, 
     if (lane == 0) {
         startidx[r] = (uint32_t)base;
         counts[r] = n_kept;
-        fallback[r] = pl.seg && !pl.ok;
+        fallback[r] = pl.seg && !pl.ok && !pl.empty;
     }
     if (pl.keep) {
         const size_t j = base + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull));
@@ -671,9 +674,9 @@ extern "C" void proposal_sample(ngp_stream_t stream, void **buffers, const char 
     ProposalSampleDescriptor d;
     if (!read_descriptor(opaque, opaque_len, &d, target)) return;
     if (d.n_per_ray == 0 || d.n_per_ray > kBins || !(d.mass_threshold >= 0.0f) ||
-        d.mode > NGP_PROPOSAL_SUPPORT || !(d.eps >= 0.0f && d.eps < 0.5f)) {
+        d.mode > NGP_PROPOSAL_SUPPORT || !(d.eps >= 0.0f && d.eps < 0.5f) || !(d.empty_mass >= 0.0f)) {
         set_error(kBadDescriptor, "%s: n_per_ray must be in [1, %d], mass_threshold >= 0, mode 0 or 1, eps in "
-                  "[0, 0.5)", target, kBins);
+                  "[0, 0.5), empty_mass >= 0", target, kBins);
         return;
     }
     const uint32_t G = d.G;
@@ -690,6 +693,7 @@ extern "C" void proposal_sample(ngp_stream_t stream, void **buffers, const char 
     sa.mode = d.mode;
     sa.mass_threshold = d.mass_threshold;
     sa.eps = d.eps;
+    sa.empty_mass = d.empty_mass;
     sa.mp = make_march_params(d.diagonal_n_steps, d.K, d.G, d.bound, d.stepsize_portion);
     hipLaunchKernelGGL(proposal_sample_kernel, dim3(div_up(d.n_rays, kBlock / kWave)), dim3(kBlock), 0,
                        (hipStream_t)stream, d.n_rays, sa, (const float *)buffers[0], (const float *)buffers[1],
@@ -720,7 +724,7 @@ static bool check_render_descriptor(const ProposalRenderDescriptor &d, SampleArg
     const bool g_ok = G != 0 && G <= 1024 && (G & (G - 1)) == 0 && (uint64_t)q.K * G * G * G <= 0xffffffffull;
     if (q.n_per_ray == 0 || q.n_per_ray > kBins || !(q.mass_threshold >= 0.0f) || q.mode > NGP_PROPOSAL_SUPPORT ||
         !(q.eps >= 0.0f && q.eps < 0.5f) || q.diagonal_n_steps == 0 || q.K == 0 || !g_ok || !(q.bound > 0.0f) ||
-        !(q.stepsize_portion >= 0.0f)) {
+        !(q.stepsize_portion >= 0.0f) || !(q.empty_mass >= 0.0f)) {
         set_error(kBadDescriptor, "%s: invalid sample placement or marching statics", target);
         return false;
     }
@@ -728,6 +732,7 @@ static bool check_render_descriptor(const ProposalRenderDescriptor &d, SampleArg
     sa.mode = q.mode;
     sa.mass_threshold = q.mass_threshold;
     sa.eps = q.eps;
+    sa.empty_mass = q.empty_mass;
     sa.mp = make_march_params(q.diagonal_n_steps, q.K, q.G, q.bound, q.stepsize_portion);
     return true;
 }

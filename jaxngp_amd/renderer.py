@@ -211,7 +211,7 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
                           placement: int = _lib.PROPOSAL_QUANTILES, K=1, G=128,
                           diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8, pool: int = 1 << 18,
                           fallback_steps_cap: int = 32, fallback_pool: int = 1 << 16,
-                          stats: dict | None = None):
+                          empty_mass: float = 0.0, stats: dict | None = None):
     """Renders one frame with samples placed by the learned proposal: its density at
     64 bins of every ray gives 48 samples, by default at the quantiles of the ray's
     predicted mass (placement PROPOSAL_QUANTILES; PROPOSAL_SUPPORT spreads them evenly
@@ -236,7 +236,10 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
     15.8 ms per 800x800 frame, same pixels; the plain render of all rays is fastest
     at 2^18).  `march_steps_cap` and `pool` are accepted for the same keyword set as
     render_image_inference.  Same return as
-    render_image_inference; `stats` (if given) receives the fallback count."""
+    render_image_inference; `stats` (if given) receives the fallback count.  Rays whose
+    predicted mass is below `empty_mass` (0: none) do not fall back: they get no
+    samples and composite to the background, as the plain render does where the field
+    holds no mass along the ray."""
     dev = pose.device
     o, d = make_rays_worldspace(camera, pose)
     t0, t1 = make_near_far_from_bound(model.bound, o, d)
@@ -255,7 +258,7 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
               _lib.proposal_render_descriptor(ppe.descriptor(0, proposal.bound),
                                               _lib.proposal_sample_descriptor(n, n_per_ray, mass_threshold, placement,
                                                                               1e-3, diagonal_n_steps, K, G, model.bound,
-                                                                              stepsize_portion), ppd))
+                                                                              stepsize_portion, empty_mass), ppd))
     drgbs = torch.empty(n, n_per_ray, 4, device=dev)
     pe = model.position_encoder
     half = getattr(model, "mlp_half", None) is not None

@@ -88,6 +88,12 @@ class TrainConfig:
     # run the optimizer update on a side stream, overlapping the next step's ray
     # generation and march (False: on the main stream, in order)
     overlap_update: bool = True
+    # make the next step's rays and samples (train_make_rays + march_rays or the guided
+    # sampler) on a stream of their own while this step computes, into a second set
+    # of ray and sample arrays, whenever nothing this step does changes what they read
+    # (no grid update after it; in guided mode, no proposal update in it).  The
+    # samples are the same either way (False: made at the start of each step)
+    prefetch_march: bool = True
 
 
 def lr_at(step: int, lr: float) -> float:
@@ -166,6 +172,11 @@ class Trainer:
         self.b["hg_ws"] = torch.empty(
             _lib.hashgrid_backward_workspace_bytes(enc_mod.descriptor(S, self.bound, _lib.ENC_LEVEL_MAJOR)),
             dtype=torch.uint8, device=self.dev)
+        self.b["counters"] = cnt
+        # two sets of the ray and sample arrays (the other buffers are shared): step s
+        # uses set s % 2, so the next step's can be filled while this one computes
+        self._sets = [self.b, self._second_set(self.b)]
+        self._cur = 0
         table = self.model.position_encoder.table
         self.table_master = table.detach().float().clone()
         self.opt = dict(tm=torch.zeros_like(self.table_master), tv=torch.zeros_like(self.table_master),
@@ -206,8 +217,24 @@ class Trainer:
         self.side_stream = torch.cuda.Stream(self.dev) if cfg.overlap_update else self.main_stream  # default priority
         self._bwd_done = torch.cuda.Event()
         self._adam_done = torch.cuda.Event()
+        self.pf_stream = torch.cuda.Stream(self.dev) if cfg.prefetch_march else None  # default priority
+        self._pf_ready = [torch.cuda.Event(), torch.cuda.Event()]  # a set's prefetched samples are written
+        self._set_free = [torch.cuda.Event(), torch.cuda.Event()]  # the last step that used a set is done
+        self._pf_pending = [None, None]  # per set: (bitfield, step) of a prefetch not yet consumed
         self._calls = None
         self.mark_untrained_density_grid()
+
+    # the arrays train_make_rays and the sampler write, and the step's counters
+    RAY_KEYS = ("o", "d", "t0", "t1", "noise", "bgs", "gt", "pix", "valid", "ns", "start", "idcs", "xyzs", "dirs",
+                "dss", "zs")
+
+    def _second_set(self, b: dict) -> dict:
+        b2 = dict(b)
+        for key in self.RAY_KEYS:
+            b2[key] = torch.zeros_like(b[key])
+        cnt = torch.zeros_like(b["counters"])
+        b2.update(counters=cnt, next_loc=cnt[0:1], exceeded=cnt[1:2], n_valid=cnt[2:3], measured=cnt[3:4])
+        return b2
 
     # -- parameters and the optimizer's side stream ----------------------------------------
 
@@ -226,6 +253,10 @@ class Trainer:
         if getattr(self, "_adam_pending", False):
             self.main_stream.wait_event(self._adam_done)
             self._adam_pending = False
+        for i, pending in enumerate(getattr(self, "_pf_pending", ())):
+            if pending is not None:
+                self.main_stream.wait_event(self._pf_ready[i])
+                self._pf_pending[i] = None
 
     def __del__(self):
         try:
@@ -345,9 +376,10 @@ class Trainer:
 
     # -- one step -----------------------------------------------------------------------
 
-    def _prepare(self) -> dict:
-        """Bound calls of one training step (buffers and static descriptors fixed)."""
-        c, b, m = self.cfg, self.b, self.model
+    def _prepare(self, b: dict) -> dict:
+        """Bound calls of one training step on buffer set `b` (buffers and static
+        descriptors fixed)."""
+        c, m = self.cfg, self.model
         enc_mod = m.position_encoder
         table = enc_mod.table.detach()
         params = m.mlp_params.detach()
@@ -458,32 +490,50 @@ class Trainer:
 
     def step(self) -> None:
         """One training step (app/nerf/_utils.py:80-170), then the grid cadence.
-        Kernel order: train_make_rays -> march_rays -> (wait for the previous update)
-        -> hashgrid_encode -> nerf_mlp_forward -> integrate_rays -> train_loss_grad ->
-        integrate_rays_backward -> nerf_mlp_backward -> hashgrid_encode_backward, then
-        adam_update (table, MLP) on the side stream, where it overlaps the next step's
-        train_make_rays and march_rays.  With a background model, sky_sphere_bg +
-        train_gt_blend follow the wait and train_bg_grad + sky_sphere_bg_backward
-        follow integrate_rays_backward.  Readers of the parameters outside the step go
-        through `model`, `bg_model` or `sync()`."""
+        Kernel order: train_make_rays -> march_rays (or the guided sampler) -> (wait for
+        the previous update) -> hashgrid_encode -> nerf_mlp_forward -> integrate_rays ->
+        train_loss_grad -> integrate_rays_backward -> nerf_mlp_backward ->
+        hashgrid_encode_backward, then adam_update (table, MLP) on the side stream.
+        With `prefetch_march`, the next step's train_make_rays and sampler run on the
+        prefetch stream into the other buffer set right after this step's sampler, so
+        the latency-bound marcher overlaps this step's compute; a step whose samples
+        were prefetched starts with its encoder.  A prefetch is made only when nothing
+        before the next step changes what it reads (see _prefetchable), and is dropped
+        if the grid's bitfield was replaced meanwhile.  With a background model,
+        sky_sphere_bg + train_gt_blend follow the wait and train_bg_grad +
+        sky_sphere_bg_backward follow integrate_rays_backward.  Readers of the
+        parameters outside the step go through `model`, `bg_model` or `sync()`."""
         c = self.cfg
         if self._calls is None:
-            self._calls = self._prepare()
-        k = self._calls
-        t0 = self.step_count
-        guided = k["guided"] is not None and t0 >= c.proposal_guided_warmup and not (
-            c.proposal_guided_plain_every > 0 and (t0 - c.proposal_guided_warmup) % c.proposal_guided_plain_every == 0)
-        march = k["guided"] if guided else k["march"]
-        march.set_buffer(7 if guided else 5, self.grid.bitfield)
-        cam, data = self.data.camera, self.data
-        bg = self._bg_model is not None
-        flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg and not bg else 0)
-        k["rays"](_lib.train_rays_descriptor(self.n_rays, data.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx,
-                                             cam.cy, self.bound, c.seed, self.step_count, flags, c.bg))
-        # rays and march read no parameters: they overlap the last step's update on the
-        # side stream; everything after waits for it
-        march()
+            # (re)bind: a prefetch made before (e.g. before a checkpoint restore) is dropped
+            self.close()
+            self._calls = [self._prepare(b) for b in self._sets]
+        cur = self._cur
+        self.b = self._sets[cur]
+        self.counters = self.b["counters"]
+        k = self._calls[cur]
+        s0 = self.step_count
+        bits = self.grid.bitfield
+        pending = self._pf_pending[cur]
+        if pending is not None:
+            self.main_stream.wait_event(self._pf_ready[cur])
+            self._pf_pending[cur] = None
+        if pending is None or pending[0] is not bits or pending[1] != s0:
+            self._rays_and_samples(k, s0, None)
+        # the next step's rays and samples, on the prefetch stream into the other set,
+        # once the step that last used that set is done
+        nxt = 1 - cur
+        if self.pf_stream is not None and self._prefetchable(s0):
+            pf = self.pf_stream
+            pf.wait_event(self._set_free[nxt])
+            with torch.cuda.stream(pf):
+                self._rays_and_samples(self._calls[nxt], s0 + 1, pf.cuda_stream)
+            self._pf_ready[nxt].record(pf)
+            self._pf_pending[nxt] = (bits, s0 + 1)
+        # rays and samples read no field parameters: they overlap the last step's update
+        # on the side stream; everything after waits for it
         self.sync()
+        bg = self._bg_model is not None
         order = ("encode", "mlp", "integrate", "loss", "integrate_bwd", "mlp_bwd", "encode_bwd")
         if bg:
             # per-ray backgrounds from the model, the ground truth blended over them, and
@@ -533,8 +583,44 @@ class Trainer:
             k["p_adam_mlp"](_lib.adam_descriptor(_lib.PROPOSAL_N_PARAMS, lr, b1, b2, 1e-15, 1e-15, q1, q2, 1e-6,
                                                  mirror_dtype=_lib.DTYPE_F16 if self.prop.mlp_half is not None
                                                  else _lib.DTYPE_F32))
+        self._set_free[cur].record(self.main_stream)
+        self._cur = nxt
         _lib.check()
         self.maybe_update_grid()
+
+    def _guided_at(self, s: int) -> bool:
+        """Whether step s takes the guided sampler's samples (else the marcher's)."""
+        c = self.cfg
+        return c.proposal_guided and s >= c.proposal_guided_warmup and not (
+            c.proposal_guided_plain_every > 0 and (s - c.proposal_guided_warmup) % c.proposal_guided_plain_every == 0)
+
+    def _grid_update_after(self, s: int) -> bool:
+        """Whether maybe_update_grid runs an update at the end of step s."""
+        t = s + 1
+        return t % min(16, t // 16 + 1) == 0
+
+    def _prefetchable(self, s: int) -> bool:
+        """Step s + 1's samples can be made during step s: the grid does not change
+        after step s, and a guided step s + 1 reads no proposal that step s updates."""
+        c = self.cfg
+        if self._grid_update_after(s):
+            return False
+        if self.prop is not None and self._guided_at(s + 1) and s % c.proposal_every == 0:
+            return False
+        return True
+
+    def _rays_and_samples(self, k: dict, s: int, stream) -> None:
+        """train_make_rays and the sampler of step s on set k's buffers (on `stream`,
+        or the calls' own)."""
+        c, cam, data = self.cfg, self.data.camera, self.data
+        guided = self._guided_at(s)
+        march = k["guided"] if guided else k["march"]
+        march.set_buffer(7 if guided else 5, self.grid.bitfield)
+        bg = self._bg_model is not None
+        flags = (_lib.TRAIN_PERTURB if c.perturb else 0) | (_lib.TRAIN_RANDOM_BG if c.random_bg and not bg else 0)
+        k["rays"](_lib.train_rays_descriptor(self.n_rays, data.n_views, cam.width, cam.height, cam.fx, cam.fy, cam.cx,
+                                             cam.cy, self.bound, c.seed, s, flags, c.bg), stream=stream)
+        march(stream=stream)
 
     # -- metrics (these synchronise) -------------------------------------------------------
 

@@ -61,9 +61,9 @@ def test_descriptor_sizes_match_c_structs():
     assert len(_lib.proposal_rays_descriptor(1)) == 4
     assert len(_lib.proposal_mlp_descriptor(1, 2)) == 16
     assert len(_lib.proposal_loss_descriptor(1)) == 8
-    assert len(_lib.proposal_sample_descriptor(1)) == 40
+    assert len(_lib.proposal_sample_descriptor(1)) == 44
     hg5 = _lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])
-    assert len(_lib.proposal_render_descriptor(hg5, _lib.proposal_sample_descriptor(1))) == len(hg5) + 44
+    assert len(_lib.proposal_render_descriptor(hg5, _lib.proposal_sample_descriptor(1))) == len(hg5) + 48
     hg = _lib.hashgrid_descriptor(1, 2, 2, 8, 1, 0, 16, 1.0, [1.0, 2.0], [2, 3], [0, 8, 16])
     assert len(_lib.field_chunks_descriptor(hg, 1, 8, 3)) == len(hg) + 16
     # field order: n_rays first, bound at byte 20
@@ -94,8 +94,8 @@ _Static_assert(sizeof(NgpPoolScatterDescriptor) == 12, "");
 _Static_assert(sizeof(ProposalRaysDescriptor) == 4, "");
 _Static_assert(sizeof(ProposalMlpDescriptor) == 16, "");
 _Static_assert(sizeof(ProposalLossDescriptor) == 8, "");
-_Static_assert(sizeof(ProposalSampleDescriptor) == 40, "");
-_Static_assert(sizeof(ProposalRenderDescriptor) == sizeof(HashGridDescriptor) + 44, "");
+_Static_assert(sizeof(ProposalSampleDescriptor) == 44, "");
+_Static_assert(sizeof(ProposalRenderDescriptor) == sizeof(HashGridDescriptor) + 48, "");
 _Static_assert(sizeof(NerfFieldChunksDescriptor) == sizeof(HashGridDescriptor) + 16, "");
 _Static_assert(sizeof(TcnnHashGridDescriptor) == 20, "");
 _Static_assert(sizeof(SkySphereBgDescriptor) == 16, "");

@@ -289,3 +289,35 @@ def test_overlapped_update_matches_in_order(small_data):
     g1, g0 = res[True][1], res[False][1]
     assert float(g0.abs().max()) > 0
     assert float((g1 - g0).abs().max()) <= 1e-3 * float(g0.abs().max())
+
+
+@pytest.mark.parametrize("guided", [False, True])
+def test_prefetched_samples_equal_a_fresh_run(small_data, guided):
+    """prefetch_march: the next step's rays and samples, made on the prefetch stream
+    while a step computes (into the second buffer set), equal train_make_rays and the
+    sampler run afresh, alone, for the same step index on the same bitfield (and, in
+    guided mode, the same proposal).  The step then takes them."""
+    cfg = TrainConfig(total_samples=1 << 16, precision="fp16", proposal=guided, proposal_guided=guided)
+    tr = Trainer(small_data, cfg)
+    for _ in range(60):
+        tr.step()
+        if tr._pf_pending[tr._cur] is not None:
+            break
+    nxt = tr._cur
+    assert tr._pf_pending[nxt] is not None, "no step was prefetched"
+    bits, s = tr._pf_pending[nxt]
+    assert bits is tr.grid.bitfield and s == tr.step_count
+    torch.cuda.synchronize()
+    got = tr._sets[nxt]
+    fresh = tr._second_set(got)
+    tr._rays_and_samples(tr._prepare(fresh), s, None)
+    torch.cuda.synchronize()
+    for key in Trainer.RAY_KEYS:
+        assert torch.equal(got[key], fresh[key]), key
+    for key in ("next_loc", "exceeded"):
+        assert torch.equal(got[key], fresh[key]), key
+    assert int(got["next_loc"].view(torch.int32)) > 0
+    tr.step()
+    torch.cuda.synchronize()
+    m = tr.metrics()
+    assert math.isfinite(m["loss"]) and m["n_valid_rays"] > 0
