@@ -263,7 +263,10 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
         tr.step()
     torch.cuda.synchronize()
     per = per_launch(calib)
-    dom = max((k for k in per if k in ROOFLINE_CANDIDATES), key=lambda k: per[k])
+    # the roofline kernel is the longest on the step's critical path: a prefetched
+    # sampler runs beside the step's compute (its events span that overlap)
+    off_path = {"march_rays"} if tr.pf_stream is not None else set()
+    dom = max((k for k in per if k in ROOFLINE_CANDIDATES and k not in off_path), key=lambda k: per[k])
     live: dict = {}
     instrument([dom], live)
     # per-step counters (next_loc, exceeded, n_valid, measured) logged on the device

@@ -144,12 +144,23 @@ __global__ void __launch_bounds__(kCountThreads) count_valid_kernel(uint32_t n, 
     const uint8_t *f = reinterpret_cast<const uint8_t *>(flags);
     uint32_t c = 0;
     const uint32_t n16 = (reinterpret_cast<uintptr_t>(f) & 15u) == 0 ? n / 16 : 0;
-    for (uint32_t q = threadIdx.x; q < n16; q += kCountThreads) {
-        const uint4 v = reinterpret_cast<const uint4 *>(f)[q];
-        // bools are 0 or 1: add the bytes of each word
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    // eight 16-byte loads in flight per thread (one round for 2^17 flags)
+    constexpr uint32_t kU = 8;
+    for (uint32_t q0 = 0; q0 < n16; q0 += kU * kCountThreads) {
+        uint4 v[kU];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) c += (w[k] & 0xffu) + ((w[k] >> 8) & 0xffu) + ((w[k] >> 16) & 0xffu) + (w[k] >> 24);
+        for (uint32_t u = 0; u < kU; ++u) {
+            const uint32_t q = q0 + u * kCountThreads + threadIdx.x;
+            v[u] = q < n16 ? reinterpret_cast<const uint4 *>(f)[q] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            // bools are 0 or 1: add the bytes of each word
+            const uint32_t w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                c += (w[k] & 0xffu) + ((w[k] >> 8) & 0xffu) + ((w[k] >> 16) & 0xffu) + (w[k] >> 24);
+        }
     }
     for (uint32_t i = 16 * n16 + threadIdx.x; i < n; i += kCountThreads) c += f[i] ? 1u : 0u;
     c = wave_sum(c);

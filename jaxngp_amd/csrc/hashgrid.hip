@@ -773,10 +773,10 @@ __global__ void __launch_bounds__(kReduceThreads)
 // One-pass table gradient, part 4: workgroup b sums part b of slice s, as
 // bwd_reduce_kernel does, from the runs of the items whose level reaches s.  The
 // items are visited in (level, chunk) order and a part takes those whose exclusive
-// prefix of run lengths falls in [part * kPart, (part + 1) * kPart): a contiguous
-// range of items per tile of kReduceThreads items.  The selected runs are read as
-// one flat sequence, kU entries per thread in flight; a flat index finds its run by
-// binary search over the runs' starting prefixes in LDS.
+// prefix of run lengths falls in [part * kPart, (part + 1) * kPart).  A wavefront
+// reads kRunsPerWave runs side by side, two loads per lane and run in flight (a
+// hashed level's run holds about 128 entries; longer runs loop), so eight entry
+// loads per lane are in flight as in bwd_reduce_kernel.
 template <int F, bool kHalf>
 __global__ void __launch_bounds__(kReduceThreads)
     bwd_reduce_runs_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ meta,
@@ -786,12 +786,12 @@ __global__ void __launch_bounds__(kReduceThreads)
                            float *__restrict__ dtable) {
     constexpr int EW = entry_words<F, kHalf>();
     constexpr uint32_t R = slice_rows<F>();
-    constexpr int kU = 8;
+    constexpr uint32_t kWaves = kReduceThreads / kWave;
+    constexpr int kRunsPerWave = 4, kPerRun = 2;
     extern __shared__ double acc[];
-    // selected runs: entry base (item region + run offset) and starting prefix
+    // selected runs: entry base (item region + run offset) and length
     __shared__ uint64_t sel_base[kReduceThreads];
-    __shared__ uint32_t sel_start[kReduceThreads];
-    __shared__ uint32_t sel_lo, sel_hi;
+    __shared__ uint32_t sel_cnt[kReduceThreads];
     const uint32_t b = blockIdx.x;
     if (b >= meta[1]) return;
     const uint32_t s = part_slice[b];
@@ -800,6 +800,8 @@ __global__ void __launch_bounds__(kReduceThreads)
     const uint32_t rows = min(d.n_rows - s * R, R);
     for (uint32_t t = threadIdx.x; t < R * F; t += kReduceThreads) acc[t] = 0.0;
     __syncthreads();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
     uint32_t carry = 0;  // entries of the items visited before
     for (uint32_t l = 0; l < d.L; ++l) {
         if (s < p.s_lo[l] || s > p.s_hi[l]) continue;
@@ -818,35 +820,40 @@ __global__ void __launch_bounds__(kReduceThreads)
             const uint32_t at = block_exclusive_scan<kReduceThreads>(sel ? 1u : 0u, &n_sel);
             if (sel) {
                 sel_base[at] = (uint64_t)item * kBucketEntries + off;
-                sel_start[at] = carry + ex;
-                if (at == 0) sel_lo = carry + ex;
-                if (at == n_sel - 1) sel_hi = carry + ex + c;
+                sel_cnt[at] = c;
             }
             carry += tot;
             __syncthreads();
-            if (n_sel > 0) {
-                const uint32_t lo = sel_lo, hi = sel_hi;
-                for (uint32_t e0 = lo; e0 < hi; e0 += kU * kReduceThreads) {
-                    uint32_t w[kU][EW];
+            for (uint32_t j0 = wave; j0 < n_sel; j0 += kWaves * kRunsPerWave) {
+                uint64_t base[kRunsPerWave];
+                uint32_t cnt[kRunsPerWave], longest = 0;
 #pragma unroll
-                    for (int u = 0; u < kU; ++u) {
-                        const uint32_t e = min(e0 + u * kReduceThreads + threadIdx.x, hi - 1u);
-                        // last selected run starting at or before e
-                        uint32_t j = 0;
+                for (int q = 0; q < kRunsPerWave; ++q) {
+                    const uint32_t j = j0 + q * kWaves;
+                    cnt[q] = j < n_sel ? sel_cnt[j] : 0u;
+                    base[q] = j < n_sel ? sel_base[j] : 0ull;
+                    longest = max(longest, cnt[q]);
+                }
+                for (uint32_t e0 = 0; e0 < longest; e0 += kPerRun * kWave) {
+                    uint32_t w[kRunsPerWave][kPerRun][EW];
 #pragma unroll
-                        for (uint32_t step = kReduceThreads / 2; step > 0; step >>= 1)
-                            if (j + step < n_sel && sel_start[j + step] <= e) j += step;
-                        load_entry<EW>(entries + (sel_base[j] + (e - sel_start[j])) * EW, w[u]);
-                    }
+                    for (int q = 0; q < kRunsPerWave; ++q)
 #pragma unroll
-                    for (int u = 0; u < kU; ++u) {
-                        const bool in = e0 + u * kReduceThreads + threadIdx.x < hi;
-#pragma unroll
-                        for (int f = 0; f < F; ++f) {
-                            const float v = entry_value<F, kHalf>(w[u], f);
-                            if (in && v != 0.0f) atomicAdd(&acc[f * R + w[u][0]], (double)v);
+                        for (int h = 0; h < kPerRun; ++h) {
+                            const uint32_t e = cnt[q] ? min(e0 + h * kWave + lane, cnt[q] - 1u) : 0u;
+                            load_entry<EW>(entries + (base[q] + e) * EW, w[q][h]);
                         }
-                    }
+#pragma unroll
+                    for (int q = 0; q < kRunsPerWave; ++q)
+#pragma unroll
+                        for (int h = 0; h < kPerRun; ++h) {
+                            const bool in = e0 + h * kWave + lane < cnt[q];
+#pragma unroll
+                            for (int f = 0; f < F; ++f) {
+                                const float v = entry_value<F, kHalf>(w[q][h], f);
+                                if (in && v != 0.0f) atomicAdd(&acc[f * R + w[q][h][0]], (double)v);
+                            }
+                        }
                 }
             }
             __syncthreads();  // the selection is rewritten by the next tile

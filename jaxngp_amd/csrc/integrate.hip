@@ -231,7 +231,17 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
     }
-    if (lane == 0 && composited != 0) atomicAdd(measured, composited);
+    // one device atomic per workgroup: every wave of a batch of valid rays would
+    // otherwise add to the one counter (4096 serialised atomics took ~40 us)
+    __shared__ uint32_t wave_n[kBlock / kWave];
+    if (lane == 0) wave_n[threadIdx.x / kWave] = composited;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; ++w) tot += wave_n[w];
+        if (tot != 0) atomicAdd(measured, tot);
+    }
 }
 
 // Backward, one wavefront per ray with samples (rays batched as in the forward;

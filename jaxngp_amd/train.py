@@ -85,8 +85,8 @@ class TrainConfig:
     # proposal_uniform_samples) in place of diagonal_n_steps.  The grid thresholds
     # at the smaller of the cap and the mean density
     density_threshold: float | None = None
-    # run the optimizer update on a side stream, overlapping the next step's ray
-    # generation and march (False: on the main stream, in order)
+    # without prefetch_march: run the optimizer update on a side stream, overlapping
+    # the next step's ray generation and march (False: on the main stream, in order)
     overlap_update: bool = True
     # make the next step's rays and samples (train_make_rays + march_rays or the guided
     # sampler) on a stream of their own while this step computes, into a second set
@@ -123,6 +123,22 @@ class OccupancyGrid:
 
     def mean_density_up_to_cascade(self, cas: int) -> torch.Tensor:
         return self.density[self.alive[: self.alive_offsets[cas]]].mean()
+
+
+_PF_STREAMS: dict = {}
+
+
+def _prefetch_stream(dev) -> torch.cuda.Stream:
+    """The prefetch stream of a device (default priority), created once per process:
+    HIP maps streams onto a few hardware queues in creation order, and a stream created
+    anew for every trainer sometimes shared the main stream's queue, which serialises
+    the prefetch with the step (measured: 0.77 or 0.85 ms per step from run to run)."""
+    key = torch.device(dev).index
+    if key is None:
+        key = torch.cuda.current_device()
+    if key not in _PF_STREAMS:
+        _PF_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return _PF_STREAMS[key]
 
 
 def density_threshold_from_min_step_size(diagonal_n_steps: int, bound: float) -> float:
@@ -214,12 +230,17 @@ class Trainer:
             self.bb = dict(dbg=z(n, 3), dparams=z(nb), mm=z(nb), mv=z(nb),
                            ws=torch.empty(_lib.sky_sphere_bg_workspace_bytes(sky), dtype=torch.uint8, device=self.dev))
         self.main_stream = torch.cuda.current_stream(self.dev)
-        self.side_stream = torch.cuda.Stream(self.dev) if cfg.overlap_update else self.main_stream  # default priority
+        # With the prefetch the next step's sampler already runs beside this step, and an
+        # update on a side stream would only add two cross-stream waits (about 30 us a
+        # step, measured) to the critical path: the update then runs in order.
+        side = cfg.overlap_update and not cfg.prefetch_march
+        self.side_stream = torch.cuda.Stream(self.dev) if side else self.main_stream  # default priority
         self._bwd_done = torch.cuda.Event()
         self._adam_done = torch.cuda.Event()
-        self.pf_stream = torch.cuda.Stream(self.dev) if cfg.prefetch_march else None  # default priority
+        self.pf_stream = _prefetch_stream(self.dev) if cfg.prefetch_march else None
         self._pf_ready = [torch.cuda.Event(), torch.cuda.Event()]  # a set's prefetched samples are written
-        self._set_free = [torch.cuda.Event(), torch.cuda.Event()]  # the last step that used a set is done
+        self._pf_gate = torch.cuda.Event()  # the main stream's work enqueued before a prefetch
+        self._pf_before = "mlp_bwd"  # the call of the step before which the prefetch is issued (measured best)
         self._pf_pending = [None, None]  # per set: (bitfield, step) of a prefetch not yet consumed
         self._calls = None
         self.mark_untrained_density_grid()
@@ -494,10 +515,12 @@ class Trainer:
         the previous update) -> hashgrid_encode -> nerf_mlp_forward -> integrate_rays ->
         train_loss_grad -> integrate_rays_backward -> nerf_mlp_backward ->
         hashgrid_encode_backward, then adam_update (table, MLP) on the side stream.
-        With `prefetch_march`, the next step's train_make_rays and sampler run on the
-        prefetch stream into the other buffer set right after this step's sampler, so
-        the latency-bound marcher overlaps this step's compute; a step whose samples
-        were prefetched starts with its encoder.  A prefetch is made only when nothing
+        With `prefetch_march`, the next step's train_make_rays and sampler are issued on
+        the prefetch stream into the other buffer set before this step's MLP backward, so
+        the latency-bound marcher overlaps the MLP backward, the table gradient and the
+        update (issued before the encoder, it slowed the encoder and MLP forward it ran
+        beside by about 40 us); a step whose samples were prefetched starts with its
+        encoder.  A prefetch is made only when nothing
         before the next step changes what it reads (see _prefetchable), and is dropped
         if the grid's bitfield was replaced meanwhile.  With a background model,
         sky_sphere_bg + train_gt_blend follow the wait and train_bg_grad +
@@ -520,16 +543,8 @@ class Trainer:
             self._pf_pending[cur] = None
         if pending is None or pending[0] is not bits or pending[1] != s0:
             self._rays_and_samples(k, s0, None)
-        # the next step's rays and samples, on the prefetch stream into the other set,
-        # once the step that last used that set is done
         nxt = 1 - cur
-        if self.pf_stream is not None and self._prefetchable(s0):
-            pf = self.pf_stream
-            pf.wait_event(self._set_free[nxt])
-            with torch.cuda.stream(pf):
-                self._rays_and_samples(self._calls[nxt], s0 + 1, pf.cuda_stream)
-            self._pf_ready[nxt].record(pf)
-            self._pf_pending[nxt] = (bits, s0 + 1)
+        prefetch = self.pf_stream is not None and self._prefetchable(s0)
         # rays and samples read no field parameters: they overlap the last step's update
         # on the side stream; everything after waits for it
         self.sync()
@@ -540,6 +555,19 @@ class Trainer:
             # their gradient after the integrator's (app/nerf/_utils.py:128-150)
             order = ("bg_fwd", "bg_blend") + order[:5] + ("bg_grad", "bg_bwd") + order[5:]
         for name in order:
+            if prefetch and name == self._pf_before:
+                # the next step's rays and samples, on the prefetch stream into the other
+                # set, after everything enqueued on the main stream so far (the last step
+                # that used that set, the grid update that may have followed it, and this
+                # step's kernels up to here): the latency-bound marcher then runs beside
+                # the rest of this step
+                pf = self.pf_stream
+                self._pf_gate.record(self.main_stream)
+                pf.wait_event(self._pf_gate)
+                with torch.cuda.stream(pf):
+                    self._rays_and_samples(self._calls[nxt], s0 + 1, pf.cuda_stream)
+                self._pf_ready[nxt].record(pf)
+                self._pf_pending[nxt] = (bits, s0 + 1)
             k[name]()
         self.step_count += 1
         t = self.step_count
@@ -583,7 +611,6 @@ class Trainer:
             k["p_adam_mlp"](_lib.adam_descriptor(_lib.PROPOSAL_N_PARAMS, lr, b1, b2, 1e-15, 1e-15, q1, q2, 1e-6,
                                                  mirror_dtype=_lib.DTYPE_F16 if self.prop.mlp_half is not None
                                                  else _lib.DTYPE_F32))
-        self._set_free[cur].record(self.main_stream)
         self._cur = nxt
         _lib.check()
         self.maybe_update_grid()

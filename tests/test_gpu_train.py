@@ -279,7 +279,7 @@ def test_overlapped_update_matches_in_order(small_data):
     gradient element near zero by a full +-lr whichever rounding it got.)"""
     res = {}
     for ov in (True, False):
-        tr = Trainer(small_data, TrainConfig(total_samples=1 << 16, overlap_update=ov))
+        tr = Trainer(small_data, TrainConfig(total_samples=1 << 16, overlap_update=ov, prefetch_march=False))
         tr.step()
         m1 = tr.model.mlp_params.detach().clone()
         tr.step()
