@@ -442,7 +442,7 @@ void proposal_loss(ngp_stream_t stream, void **buffers, const char *opaque, size
 typedef struct ProposalSampleDescriptor {
     uint32_t n_rays;
     uint32_t n_per_ray;   /* samples per ray (48), at most 64 */
-    float mass_threshold; /* below it the ray gets no samples and is marked (0.9) */
+    float mass_threshold; /* below it the ray gets no samples and is marked (0.98 renders) */
     uint32_t mode;        /* NGP_PROPOSAL_QUANTILES or NGP_PROPOSAL_SUPPORT */
     float eps;            /* SUPPORT: mass share left out at either end (1e-3) */
     /* marching statics of the occupancy grid (as MarchingDescriptor) */

@@ -318,7 +318,7 @@ PROPOSAL_QUANTILES = 0
 PROPOSAL_SUPPORT = 1
 
 
-def proposal_sample_descriptor(n_rays, n_per_ray=48, mass_threshold=0.9, mode=PROPOSAL_QUANTILES, eps=1e-3,
+def proposal_sample_descriptor(n_rays, n_per_ray=48, mass_threshold=0.98, mode=PROPOSAL_QUANTILES, eps=1e-3,
                                diagonal_n_steps=1024, K=1, G=128, bound=1.0, stepsize_portion=0.0,
                                empty_mass=0.0) -> bytes:
     return struct.pack("<2IfIf3I3f", n_rays, n_per_ray, mass_threshold, mode, eps, diagonal_n_steps, K, G, bound,
