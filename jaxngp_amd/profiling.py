@@ -24,6 +24,7 @@ from __future__ import annotations
 import collections
 import csv
 import glob
+import re
 import sys
 
 # the kernels each target launches, and one that runs exactly once per launch
@@ -51,6 +52,12 @@ EXTRA_KERNELS = ("mlp_forward_h_kernel", "mlp_backward_h_kernel", "field_chunks_
 
 
 def _short(name: str) -> str:
+    if name.startswith("_Z"):
+        # a mangled name the tool left as is: its first unqualified identifier after the
+        # namespaces (length-prefixed), e.g. _ZN3ngp12_GLOBAL__N_121mlp_backward_h_kernelE...
+        m = re.search(r"_GLOBAL__N_1(\d+)", name) or re.search(r"^_ZN3ngp(\d+)", name)
+        if m:
+            return name[m.end():m.end() + int(m.group(1))]
     n = name.replace("(anonymous namespace)::", "").replace("void ", "").replace("ngp::", "")
     return n if n.startswith("at::") else n.split("(")[0]
 

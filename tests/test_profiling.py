@@ -83,3 +83,10 @@ def test_gaps_counts_overlap_once_and_idle_time(tmp_path):
     # span 100 us; busy 10 + (15..80 = 65) = 75 us; idle 25 us
     assert "median span 100.0 us" in out and "kernels running 75.0 us" in out and "idle 25.0 us" in out
     assert "3 dispatches per step" in out
+
+
+def test_mangled_kernel_names_are_shortened():
+    from jaxngp_amd.profiling import _base
+    assert _base("_ZN3ngp12_GLOBAL__N_121mlp_backward_h_kernelEjbPKfS2_PKDF16_S2_P") == "mlp_backward_h_kernel"
+    assert _base("_ZN3ngp12_GLOBAL__N_119ogrid_update_kernelIDF16_Lb1EEEv18HashGri") == "ogrid_update_kernel"
+    assert _base("void ngp::(anonymous namespace)::bwd_bucket_kernel<2, false>(HashGridDescriptor)") == "bwd_bucket_kernel"
