@@ -218,15 +218,48 @@ __global__ void __launch_bounds__(kBlock)
         if (scale_state->overflow != 0u) return;  // skipped step: nothing changes
         gs = gs / scale_state->scale;
     }
+    // one element's update, in optax's order of operations
+    auto step = [&](float pk, float gk_raw, float &mk, float &vk) {
+        const float gk = gk_raw * gs;
+        mk = d.b1 * mk + (1.0f - d.b1) * gk;
+        vk = d.b2 * vk + (1.0f - d.b2) * (gk * gk);
+        const float mh = mk / c1, vh = vk / c2;
+        const float u = -d.lr * (mh / (sqrtf(vh + d.eps_root) + d.eps)) + d.weight_decay * pk;
+        return pk + u;
+    };
+    // four elements per thread and iteration with 16-byte accesses when every array
+    // allows them (the table's), else one
+    const bool vec = (d.n & 3u) == 0 && ((reinterpret_cast<uintptr_t>(p) | reinterpret_cast<uintptr_t>(g) |
+                                          reinterpret_cast<uintptr_t>(m) | reinterpret_cast<uintptr_t>(v)) & 15u) == 0 &&
+                     (mirror == nullptr || (reinterpret_cast<uintptr_t>(mirror) & (4 * sizeof(TT) - 1)) == 0);
+    if (vec) {
+        const size_t n4 = d.n / 4;
+        for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < n4; k += stride) {
+            const float4 g4 = reinterpret_cast<const float4 *>(g)[k];
+            float4 p4 = reinterpret_cast<float4 *>(p)[k];
+            float4 m4 = reinterpret_cast<float4 *>(m)[k];
+            float4 v4 = reinterpret_cast<float4 *>(v)[k];
+            p4.x = step(p4.x, g4.x, m4.x, v4.x);
+            p4.y = step(p4.y, g4.y, m4.y, v4.y);
+            p4.z = step(p4.z, g4.z, m4.z, v4.z);
+            p4.w = step(p4.w, g4.w, m4.w, v4.w);
+            reinterpret_cast<float4 *>(m)[k] = m4;
+            reinterpret_cast<float4 *>(v)[k] = v4;
+            reinterpret_cast<float4 *>(p)[k] = p4;
+            if (mirror != nullptr) {
+                mirror[4 * k + 0] = (TT)p4.x;
+                mirror[4 * k + 1] = (TT)p4.y;
+                mirror[4 * k + 2] = (TT)p4.z;
+                mirror[4 * k + 3] = (TT)p4.w;
+            }
+        }
+        return;
+    }
     for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < d.n; k += stride) {
-        const float gk = g[k] * gs;
-        const float mk = d.b1 * m[k] + (1.0f - d.b1) * gk;
-        const float vk = d.b2 * v[k] + (1.0f - d.b2) * (gk * gk);
+        float mk = m[k], vk = v[k];
+        const float pk = step(p[k], g[k], mk, vk);
         m[k] = mk;
         v[k] = vk;
-        const float mh = mk / c1, vh = vk / c2;
-        const float u = -d.lr * (mh / (sqrtf(vh + d.eps_root) + d.eps)) + d.weight_decay * p[k];
-        const float pk = p[k] + u;
         p[k] = pk;
         if (mirror != nullptr) mirror[k] = (TT)pk;
     }

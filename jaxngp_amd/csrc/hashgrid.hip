@@ -43,10 +43,10 @@ __global__ void __launch_bounds__(kBlock)
     if (gid >= (uint64_t)d.n_points * d.L) return;
     const uint32_t n = (uint32_t)(gid / d.L);
     const uint32_t l = (uint32_t)(gid - (uint64_t)n * d.L);
-    const float two_b = 2.0f * d.bound;
-    const float px = (pos[3 * (size_t)n + 0] + d.bound) / two_b;
-    const float py = (pos[3 * (size_t)n + 1] + d.bound) / two_b;
-    const float pz = (pos[3 * (size_t)n + 2] + d.bound) / two_b;
+    const Divisor two_b = make_divisor(2.0f * d.bound);
+    const float px = divide(two_b, pos[3 * (size_t)n + 0] + d.bound);
+    const float py = divide(two_b, pos[3 * (size_t)n + 1] + d.bound);
+    const float pz = divide(two_b, pos[3 * (size_t)n + 2] + d.bound);
     uint32_t idx[8];
     float w[8];
     level_vertices(px, py, pz, lt.scale[l], lt.res[l], l < d.first_hash_level, d.T, (d.T & (d.T - 1u)) == 0,
@@ -109,16 +109,16 @@ __global__ void __launch_bounds__(kBlock)
     const LevelBlock lb = level_block(d.n_points, d.L);
     if (!lb.live) return;
     const uint32_t l = lb.level;
-    const float two_b = 2.0f * d.bound;
+    const Divisor two_b = make_divisor(2.0f * d.bound);
     uint32_t idx[P][8];
     float w[P][8];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const uint32_t n = lb.point0 + j * kBlock + threadIdx.x;
         const uint32_t np = n < d.n_points ? n : d.n_points - 1;
-        const float px = (pos[3 * (size_t)np + 0] + d.bound) / two_b;
-        const float py = (pos[3 * (size_t)np + 1] + d.bound) / two_b;
-        const float pz = (pos[3 * (size_t)np + 2] + d.bound) / two_b;
+        const float px = divide(two_b, pos[3 * (size_t)np + 0] + d.bound);
+        const float py = divide(two_b, pos[3 * (size_t)np + 1] + d.bound);
+        const float pz = divide(two_b, pos[3 * (size_t)np + 2] + d.bound);
         level_vertices(px, py, pz, d.scales[l], d.resolutions[l], l < d.first_hash_level, d.T,
                        (d.T & (d.T - 1u)) == 0, d.offsets[l], d.n_rows - 1u, idx[j], w[j]);
     }
@@ -331,10 +331,10 @@ __device__ __forceinline__ void point_runs(const HashGridDescriptor &d, uint32_t
                                            bool dense, bool pow2, const float *xyz, const float *gin, bool active,
                                            Emit emit) {
     const int lane = lane_id();
-    const float two_b = 2.0f * d.bound;
-    const float px = (xyz[0] + d.bound) / two_b;
-    const float py = (xyz[1] + d.bound) / two_b;
-    const float pz = (xyz[2] + d.bound) / two_b;
+    const Divisor two_b = make_divisor(2.0f * d.bound);
+    const float px = divide(two_b, xyz[0] + d.bound);
+    const float py = divide(two_b, xyz[1] + d.bound);
+    const float pz = divide(two_b, xyz[2] + d.bound);
     uint32_t idx[8];
     float w[8];
     level_vertices(px, py, pz, scale, res, dense, d.T, pow2, d.offsets[l], d.n_rows - 1u, idx, w);

@@ -864,7 +864,7 @@ __global__ void __launch_bounds__(kThreads)
     const uint64_t n_tiles = (n_entries + kTile - 1) / kTile;
     const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / kWave;
     const uint32_t n_waves = gridDim.x * kWaves;
-    const float two_b = 2.0f * d.bound;
+    const Divisor two_b = make_divisor(2.0f * d.bound);
     const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
     for (uint64_t t = wave; t < n_tiles; t += n_waves) {
         const uint64_t e = t * kTile + i;
@@ -884,9 +884,9 @@ __global__ void __launch_bounds__(kThreads)
         }
         float px = 0.0f, py = 0.0f, pz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
         if (live) {
-            px = (xyzs[3 * e + 0] + d.bound) / two_b;
-            py = (xyzs[3 * e + 1] + d.bound) / two_b;
-            pz = (xyzs[3 * e + 2] + d.bound) / two_b;
+            px = divide(two_b, xyzs[3 * e + 0] + d.bound);
+            py = divide(two_b, xyzs[3 * e + 1] + d.bound);
+            pz = divide(two_b, xyzs[3 * e + 2] + d.bound);
             dx = rays_d[3 * (size_t)ray + 0];
             dy = rays_d[3 * (size_t)ray + 1];
             dz = rays_d[3 * (size_t)ray + 2];
@@ -1030,7 +1030,7 @@ __global__ void __launch_bounds__(kThreads)
     const uint32_t n_tiles = (a.n_update + kTile - 1) / kTile;
     const uint32_t wave = blockIdx.x * kWaves + threadIdx.x / kWave;
     const uint32_t n_waves = gridDim.x * kWaves;
-    const float two_b = 2.0f * a.bound;
+    const Divisor two_b = make_divisor(2.0f * a.bound);
     const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
     const uint32_t G3 = a.G * a.G * a.G;
     float *dens = density + (size_t)a.cascade * G3;
@@ -1046,7 +1046,8 @@ __global__ void __launch_bounds__(kThreads)
         if (live) {
             float wx, wy, wz;
             ogrid_point(a, m, j, wx, wy, wz);
-            const float px = (wx + a.bound) / two_b, py = (wy + a.bound) / two_b, pz = (wz + a.bound) / two_b;
+            const float px = divide(two_b, wx + a.bound), py = divide(two_b, wy + a.bound),
+                        pz = divide(two_b, wz + a.bound);
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
                 const float2 lo = encode_level(d, lt, table, px, py, pz, 8 * b + 2 * g);

@@ -82,6 +82,28 @@ inline uint32_t div_up(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) /
 // device math
 // ---------------------------------------------------------------------------
 
+// x / d for a divisor d shared by a whole launch: a power of two in the normal range
+// takes the multiply by its exact reciprocal (x / 2^k and x * 2^-k are the same exact
+// scaling, rounded alike, also below the normal range), any other d the division.
+struct Divisor {
+    float d, inv;
+    bool pow2;
+};
+
+__host__ __device__ inline Divisor make_divisor(float d) {
+    Divisor q;
+    q.d = d;
+    uint32_t b;
+    __builtin_memcpy(&b, &d, sizeof b);
+    q.pow2 = (b & 0x007fffffu) == 0u && d >= 1.1754944e-38f && d <= 8.5e37f;
+    const uint32_t ib = (254u << 23) - b;
+    __builtin_memcpy(&q.inv, &ib, sizeof ib);
+    if (!q.pow2) q.inv = 0.0f;
+    return q;
+}
+
+__device__ __forceinline__ float divide(const Divisor &q, float x) { return q.pow2 ? x * q.inv : x / q.d; }
+
 // Deterministic exp: exp(x) = 2^k exp(r), k = rint(x/ln2), r = x - k ln2 by the
 // two-constant reduction of Cody & Waite, exp(r) by its degree-7 Taylor polynomial
 // in Horner form with fused multiply-adds, 2^k applied as two exact power-of-two

@@ -408,10 +408,10 @@ __device__ __forceinline__ float bin_sigma(const HashGridDescriptor &g, const Le
     const int lane = lane_id();
     const float D = (t1 - t0) / (float)kBins;
     const float tc = t0 + ((float)lane + 0.5f) * D;
-    const float two_b = 2.0f * g.bound;
-    const float px = (rays_o[3 * (size_t)r + 0] + tc * rays_d[3 * (size_t)r + 0] + g.bound) / two_b;
-    const float py = (rays_o[3 * (size_t)r + 1] + tc * rays_d[3 * (size_t)r + 1] + g.bound) / two_b;
-    const float pz = (rays_o[3 * (size_t)r + 2] + tc * rays_d[3 * (size_t)r + 2] + g.bound) / two_b;
+    const Divisor two_b = make_divisor(2.0f * g.bound);
+    const float px = divide(two_b, rays_o[3 * (size_t)r + 0] + tc * rays_d[3 * (size_t)r + 0] + g.bound);
+    const float py = divide(two_b, rays_o[3 * (size_t)r + 1] + tc * rays_d[3 * (size_t)r + 1] + g.bound);
+    const float pz = divide(two_b, rays_o[3 * (size_t)r + 2] + tc * rays_d[3 * (size_t)r + 2] + g.bound);
     float x[kIn];
     const bool pow2 = (g.T & (g.T - 1u)) == 0;
 #pragma unroll

@@ -520,7 +520,9 @@ class Trainer:
         the latency-bound marcher overlaps the MLP backward, the table gradient and the
         update (issued before the encoder, it slowed the encoder and MLP forward it ran
         beside by about 40 us); a step whose samples were prefetched starts with its
-        encoder.  A prefetch is made only when nothing
+        encoder.  With a proposal, its update (every proposal_every-th step) runs just
+        before that point, on this step's samples and the forward's outputs, so a guided
+        next step reads the updated proposal.  A prefetch is made only when nothing
         before the next step changes what it reads (see _prefetchable), and is dropped
         if the grid's bitfield was replaced meanwhile.  With a background model,
         sky_sphere_bg + train_gt_blend follow the wait and train_bg_grad +
@@ -554,7 +556,12 @@ class Trainer:
             # per-ray backgrounds from the model, the ground truth blended over them, and
             # their gradient after the integrator's (app/nerf/_utils.py:128-150)
             order = ("bg_fwd", "bg_blend") + order[:5] + ("bg_grad", "bg_bwd") + order[5:]
+        train_prop = self.prop is not None and s0 % c.proposal_every == 0
         for name in order:
+            if name == self._pf_before and train_prop:
+                # the proposal's update, on this step's samples and the field's weights at
+                # them (the forward's outputs), before the next step's sampler reads it
+                self._train_proposal(k, s0 + 1)
             if prefetch and name == self._pf_before:
                 # the next step's rays and samples, on the prefetch stream into the other
                 # set, after everything enqueued on the main stream so far (the last step
@@ -600,20 +607,28 @@ class Trainer:
                 k["scale_update"]()
         self._adam_done.record(self.side_stream)
         self._adam_pending = True
-        if self.prop is not None and (t - 1) % c.proposal_every == 0:
-            for name in ("p_points", "p_encode", "p_mlp", "p_loss", "p_mlp_bwd", "p_encode_bwd"):
-                k[name]()
-            pm16 = self.prop.position_encoder.table.dtype == torch.float16
-            self.prop_t += 1  # the proposal's own update count for Adam's bias corrections
-            q1, q2 = float(np.float32(b1) ** self.prop_t), float(np.float32(b2) ** self.prop_t)
-            k["p_adam_table"](_lib.adam_descriptor(k["p_adam_table"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, q1, q2,
-                                                   0.0, mirror_dtype=_lib.DTYPE_F16 if pm16 else _lib.DTYPE_F32))
-            k["p_adam_mlp"](_lib.adam_descriptor(_lib.PROPOSAL_N_PARAMS, lr, b1, b2, 1e-15, 1e-15, q1, q2, 1e-6,
-                                                 mirror_dtype=_lib.DTYPE_F16 if self.prop.mlp_half is not None
-                                                 else _lib.DTYPE_F32))
         self._cur = nxt
         _lib.check()
         self.maybe_update_grid()
+
+    def _train_proposal(self, k: dict, t: int) -> None:
+        """One update of the proposal (its t-th step of training overall is the
+        trainer's step t): bin densities of the first proposal_rays rays, the
+        histogram-bound loss against the field's weights at this step's samples, the
+        backward and Adam."""
+        c = self.cfg
+        for name in ("p_points", "p_encode", "p_mlp", "p_loss", "p_mlp_bwd", "p_encode_bwd"):
+            k[name]()
+        b1, b2 = 0.9, 0.99
+        lr = lr_at(t - 1, c.lr)
+        pm16 = self.prop.position_encoder.table.dtype == torch.float16
+        self.prop_t += 1  # the proposal's own update count for Adam's bias corrections
+        q1, q2 = float(np.float32(b1) ** self.prop_t), float(np.float32(b2) ** self.prop_t)
+        k["p_adam_table"](_lib.adam_descriptor(k["p_adam_table"].keep[0].numel(), lr, b1, b2, 1e-15, 1e-15, q1, q2,
+                                               0.0, mirror_dtype=_lib.DTYPE_F16 if pm16 else _lib.DTYPE_F32))
+        k["p_adam_mlp"](_lib.adam_descriptor(_lib.PROPOSAL_N_PARAMS, lr, b1, b2, 1e-15, 1e-15, q1, q2, 1e-6,
+                                             mirror_dtype=_lib.DTYPE_F16 if self.prop.mlp_half is not None
+                                             else _lib.DTYPE_F32))
 
     def _guided_at(self, s: int) -> bool:
         """Whether step s takes the guided sampler's samples (else the marcher's)."""
@@ -628,13 +643,9 @@ class Trainer:
 
     def _prefetchable(self, s: int) -> bool:
         """Step s + 1's samples can be made during step s: the grid does not change
-        after step s, and a guided step s + 1 reads no proposal that step s updates."""
-        c = self.cfg
-        if self._grid_update_after(s):
-            return False
-        if self.prop is not None and self._guided_at(s + 1) and s % c.proposal_every == 0:
-            return False
-        return True
+        after step s.  (The proposal a guided step reads is updated in step s before the
+        prefetch is issued.)"""
+        return not self._grid_update_after(s)
 
     def _rays_and_samples(self, k: dict, s: int, stream) -> None:
         """train_make_rays and the sampler of step s on set k's buffers (on `stream`,
