@@ -19,6 +19,7 @@ Prints one JSON line.  Only one GPU is supported.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -187,12 +188,15 @@ def steady_state(tr, steps: int = 200) -> dict:
     counts = torch.zeros(steps, 4, dtype=torch.int32, device=tr.dev)
     first = tr.step_count
     torch.cuda.synchronize()
+    gc.collect()
+    gc.disable()
     t0 = time.perf_counter()
     for i in range(steps):
         tr.step()
         counts[i].copy_(tr.counters.view(torch.int32))
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    gc.enable()
     c = counts.long()
     n_samples = int((c[:, 0] - c[:, 1]).sum().item())
     n_valid = int(c[:, 2].sum().item())
@@ -273,6 +277,8 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
     # with one 16-byte copy per step, summed after the timed region
     counts = torch.zeros(steps, 4, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
+    gc.collect()
+    gc.disable()  # no collector pause inside the timed region (the host must keep ahead of the GPU)
     t_start = time.perf_counter()
     for i in range(steps):
         tr.step()
@@ -280,6 +286,7 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
     t_host = time.perf_counter() - t_start
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    gc.enable()
     tr._calls = plain
     c = counts.long()
     n_samples = int((c[:, 0] - c[:, 1]).sum().item())
