@@ -60,8 +60,9 @@ class TrainConfig:
     # `proposal_samples_per_ray` per ray at stratified quantiles of its predicted
     # mass, minus empty occupancy cells) instead of the occupancy-grid marcher's;
     # the proposal then learns from the field's weights at its own samples.  Needs
-    # `proposal`.  n_rays defaults to total_samples / 16 in this mode (rays through
-    # empty space keep no samples; the rays that do not fit are dropped)
+    # `proposal`.  n_rays defaults to total_samples / 32 in this mode (about 44 kept
+    # samples a ray fill the slots from the first ~6,000; the rays that do not fit are
+    # dropped, and each offered ray costs one proposal evaluation)
     proposal_guided: bool = False
     proposal_samples_per_ray: int = 32
     # guided mode: samples per ray added at stratified uniform positions, so the
@@ -169,7 +170,7 @@ class Trainer:
         S = cfg.total_samples
         if cfg.proposal_guided and not cfg.proposal:
             raise ValueError("proposal_guided needs proposal=True")
-        n = cfg.n_rays or (S // 16 if cfg.proposal_guided else S)
+        n = cfg.n_rays or (S // 32 if cfg.proposal_guided else S)
         self.n_rays = n
         f32, u32 = torch.float32, torch.uint32
         z = lambda *shape, dtype=f32: torch.zeros(*shape, dtype=dtype, device=self.dev)

@@ -207,7 +207,7 @@ def test_proposal_guided_training(small_data):
         assert math.isfinite(m["loss"]) and m["loss"] < 0.5 * first["loss"], (guided, first, m)
         finals[guided] = m["loss"]
         if guided:
-            assert tr.n_rays == (1 << 16) // 16
+            assert tr.n_rays == (1 << 16) // 32
             assert m["measured_batch_size_before_compaction"] > 0.25 * (1 << 16), m
             assert m["n_valid_rays"] > 0 and math.isfinite(m["proposal_loss"])
     assert finals[True] < 1.5 * finals[False], finals
