@@ -314,16 +314,24 @@ def train_to(tr, total_steps: int) -> None:
     torch.cuda.synchronize()
 
 
+# Proposal render settings the bench reports: (samples per ray, fallback mass threshold,
+# placement).  The first is the renderer's default; Trainer.proposal_render_settings names
+# the one each field is rendered at (the last two for a proposal-guided field).
+PROPOSAL_RENDER_SETTINGS = ((48, 0.98, "quantiles"), (48, 0.9, "support"), (32, 0.4, "support"),
+                            (32, 0.5, "support"))
+
+
 def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False, empty_mass: float = 0.0) -> dict:
     """ms per 800x800 frame of the trained field from held-out seeded poses, with the
     reference's ray-pool renderer (march_steps_cap 8, white background) on a pool of
     2^18 slots, median over the timed frames after three warm-up renders; PSNR of the
     u8 render against the float64 ground truth blended on white.  With `proposal`,
-    the frames are rendered with 48 proposal-placed samples per ray (occupancy-grid
-    fallback below predicted mass 0.98, on its own pool of 2^16 slots, the
-    renderer's default) and the same field is also rendered the plain
-    way, for the PSNR difference; the same comparison at the fallback threshold 0.9
-    is reported beside it."""
+    the frames are rendered with `render_image_proposal` at each of
+    PROPOSAL_RENDER_SETTINGS (occupancy-grid fallback below the predicted mass
+    threshold, on its own pool of 2^16 slots) and the same field is also rendered the
+    plain way, for the PSNR difference; the top-level figures are those of the
+    trainer's own setting (Trainer.proposal_render_settings), `settings` holds every
+    setting's."""
     import torch
     from jaxngp_amd import _lib
     from jaxngp_amd import scene as sc
@@ -335,54 +343,54 @@ def bench_inference(tr, data, n_frames: int, dev, proposal: bool = False, empty_
     kw = dict(K=tr.K, G=c.grid_res, diagonal_n_steps=c.diagonal_n_steps, stepsize_portion=tr.portion,
               march_steps_cap=8, pool=POOL)
     pose_t = [torch.as_tensor(p, dtype=torch.float32, device=dev) for p in poses]
-    plain = lambda pose: render_image_inference(tr.model, data.camera, pose, tr.grid.bitfield, **kw)
-    if proposal:
-        render = lambda pose, st=None, **o: render_image_proposal(tr.model, tr.prop, data.camera, pose,
-                                                                  tr.grid.bitfield, stats=st, empty_mass=empty_mass,
-                                                                  **o, **kw)
-    else:
-        render = lambda pose, st=None: plain(pose)
-    for i in range(3):
-        render(pose_t[i % n_frames])
-    times, psnrs, plain_psnrs, fallback, psnrs09, fallback09, times09 = [], [], [], [], [], [], []
-    for i in range(n_frames):
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        st = {}
-        _, rgb, _ = render(pose_t[i], st)
-        torch.cuda.synchronize()
-        times.append((time.perf_counter() - t) * 1e3)
-        ref = sc.blend_u8(gt.rgba[i], (1.0, 1.0, 1.0))
-        psnrs.append(sc.psnr_u8(rgb, ref))
-        if proposal:
-            fallback.append(st["fallback_rays"] / (st["fallback_rays"] + st["proposal_rays"]))
-            plain_psnrs.append(sc.psnr_u8(plain(pose_t[i])[1], ref))
+    refs = [sc.blend_u8(gt.rgba[i], (1.0, 1.0, 1.0)) for i in range(n_frames)]
+    plain = lambda pose, st=None: render_image_inference(tr.model, data.camera, pose, tr.grid.bitfield, **kw)
+    placements = {"quantiles": _lib.PROPOSAL_QUANTILES, "support": _lib.PROPOSAL_SUPPORT}
+
+    def frames(render):
+        for i in range(3):
+            render(pose_t[i % n_frames], {})
+        times, psnrs, fallback = [], [], []
+        for i in range(n_frames):
             st = {}
             torch.cuda.synchronize()
             t = time.perf_counter()
-            rgb09 = render(pose_t[i], st, mass_threshold=0.9, placement=_lib.PROPOSAL_SUPPORT)[1]
+            _, rgb, _ = render(pose_t[i], st)
             torch.cuda.synchronize()
-            times09.append((time.perf_counter() - t) * 1e3)
-            psnrs09.append(sc.psnr_u8(rgb09, ref))
-            fallback09.append(st["fallback_rays"] / (st["fallback_rays"] + st["proposal_rays"]))
-    times.sort()
-    res = {"ms_per_frame": times[len(times) // 2], "frames": n_frames, "psnr_db": float(np.mean(psnrs)),
-           "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
-           "train_steps": tr.step_count}
-    if proposal:
-        res.update(samples_per_ray=48, mass_threshold=0.98, placement="quantiles", empty_mass=empty_mass,
-                   fallback_ray_fraction=float(np.mean(fallback)),
-                   psnr_plain_render_db=float(np.mean(plain_psnrs)),
-                   delta_psnr_db=float(np.mean(psnrs) - np.mean(plain_psnrs)),
-                   worst_view_delta_psnr_db=float(min(a - b for a, b in zip(psnrs, plain_psnrs))),
-                   at_mass_threshold_0_9={"placement": "support", "ms_per_frame": sorted(times09)[len(times09) // 2],
-                                          "delta_psnr_db": float(np.mean(psnrs09) - np.mean(plain_psnrs)),
-                                          "fallback_ray_fraction": float(np.mean(fallback09))})
-    log(f"inference{' (proposal)' if proposal else ''}: {res['ms_per_frame']:.2f} ms/frame (median of {n_frames}), "
-        f"PSNR {res['psnr_db']:.2f} dB after {tr.step_count} steps" +
-        (f", plain render of the same field {res['psnr_plain_render_db']:.2f} dB, fallback rays "
-         f"{res['fallback_ray_fraction']:.1%}; at mass threshold 0.9: delta "
-         f"{res['at_mass_threshold_0_9']['delta_psnr_db']:+.3f} dB" if proposal else ""))
+            times.append((time.perf_counter() - t) * 1e3)
+            psnrs.append(sc.psnr_u8(rgb, refs[i]))
+            if st:
+                fallback.append(st["fallback_rays"] / max(1, st["fallback_rays"] + st["proposal_rays"]))
+        return sorted(times)[len(times) // 2], psnrs, fallback
+
+    base = {"frames": n_frames, "resolution": "800x800", "pool": POOL, "march_steps_cap": 8, "background": "white",
+            "train_steps": tr.step_count}
+    ms, plain_psnrs, _ = frames(plain)
+    if not proposal:
+        res = dict(ms_per_frame=ms, psnr_db=float(np.mean(plain_psnrs)), **base)
+        log(f"inference: {ms:.2f} ms/frame (median of {n_frames}), PSNR {res['psnr_db']:.2f} dB after "
+            f"{tr.step_count} steps")
+        return res
+    settings = []
+    for n_per_ray, thr, placement in PROPOSAL_RENDER_SETTINGS:
+        render = lambda pose, st: render_image_proposal(tr.model, tr.prop, data.camera, pose, tr.grid.bitfield,
+                                                        n_per_ray=n_per_ray, mass_threshold=thr,
+                                                        placement=placements[placement], stats=st,
+                                                        empty_mass=empty_mass, **kw)
+        pms, psnrs, fallback = frames(render)
+        settings.append({"samples_per_ray": n_per_ray, "mass_threshold": thr, "placement": placement,
+                         "empty_mass": empty_mass, "ms_per_frame": pms, "psnr_db": float(np.mean(psnrs)),
+                         "delta_psnr_db": float(np.mean(psnrs) - np.mean(plain_psnrs)),
+                         "worst_view_delta_psnr_db": float(min(a - b for a, b in zip(psnrs, plain_psnrs))),
+                         "fallback_ray_fraction": float(np.mean(fallback))})
+        log(f"inference (proposal, {n_per_ray} samples, threshold {thr}, {placement}): {pms:.2f} ms/frame, "
+            f"PSNR {settings[-1]['psnr_db']:.2f} dB, delta {settings[-1]['delta_psnr_db']:+.3f} dB "
+            f"(worst view {settings[-1]['worst_view_delta_psnr_db']:+.3f}), fallback rays "
+            f"{settings[-1]['fallback_ray_fraction']:.1%}")
+    own = tr.proposal_render_settings
+    own = (own["n_per_ray"], own["mass_threshold"], {v: k for k, v in placements.items()}[own["placement"]])
+    res = dict(settings[PROPOSAL_RENDER_SETTINGS.index(own)], **base)
+    res.update(psnr_plain_render_db=float(np.mean(plain_psnrs)), ms_per_frame_plain_render=ms, settings=settings)
     return res
 
 
@@ -544,17 +552,21 @@ def main() -> None:
             # the target's ratios, steady state (after the render_steps of training)
             pf, pl = out["proposal_fp16"], out
             ctl = out["plain_fp32_at_guided_threshold"]
+            fr = pf["inference"]
             out["target"] = {
                 "train_samples_per_s_ratio": pf["steady_state"]["value"] / pl["steady_state"]["value"],
                 "train_rays_per_s_ratio": pf["steady_state"]["rays_per_s"] / pl["steady_state"]["rays_per_s"],
-                "frame_ms_ratio": pf["inference"]["ms_per_frame"] / pl["inference"]["ms_per_frame"],
-                "delta_psnr_db": pf["inference"]["psnr_db"] - pl["inference"]["psnr_db"],
-                "same_field_delta_psnr_db": pf["inference"]["delta_psnr_db"],
+                "render_setting": {k: fr[k] for k in ("samples_per_ray", "mass_threshold", "placement")},
+                "frame_ms_ratio": fr["ms_per_frame"] / pl["inference"]["ms_per_frame"],
+                "frame_ms_ratio_renderer_default": fr["settings"][0]["ms_per_frame"] / pl["inference"]["ms_per_frame"],
+                "delta_psnr_db": fr["psnr_db"] - pl["inference"]["psnr_db"],
+                "same_field_delta_psnr_db": fr["delta_psnr_db"],
+                "same_field_worst_view_delta_psnr_db": fr["worst_view_delta_psnr_db"],
                 "vs_plain_at_guided_threshold": {
                     "train_samples_per_s_ratio": pf["steady_state"]["value"] / ctl["steady_state"]["value"],
                     "train_rays_per_s_ratio": pf["steady_state"]["rays_per_s"] / ctl["steady_state"]["rays_per_s"],
-                    "frame_ms_ratio": pf["inference"]["ms_per_frame"] / ctl["inference"]["ms_per_frame"],
-                    "delta_psnr_db": pf["inference"]["psnr_db"] - ctl["inference"]["psnr_db"]},
+                    "frame_ms_ratio": fr["ms_per_frame"] / ctl["inference"]["ms_per_frame"],
+                    "delta_psnr_db": fr["psnr_db"] - ctl["inference"]["psnr_db"]},
                 "note": "plain = the reference configuration (occupancy threshold cap 0.01 * 1024 / (2 sqrt 3)); "
                         "the guided path's cap follows the same formula with its 64 samples per ray",
             }

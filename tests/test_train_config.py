@@ -28,3 +28,16 @@ def test_explicit_cap_wins():
     assert Trainer.threshold_cap_of(TrainConfig(density_threshold=0.5), 1.0) == 0.5
     c = TrainConfig(precision="fp16", proposal=True, proposal_guided=True, density_threshold=2.0)
     assert Trainer.threshold_cap_of(c, 1.0) == 2.0
+
+
+def test_proposal_render_settings_follow_how_the_proposal_trained():
+    from jaxngp_amd import _lib
+    guided = Trainer.proposal_render_settings_of(TrainConfig(precision="fp16", proposal=True, proposal_guided=True))
+    assert guided == {"n_per_ray": 32, "mass_threshold": 0.5, "placement": _lib.PROPOSAL_SUPPORT}
+    beside = Trainer.proposal_render_settings_of(TrainConfig(precision="fp16", proposal=True))
+    assert beside == {"n_per_ray": 48, "mass_threshold": 0.98, "placement": _lib.PROPOSAL_QUANTILES}
+    # the marcher-trained settings are render_image_proposal's own defaults
+    import inspect
+    from jaxngp_amd.renderer import render_image_proposal
+    sig = inspect.signature(render_image_proposal).parameters
+    assert {k: sig[k].default for k in beside} == beside
