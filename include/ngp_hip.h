@@ -512,7 +512,10 @@ typedef struct ProposalTrainDescriptor {
  *      occupancy_bitfield u8[K*G^3/8]
  * out: as march_rays (next_sample_write_location u32[1], number_of_exceeded_samples
  *      u32[1], ray_is_valid bool[n], rays_n_samples u32[n], rays_sample_startidx
- *      u32[n], idcs u32[S], xyzs f32[S,3], dirs f32[S,3], dss f32[S], z_vals f32[S]) */
+ *      u32[n], idcs u32[S], xyzs f32[S,3], dirs f32[S,3], dss f32[S], z_vals f32[S]),
+ *      then workspace f32[5, n, 64] (scratch: the counting pass stages each kept
+ *      sample there, so the proposal is evaluated once per ray; contents undefined
+ *      afterwards; may be omitted when n = 0) */
 void proposal_train_samples(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
 /* ---------------------------------------------------------------------------

@@ -132,13 +132,23 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Column k of the workspace summed over its rows: one workgroup per column, a
+// strided partial sum per thread, the wavefront sums, then the wavefronts' in order
+// (a thread per column walking all rows waited on one load per row: 58 us at 512 rows).
 __global__ void __launch_bounds__(kBlock)
     proposal_sum_rows_kernel(uint32_t rows, const float *__restrict__ ws, float *__restrict__ out) {
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= (uint32_t)kParams) return;
+    __shared__ float part[kBlock / kWave];
+    const uint32_t k = blockIdx.x;
     float s = 0.0f;
-    for (uint32_t r = 0; r < rows; ++r) s += ws[(size_t)r * kParams + k];
-    out[k] = s;
+    for (uint32_t r = threadIdx.x; r < rows; r += kBlock) s += ws[(size_t)r * kParams + k];
+    s = wave_sumf(s);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kBlock / kWave; ++w) t += part[w];
+        out[k] = t;
+    }
 }
 
 // Inclusive product scan over the wavefront.
@@ -471,74 +481,99 @@ __global__ void __launch_bounds__(kBlock)
 // Training samples placed by the proposal, in march_rays' output layout (rays'
 // samples contiguous in ray order, invalid rays and the unused tail empty), so the
 // rest of the training step runs unchanged on them.  Stratified: sample j of ray r
-// sits at the (j + noise_r) / n_per_ray quantile.  Three passes, no scratch:
-//   1. count: the kept samples per ray -> rays_n_samples;
+// sits at the (j + noise_r) / n_per_ray quantile.  Passes:
+//   1. count: the kept samples per ray -> rays_n_samples, and each kept sample's t,
+//      ds and position staged at [r * 64 + rank] of the caller's workspace;
 //   2. scan (one workgroup): start = exclusive prefix in ray order; a ray is valid
 //      iff t_end > t_start and its samples fit (start + n <= total_samples), as
 //      march_rays decides; next_sample_write_location = every ray's count,
 //      number_of_exceeded_samples = the counts of the rays that did not fit;
-//   3. write: each valid ray recomputes its plan and writes its kept samples at
-//      its start; the samples past the valid ones are zeroed.
+//   3. write: each valid ray's staged samples are copied to its start;
+//   4. the samples past the valid ones are zeroed.
 // ---------------------------------------------------------------------------
 
-template <typename TT, typename PT, bool kWrite>
+// Workspace (stage): five f32 planes of n_rays * 64, the t, ds, x, y and z of ray r's
+// kept sample k at [r * 64 + k] of each.
+template <typename TT, typename PT>
 __global__ void __launch_bounds__(kBlock)
     proposal_train_kernel(uint32_t n_rays, HashGridDescriptor g, SampleArgs sa, const float *__restrict__ rays_o,
                           const float *__restrict__ rays_d, const float *__restrict__ t0s,
                           const float *__restrict__ t1s, const float *__restrict__ noises,
                           const TT *__restrict__ table, const PT *__restrict__ params, const uint8_t *__restrict__ bits,
-                          uint32_t *__restrict__ rays_n_samples, const uint32_t *__restrict__ startidx,
-                          const bool *__restrict__ valid, uint32_t *__restrict__ idcs, float *__restrict__ xyzs,
-                          float *__restrict__ dirs, float *__restrict__ dss, float *__restrict__ zs) {
+                          uint32_t *__restrict__ rays_n_samples, float *__restrict__ stage) {
     __shared__ float w[kParams];
     __shared__ LevelTables lt;
     stage_proposal(g, w, lt, params);
     const int wave = (int)threadIdx.x / kWave, lane = lane_id();
     const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
     if (r >= n_rays) return;
-    if (kWrite && (!valid[r] || rays_n_samples[r] == 0)) return;
     const float t0 = t0s[r], t1 = t1s[r];
     const float sigma_b = bin_sigma(g, lt, w, table, rays_o, rays_d, r, t0, t1);
     const SamplePlan pl = plan_samples(sa, r, sigma_b, t0, t1, noises[r], rays_o, rays_d, bits);
     const uint64_t kept = __ballot(pl.keep);
-    if (!kWrite) {
-        if (lane == 0) rays_n_samples[r] = (uint32_t)__popcll(kept);
-        return;
-    }
-    const uint32_t rank = (uint32_t)__popcll(kept & ((1ull << lane) - 1ull));
-    if (pl.keep && rank < rays_n_samples[r]) {  // the count pass's plan: never past the ray's range
-        const size_t j = (size_t)startidx[r] + rank;
-        idcs[j] = r;
+    if (pl.keep) {
+        const size_t plane = (size_t)n_rays * kBins;
+        const size_t at = (size_t)r * kBins + (uint32_t)__popcll(kept & ((1ull << lane) - 1ull));
+        stage[at] = pl.t;
+        stage[plane + at] = pl.ds;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xyzs[3 * j + c] = pl.p[c];
-            dirs[3 * j + c] = rays_d[3 * (size_t)r + c];
-        }
-        dss[j] = pl.ds;
-        zs[j] = pl.t;
+        for (int c = 0; c < 3; ++c) stage[(2 + c) * plane + at] = pl.p[c];
     }
+    if (lane == 0) rays_n_samples[r] = (uint32_t)__popcll(kept);
 }
 
-constexpr int kTrainScanThreads = 1024;
+__global__ void __launch_bounds__(kBlock)
+    proposal_train_write_kernel(uint32_t n_rays, const float *__restrict__ rays_d, const float *__restrict__ stage,
+                                const uint32_t *__restrict__ rays_n_samples, const uint32_t *__restrict__ startidx,
+                                const bool *__restrict__ valid, uint32_t *__restrict__ idcs, float *__restrict__ xyzs,
+                                float *__restrict__ dirs, float *__restrict__ dss, float *__restrict__ zs) {
+    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + wave;
+    if (r >= n_rays || !valid[r]) return;
+    if ((uint32_t)lane >= rays_n_samples[r]) return;  // <= 64: a count of one ballot
+    const size_t plane = (size_t)n_rays * kBins, at = (size_t)r * kBins + lane;
+    const size_t j = (size_t)startidx[r] + lane;
+    idcs[j] = r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        xyzs[3 * j + c] = stage[(2 + c) * plane + at];
+        dirs[3 * j + c] = rays_d[3 * (size_t)r + c];
+    }
+    dss[j] = stage[plane + at];
+    zs[j] = stage[at];
+}
+
+// Eight consecutive rays per thread, so 8,192 rays take one block scan.
+constexpr int kTrainScanThreads = 1024, kTrainScanPer = 8;
 __global__ void __launch_bounds__(kTrainScanThreads)
     proposal_train_scan_kernel(uint32_t n_rays, uint32_t total_samples, const float *__restrict__ t0s,
                                const float *__restrict__ t1s, uint32_t *__restrict__ next_loc,
                                uint32_t *__restrict__ exceeded, bool *__restrict__ valid,
                                uint32_t *__restrict__ rays_n_samples, uint32_t *__restrict__ startidx) {
     uint64_t carry = 0, over = 0;
-    for (uint32_t c = 0; c < n_rays; c += kTrainScanThreads) {
-        const uint32_t r = c + threadIdx.x;
-        const uint32_t n = r < n_rays ? rays_n_samples[r] : 0u;
+    for (uint64_t c = 0; c < n_rays; c += (uint64_t)kTrainScanThreads * kTrainScanPer) {
+        const uint64_t r0 = c + (uint64_t)threadIdx.x * kTrainScanPer;
+        uint32_t n[kTrainScanPer], sum = 0;
+#pragma unroll
+        for (int q = 0; q < kTrainScanPer; ++q) {
+            n[q] = r0 + q < n_rays ? rays_n_samples[r0 + q] : 0u;
+            sum += n[q];
+        }
         uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<kTrainScanThreads>(n, &tot);
+        const uint32_t ex = block_exclusive_scan<kTrainScanThreads>(sum, &tot);
+        uint64_t start = carry + ex;
         uint32_t miss = 0;
-        if (r < n_rays) {
-            const uint64_t start = carry + ex;
-            const bool ok = t1s[r] > t0s[r] && start < total_samples && start + n <= total_samples;
-            valid[r] = ok;
-            rays_n_samples[r] = ok ? n : 0u;
-            startidx[r] = ok && n > 0 ? (uint32_t)start : 0u;
-            miss = ok ? 0u : n;
+#pragma unroll
+        for (int q = 0; q < kTrainScanPer; ++q) {
+            const uint64_t r = r0 + q;
+            if (r < n_rays) {
+                const bool ok = t1s[r] > t0s[r] && start < total_samples && start + n[q] <= total_samples;
+                valid[r] = ok;
+                rays_n_samples[r] = ok ? n[q] : 0u;
+                startidx[r] = ok && n[q] > 0 ? (uint32_t)start : 0u;
+                miss += ok ? 0u : n[q];
+            }
+            start += n[q];
         }
         uint32_t miss_tot;
         block_exclusive_scan<kTrainScanThreads>(miss, &miss_tot);
@@ -641,7 +676,7 @@ extern "C" void proposal_mlp_backward(ngp_stream_t stream, void **buffers, const
                            (const float *)buffers[0], (const float *)buffers[1], (const float *)buffers[2],
                            (float *)buffers[3], (float *)buffers[5]);
     if (!check_launch(target)) return;
-    hipLaunchKernelGGL(proposal_sum_rows_kernel, dim3(div_up(kParams, kBlock)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(proposal_sum_rows_kernel, dim3(kParams), dim3(kBlock), 0, s,
                        d.n_workspace_blocks, (const float *)buffers[5], (float *)buffers[4]);
     check_launch(target);
 }
@@ -790,7 +825,7 @@ extern "C" void proposal_train_samples(ngp_stream_t stream, void **buffers, cons
         return;
     }
     sa.n_uniform = d.n_uniform;
-    if (!check_buffers(buffers, 18, target)) return;
+    if (!check_buffers(buffers, n > 0 ? 19 : 18, target)) return;
     hipStream_t s = (hipStream_t)stream;
     const float *ro = (const float *)buffers[0], *rd = (const float *)buffers[1];
     const float *t0 = (const float *)buffers[2], *t1 = (const float *)buffers[3], *noise = (const float *)buffers[4];
@@ -802,28 +837,26 @@ extern "C" void proposal_train_samples(ngp_stream_t stream, void **buffers, cons
     float *zs = (float *)buffers[17];
     const bool t16 = g.table_dtype == NGP_DTYPE_F16, p16 = d.render.param_dtype == NGP_DTYPE_F16;
     const dim3 grid(std::max<uint32_t>(1, div_up(n, kBlock / kWave)));
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
-            hipLaunchKernelGGL(proposal_train_scan_kernel, dim3(1), dim3(kTrainScanThreads), 0, s, n, d.total_samples,
-                               t0, t1, next_loc, exceeded, valid, ns, start);
-            if (!check_launch(target)) return;
-        }
-        if (n == 0) continue;
-#define NGP_TRAIN_LAUNCH(TT, PT, W)                                                                                 \
-    hipLaunchKernelGGL((proposal_train_kernel<TT, PT, W>), grid, dim3(kBlock), 0, s, n, g, sa, ro, rd, t0, t1, noise, \
-                       (const TT *)buffers[5], (const PT *)buffers[6], bits, ns, start, valid, idcs, xyzs, dirs, dss, zs)
-        if (pass == 0) {
-            if (t16 && p16) NGP_TRAIN_LAUNCH(__half, __half, false);
-            else if (t16) NGP_TRAIN_LAUNCH(__half, float, false);
-            else if (p16) NGP_TRAIN_LAUNCH(float, __half, false);
-            else NGP_TRAIN_LAUNCH(float, float, false);
-        } else {
-            if (t16 && p16) NGP_TRAIN_LAUNCH(__half, __half, true);
-            else if (t16) NGP_TRAIN_LAUNCH(__half, float, true);
-            else if (p16) NGP_TRAIN_LAUNCH(float, __half, true);
-            else NGP_TRAIN_LAUNCH(float, float, true);
-        }
+    if (n > 0) {
+        float *stage = (float *)buffers[18];
+#define NGP_TRAIN_LAUNCH(TT, PT)                                                                                   \
+    hipLaunchKernelGGL((proposal_train_kernel<TT, PT>), grid, dim3(kBlock), 0, s, n, g, sa, ro, rd, t0, t1, noise, \
+                       (const TT *)buffers[5], (const PT *)buffers[6], bits, ns, stage)
+        if (t16 && p16) NGP_TRAIN_LAUNCH(__half, __half);
+        else if (t16) NGP_TRAIN_LAUNCH(__half, float);
+        else if (p16) NGP_TRAIN_LAUNCH(float, __half);
+        else NGP_TRAIN_LAUNCH(float, float);
 #undef NGP_TRAIN_LAUNCH
+        if (!check_launch(target)) return;
+        hipLaunchKernelGGL(proposal_train_scan_kernel, dim3(1), dim3(kTrainScanThreads), 0, s, n, d.total_samples, t0,
+                           t1, next_loc, exceeded, valid, ns, start);
+        if (!check_launch(target)) return;
+        hipLaunchKernelGGL(proposal_train_write_kernel, grid, dim3(kBlock), 0, s, n, rd, stage, ns, start, valid, idcs,
+                           xyzs, dirs, dss, zs);
+        if (!check_launch(target)) return;
+    } else {
+        hipLaunchKernelGGL(proposal_train_scan_kernel, dim3(1), dim3(kTrainScanThreads), 0, s, n, d.total_samples, t0,
+                           t1, next_loc, exceeded, valid, ns, start);
         if (!check_launch(target)) return;
     }
     const uint32_t tail_grid = std::max<uint32_t>(1, std::min<uint32_t>(div_up(d.total_samples, kBlock), 1024));

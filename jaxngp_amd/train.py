@@ -185,6 +185,9 @@ class Trainer:
             loss=z(1), g_rgbd=z(n, 4), dbgs=z(n, 3), dz=z(S), ddrgbs=z(S, 4), denc=z(16, S, 2), dparams=z(N_PARAMS),
             ws=z(mlp_workspace_blocks(S), N_PARAMS), dtable=z(*self.model.position_encoder.table.shape),
         )
+        if cfg.proposal_guided:
+            # the guided sampler's staging workspace (f32[5, n, 64])
+            self.b["pstage"] = torch.empty(5 * n * 64, dtype=f32, device=self.dev)
         enc_mod = self.model.position_encoder
         self.b["hg_ws"] = torch.empty(
             _lib.hashgrid_backward_workspace_bytes(enc_mod.descriptor(S, self.bound, _lib.ENC_LEVEL_MAJOR)),
@@ -254,6 +257,8 @@ class Trainer:
         b2 = dict(b)
         for key in self.RAY_KEYS:
             b2[key] = torch.zeros_like(b[key])
+        if "pstage" in b:  # the guided sampler's scratch, per set
+            b2["pstage"] = torch.empty_like(b["pstage"])
         cnt = torch.zeros_like(b["counters"])
         b2.update(counters=cnt, next_loc=cnt[0:1], exceeded=cnt[1:2], n_valid=cnt[2:3], measured=cnt[3:4])
         return b2
@@ -451,7 +456,7 @@ class Trainer:
                 _lib.proposal_sample_descriptor(n, c.proposal_samples_per_ray, 0.0, _lib.PROPOSAL_QUANTILES, 1e-3,
                                                 c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion), ppd)
             guided = P("proposal_train_samples", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], ppe.table.detach(),
-                                                  pw, self.grid.bitfield] + march_out,
+                                                  pw, self.grid.bitfield] + march_out + [b["pstage"]],
                        _lib.proposal_train_descriptor(rdesc, S, c.proposal_uniform_samples))
         march = P("march_rays", [b["o"], b["d"], b["t0"], b["t1"], b["noise"], self.grid.bitfield] + march_out,
                   _lib.marching_descriptor(n, S, c.diagonal_n_steps, self.K, c.grid_res, self.bound, self.portion))

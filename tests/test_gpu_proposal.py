@@ -268,8 +268,9 @@ def _train_samples(prop, ot, dt, t0, t1, noise, bt, N, S):
     w, pd = prop.weights()
     rdesc = _lib.proposal_render_descriptor(prop.position_encoder.descriptor(0, 1.0),
                                             _lib.proposal_sample_descriptor(n, N, 0.0), pd)
+    stage = torch.full((5 * max(n, 1) * 64,), 7.0, device=dev)  # scratch
     _lib.call("proposal_train_samples", [ot, dt, t0, t1, noise, prop.position_encoder.table.detach(), w, bt] +
-              list(out.values()), _lib.proposal_train_descriptor(rdesc, S))
+              list(out.values()) + [stage], _lib.proposal_train_descriptor(rdesc, S))
     torch.cuda.synchronize()
     return {k: v.cpu() for k, v in out.items()}
 
@@ -388,8 +389,9 @@ def test_proposal_train_samples_with_uniform_samples(dev):
     out = {k: torch.empty_like(v, device=dev) for k, v in base.items()}
     rdesc = _lib.proposal_render_descriptor(prop.position_encoder.descriptor(0, 1.0),
                                             _lib.proposal_sample_descriptor(n, N, 0.0), pd)
+    stage = torch.empty(5 * n * 64, device=dev)
     _lib.call("proposal_train_samples", [ot, dt, t0, t1, nz, prop.position_encoder.table.detach(), w, full] +
-              list(out.values()), _lib.proposal_train_descriptor(rdesc, S, U))
+              list(out.values()) + [stage], _lib.proposal_train_descriptor(rdesc, S, U))
     got = {k: v.cpu() for k, v in out.items()}
     T0, T1 = t0.cpu().numpy(), t1.cpu().numpy()
     seg = T1 > T0
