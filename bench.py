@@ -39,6 +39,9 @@ MARCH_BYTES_PER_SAMPLE, MARCH_BYTES_PER_RAY = 36, 45
 INT_FWD_BYTES_PER_SAMPLE, INT_FWD_BYTES_PER_RAY = 24, 40
 INT_BWD_BYTES_PER_SAMPLE, INT_BWD_BYTES_PER_RAY = 44, 68
 MLP_FWD_FLOP = 2 * (32 * 64 + 64 * 16 + 32 * 64 + 64 * 64 + 64 * 3)  # 18816
+# the table gradient with the optimizer step fused: per table parameter, params, m
+# and v read and written once (f32)
+ADAM_BYTES_PER_PARAM = 6 * 4
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
@@ -208,8 +211,8 @@ def steady_state(tr, steps: int = 200) -> dict:
 
 
 CALIBRATION_STEPS = 30
-ROOFLINE_CANDIDATES = ("hashgrid_encode", "hashgrid_encode_backward", "march_rays", "integrate_rays",
-                       "integrate_rays_backward", "nerf_mlp_forward", "nerf_mlp_backward")
+ROOFLINE_CANDIDATES = ("hashgrid_encode", "hashgrid_encode_backward", "hashgrid_encode_backward_adam", "march_rays",
+                       "integrate_rays", "integrate_rays_backward", "nerf_mlp_forward", "nerf_mlp_backward")
 
 
 def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
@@ -233,7 +236,7 @@ def run_training(data, cfg, steps: int, warmup: int, dev) -> dict:
             log(f"[{cfg.precision}] warmup step {i}: {tr.metrics()}")
     torch.cuda.synchronize()
     targets = ["march_rays", "hashgrid_encode", "nerf_mlp_forward", "integrate_rays", "integrate_rays_backward",
-               "nerf_mlp_backward", "hashgrid_encode_backward", "adam_update"]
+               "nerf_mlp_backward", "hashgrid_encode_backward", "hashgrid_encode_backward_adam", "adam_update"]
 
     class Timed:
         def __init__(self, call, events):
@@ -453,6 +456,8 @@ def main() -> None:
     algo = {
         "hashgrid_encode": ("hbm", S * ENC_BYTES_PER_POINT),
         "hashgrid_encode_backward": ("hbm", S * ENC_BYTES_PER_POINT),
+        "hashgrid_encode_backward_adam": ("hbm", S * ENC_BYTES_PER_POINT +
+                                          tr.model.position_encoder.table.numel() * ADAM_BYTES_PER_PARAM),
         "march_rays": ("hbm", spp * MARCH_BYTES_PER_SAMPLE + n_rays * MARCH_BYTES_PER_RAY),
         "integrate_rays": ("hbm", S * INT_FWD_BYTES_PER_SAMPLE + n_rays * INT_FWD_BYTES_PER_RAY),
         "integrate_rays_backward": ("hbm", S * INT_BWD_BYTES_PER_SAMPLE + n_rays * INT_BWD_BYTES_PER_RAY),

@@ -628,6 +628,27 @@ typedef struct NgpAdamDescriptor {
  * with NGP_LOSS_SCALED, one more input after the mirror slot: NgpLossScaleState[1] */
 void adam_update(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
 
+typedef struct HashGridAdamDescriptor {
+    HashGridDescriptor grid; /* as hashgrid_encode_backward (F <= 2, f32 table) */
+    NgpAdamDescriptor adam;  /* as adam_update; n = n_rows * F, no loss scaling, no mirror */
+} HashGridAdamDescriptor;
+
+/* hashgrid_encode_backward followed by adam_update on the table, in one sequence of
+ * launches (no counterpart in the reference, whose optimizer step is XLA code): the
+ * table gradient's last pass takes each row's optimizer step in place of storing
+ * the gradient, so the gradient never goes through memory.  Same results as the two
+ * targets one after the other (both use one element update).  Needs the one-pass
+ * table gradient (hashgrid_encode_backward_adam_supported returns 1).
+ * in : positions f32[N,3], dL_dencodings f32[.., as hashgrid_encode_backward];
+ * in/out: params f32[n_rows, F], m f32[n_rows, F], v f32[n_rows, F];
+ * scratch: workspace u8[hashgrid_encode_backward_workspace_size(grid)] (256-byte
+ * aligned; params, m, v 16-byte aligned).  With N = 0 only the first five are read. */
+void hashgrid_encode_backward_adam(ngp_stream_t stream, void **buffers, const char *opaque, size_t opaque_len);
+
+/* 1 if hashgrid_encode_backward_adam takes this HashGridDescriptor (F <= 2, f32
+ * table, every level within the one-pass table gradient's slices), else 0. */
+int hashgrid_encode_backward_adam_supported(const char *opaque, size_t opaque_len);
+
 /* ---------------------------------------------------------------------------
  * Status.
  * ------------------------------------------------------------------------- */

@@ -56,6 +56,7 @@ TARGETS = (
     "proposal_sample",
     "proposal_render_samples",
     "proposal_train_samples",
+    "hashgrid_encode_backward_adam",
 )
 
 HASHGRID_MAX_LEVELS = 32
@@ -88,6 +89,8 @@ def load() -> ctypes.CDLL:
     lib.ngp_version.restype = ctypes.c_char_p
     lib.hashgrid_encode_backward_workspace_size.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     lib.hashgrid_encode_backward_workspace_size.restype = ctypes.c_size_t
+    lib.hashgrid_encode_backward_adam_supported.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    lib.hashgrid_encode_backward_adam_supported.restype = ctypes.c_int
     lib.sky_sphere_bg_workspace_size.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     lib.sky_sphere_bg_workspace_size.restype = ctypes.c_size_t
     lib.sky_sphere_bg_n_params.argtypes = [ctypes.c_uint32]
@@ -186,6 +189,21 @@ def hashgrid_backward_workspace_bytes(opaque: bytes) -> int:
         _check()
         raise NGPError("hashgrid_encode_backward_workspace_size: invalid descriptor")
     return n
+
+
+def hashgrid_backward_adam_supported(opaque: bytes) -> bool:
+    """Whether hashgrid_encode_backward_adam takes the HashGridDescriptor `opaque`
+    (F <= 2, f32 table, levels within the one-pass table gradient)."""
+    ok = bool(load().hashgrid_encode_backward_adam_supported(opaque, len(opaque)))
+    _check()
+    return ok
+
+
+def hashgrid_adam_descriptor(grid: bytes, adam: bytes) -> bytes:
+    """HashGridAdamDescriptor: a HashGridDescriptor, then an NgpAdamDescriptor (the
+    first is 424 bytes, a multiple of the second's 8-byte alignment)."""
+    assert len(grid) % 8 == 0
+    return grid + adam
 
 
 def sky_sphere_bg_workspace_bytes(opaque: bytes) -> int:

@@ -212,21 +212,12 @@ __global__ void __launch_bounds__(kBlock)
     adam_kernel(NgpAdamDescriptor d, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
                 float *__restrict__ v, TT *__restrict__ mirror, const NgpLossScaleState *__restrict__ scale_state) {
     const size_t stride = (size_t)gridDim.x * kBlock;
-    const float c1 = 1.0f - d.b1_pow_t, c2 = 1.0f - d.b2_pow_t;
-    float gs = d.grad_scale;
+    AdamStep a = make_adam_step(d);
     if (scale_state != nullptr) {
         if (scale_state->overflow != 0u) return;  // skipped step: nothing changes
-        gs = gs / scale_state->scale;
+        a.gs = a.gs / scale_state->scale;
     }
-    // one element's update, in optax's order of operations
-    auto step = [&](float pk, float gk_raw, float &mk, float &vk) {
-        const float gk = gk_raw * gs;
-        mk = d.b1 * mk + (1.0f - d.b1) * gk;
-        vk = d.b2 * vk + (1.0f - d.b2) * (gk * gk);
-        const float mh = mk / c1, vh = vk / c2;
-        const float u = -d.lr * (mh / (sqrtf(vh + d.eps_root) + d.eps)) + d.weight_decay * pk;
-        return pk + u;
-    };
+    auto step = [&](float pk, float gk_raw, float &mk, float &vk) { return adam_step(a, pk, gk_raw, mk, vk); };
     // four elements per thread and iteration with 16-byte accesses when every array
     // allows them (the table's), else one
     const bool vec = (d.n & 3u) == 0 && ((reinterpret_cast<uintptr_t>(p) | reinterpret_cast<uintptr_t>(g) |

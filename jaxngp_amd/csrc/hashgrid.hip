@@ -777,13 +777,16 @@ __global__ void __launch_bounds__(kReduceThreads)
 // reads kRunsPerWave runs side by side, two loads per lane and run in flight (a
 // hashed level's run holds about 128 entries; longer runs loop), so eight entry
 // loads per lane are in flight as in bwd_reduce_kernel.
-template <int F, bool kHalf>
+// With kAdam, a slice summed in one part is not stored: its rows take the optimizer
+// step in place (params, m, v), and bwd_combine does the same for the others.
+template <int F, bool kHalf, bool kAdam = false>
 __global__ void __launch_bounds__(kReduceThreads)
     bwd_reduce_runs_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ meta,
                            const uint32_t *__restrict__ parts, const uint32_t *__restrict__ part_slice,
                            const uint32_t *__restrict__ run_cnt, const uint32_t *__restrict__ run_off,
                            const uint32_t *__restrict__ entries, float *__restrict__ partials,
-                           float *__restrict__ dtable) {
+                           float *__restrict__ dtable, AdamStep adam = {}, float *__restrict__ adam_p = nullptr,
+                           float *__restrict__ adam_m = nullptr, float *__restrict__ adam_v = nullptr) {
     constexpr int EW = entry_words<F, kHalf>();
     constexpr uint32_t R = slice_rows<F>();
     constexpr uint32_t kWaves = kReduceThreads / kWave;
@@ -859,6 +862,18 @@ __global__ void __launch_bounds__(kReduceThreads)
             __syncthreads();  // the selection is rewritten by the next tile
         }
     }
+    if (kAdam && n_parts == 1) {
+        const size_t o = (size_t)s * R * F;
+        // unrolled so that several rows' params, m and v loads are in flight at once
+#pragma unroll 8
+        for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) {
+            float mk = adam_m[o + t], vk = adam_v[o + t];
+            adam_p[o + t] = adam_step(adam, adam_p[o + t], (float)acc[(t % F) * R + t / F], mk, vk);
+            adam_m[o + t] = mk;
+            adam_v[o + t] = vk;
+        }
+        return;
+    }
     float *out = n_parts == 1 ? dtable + (size_t)s * R * F : partials + (size_t)b * R * F;
     for (uint32_t t = threadIdx.x; t < rows * F; t += kReduceThreads) out[t] = (float)acc[(t % F) * R + t / F];
 }
@@ -866,10 +881,13 @@ __global__ void __launch_bounds__(kReduceThreads)
 // 5. slices of several parts: rows = sum of the parts' partial rows, in part
 // order.  Block (s, c) covers 1/kCombineSplit of slice s.
 constexpr uint32_t kCombineSplit = 16;
-template <int F>
+// With kAdam the sums are not stored: the rows take the optimizer step in place.
+template <int F, bool kAdam = false>
 __global__ void __launch_bounds__(256)
     bwd_combine_kernel(HashGridDescriptor d, BwdPlan p, const uint32_t *__restrict__ parts,
-                       const float *__restrict__ partials, float *__restrict__ dtable) {
+                       const float *__restrict__ partials, float *__restrict__ dtable, AdamStep adam = {},
+                       float *__restrict__ adam_p = nullptr, float *__restrict__ adam_m = nullptr,
+                       float *__restrict__ adam_v = nullptr) {
     const uint32_t s = blockIdx.x;
     const uint32_t p0 = parts[s], n_parts = parts[s + 1] - p0;
     if (n_parts <= 1) return;
@@ -878,7 +896,7 @@ __global__ void __launch_bounds__(256)
     const uint32_t lo = blockIdx.y * per, hi = min(n, lo + per);
     const size_t pstride = (size_t)p.R * F;
     const float *src = partials + (size_t)p0 * pstride;
-    float *out = dtable + (size_t)s * p.R * F;
+    const size_t o = (size_t)s * p.R * F;
     for (uint32_t t = lo + 4 * threadIdx.x; t < hi; t += 4 * 256) {
         if (t + 4 <= hi) {
             float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -889,20 +907,60 @@ __global__ void __launch_bounds__(256)
                 a.z += v.z;
                 a.w += v.w;
             }
-            *reinterpret_cast<float4 *>(out + t) = a;
+            if constexpr (kAdam) {
+                float4 p4 = *reinterpret_cast<const float4 *>(adam_p + o + t);
+                float4 m4 = *reinterpret_cast<const float4 *>(adam_m + o + t);
+                float4 v4 = *reinterpret_cast<const float4 *>(adam_v + o + t);
+                p4.x = adam_step(adam, p4.x, a.x, m4.x, v4.x);
+                p4.y = adam_step(adam, p4.y, a.y, m4.y, v4.y);
+                p4.z = adam_step(adam, p4.z, a.z, m4.z, v4.z);
+                p4.w = adam_step(adam, p4.w, a.w, m4.w, v4.w);
+                *reinterpret_cast<float4 *>(adam_m + o + t) = m4;
+                *reinterpret_cast<float4 *>(adam_v + o + t) = v4;
+                *reinterpret_cast<float4 *>(adam_p + o + t) = p4;
+            } else {
+                *reinterpret_cast<float4 *>(dtable + o + t) = a;
+            }
         } else {
             for (uint32_t u = t; u < hi; ++u) {
                 float a = 0.0f;
                 for (uint32_t q = 0; q < n_parts; ++q) a += src[q * pstride + u];
-                out[u] = a;
+                if constexpr (kAdam) {
+                    float mk = adam_m[o + u], vk = adam_v[o + u];
+                    adam_p[o + u] = adam_step(adam, adam_p[o + u], a, mk, vk);
+                    adam_m[o + u] = mk;
+                    adam_v[o + u] = vk;
+                } else {
+                    dtable[o + u] = a;
+                }
             }
         }
     }
 }
 
+// The optimizer step of rows no point reached (a zero gradient), for the fused
+// update with no points.
+__global__ void __launch_bounds__(256)
+    adam_zero_grad_kernel(uint64_t n, AdamStep adam, float *__restrict__ adam_p, float *__restrict__ adam_m,
+                          float *__restrict__ adam_v) {
+    for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (uint64_t)gridDim.x * 256) {
+        float mk = adam_m[k], vk = adam_v[k];
+        adam_p[k] = adam_step(adam, adam_p[k], 0.0f, mk, vk);
+        adam_m[k] = mk;
+        adam_v[k] = vk;
+    }
+}
+
+// `fused` (F <= 2, f32 table, one-pass plan only): the optimizer step in place of
+// the gradient's store; dtable is then not written.
+struct FusedAdam {
+    AdamStep step;
+    float *p, *m, *v;
+};
+
 template <int F, bool kHalf>
 bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const float *pos, const float *g, float *dtable,
-                     uint8_t *ws, const char *target) {
+                     uint8_t *ws, const char *target, const FusedAdam *fused = nullptr) {
     const BwdPlan p = make_plan(d);
     uint32_t *meta = reinterpret_cast<uint32_t *>(ws + p.meta_off);
     uint32_t *parts = reinterpret_cast<uint32_t *>(ws + p.parts_off);
@@ -926,12 +984,34 @@ bool launch_backward(const HashGridDescriptor &d, hipStream_t stream, const floa
             hipLaunchKernelGGL(bwd_slice_parts_kernel, dim3(p.n_slices), dim3(256), 0, stream, d, p, run_cnt, slice_np);
             hipLaunchKernelGGL(bwd_bucket_plan_kernel, dim3(1), dim3(kScanThreads), 0, stream, p, slice_np, meta, parts,
                                part_slice);
+            if constexpr (!kHalf) {
+                if (fused != nullptr) {
+                    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&bwd_reduce_runs_kernel<F, false, true>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                    if (e != hipSuccess) {
+                        set_error(kLaunchFailed, "%s: cannot reserve %zu bytes of LDS: %s", target, lds,
+                                  hipGetErrorString(e));
+                        return false;
+                    }
+                    hipLaunchKernelGGL((bwd_reduce_runs_kernel<F, false, true>), dim3(p.max_parts),
+                                       dim3(kReduceThreads), lds, stream, d, p, meta, parts, part_slice, run_cnt,
+                                       run_off, entries, partials, dtable, fused->step, fused->p, fused->m, fused->v);
+                    hipLaunchKernelGGL((bwd_combine_kernel<F, true>), dim3(p.n_slices, kCombineSplit), dim3(256), 0,
+                                       stream, d, p, parts, partials, dtable, fused->step, fused->p, fused->m,
+                                       fused->v);
+                    return true;
+                }
+            }
             hipLaunchKernelGGL((bwd_reduce_runs_kernel<F, kHalf>), dim3(p.max_parts), dim3(kReduceThreads), lds, stream,
                                d, p, meta, parts, part_slice, run_cnt, run_off, entries, partials, dtable);
             hipLaunchKernelGGL(bwd_combine_kernel<F>, dim3(p.n_slices, kCombineSplit), dim3(256), 0, stream, d, p,
                                parts, partials, dtable);
             return true;
         }
+    }
+    if (fused != nullptr) {
+        set_error(kBadDescriptor, "%s: the fused update needs the one-pass table gradient", target);
+        return false;
     }
     uint32_t *counts = reinterpret_cast<uint32_t *>(ws + p.counts_off);
     uint32_t *sums = reinterpret_cast<uint32_t *>(ws + p.sums_off);
@@ -1035,6 +1115,63 @@ extern "C" void hashgrid_encode(ngp_stream_t stream_, void **buffers, const char
     float *enc = (float *)buffers[2];
     if (d.table_dtype == NGP_DTYPE_F32) launch_fwd<float>(d, stream, pos, buffers[1], enc);
     else launch_fwd<__half>(d, stream, pos, buffers[1], enc);
+    check_launch(target);
+}
+
+extern "C" int hashgrid_encode_backward_adam_supported(const char *opaque, size_t opaque_len) {
+    const char *target = "hashgrid_encode_backward_adam_supported";
+    HashGridDescriptor d;
+    if (!read_descriptor(opaque, opaque_len, &d, target)) return 0;
+    if (!validate(d, target)) return 0;
+    return d.F <= 2 && d.table_dtype == NGP_DTYPE_F32 && make_plan(d).bucket ? 1 : 0;
+}
+
+extern "C" void hashgrid_encode_backward_adam(ngp_stream_t stream_, void **buffers, const char *opaque,
+                                              size_t opaque_len) {
+    const char *target = "hashgrid_encode_backward_adam";
+    static_assert(offsetof(HashGridAdamDescriptor, adam) == sizeof(HashGridDescriptor), "no padding");
+    HashGridAdamDescriptor q;
+    if (!read_descriptor(opaque, opaque_len, &q, target)) return;
+    const HashGridDescriptor &d = q.grid;
+    if (!validate(d, target)) return;
+    if (d.F > 2 || d.table_dtype != NGP_DTYPE_F32 || q.adam.flags != 0u || q.adam.mirror_dtype != NGP_DTYPE_F32 ||
+        q.adam.n != (uint64_t)d.n_rows * d.F) {
+        set_error(kBadDescriptor,
+                  "%s: needs F <= 2, an f32 table, no loss scaling or f16 mirror, and adam.n = n_rows * F", target);
+        return;
+    }
+    const BwdPlan plan = make_plan(d);
+    if (!plan.bucket || plan.n_slices > kMaxSlices) {
+        set_error(kBadDescriptor, "%s: the table's levels must fit the one-pass table gradient", target);
+        return;
+    }
+    if ((uint64_t)d.n_points * d.L * 8 >= 0xffffffffull) {
+        set_error(kBadDescriptor, "%s: n_points * L * 8 must stay below 2^32", target);
+        return;
+    }
+    if (!check_buffer_list(buffers, {2, 3, 4}, target)) return;
+    hipStream_t stream = (hipStream_t)stream_;
+    FusedAdam f{make_adam_step(q.adam), (float *)buffers[2], (float *)buffers[3], (float *)buffers[4]};
+    if (d.n_points == 0) {
+        const uint64_t n = q.adam.n;
+        const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 16384));
+        hipLaunchKernelGGL(adam_zero_grad_kernel, dim3(grid), dim3(256), 0, stream, n, f.step, f.p, f.m, f.v);
+        check_launch(target);
+        return;
+    }
+    if (!check_buffers(buffers, 6, target)) return;
+    uint8_t *ws = (uint8_t *)buffers[5];
+    if ((reinterpret_cast<uintptr_t>(ws) & 255u) != 0 ||
+        ((reinterpret_cast<uintptr_t>(f.p) | reinterpret_cast<uintptr_t>(f.m) | reinterpret_cast<uintptr_t>(f.v)) &
+         15u) != 0) {
+        set_error(kBadArgument, "%s: params, m and v must be 16-byte and the workspace 256-byte aligned", target);
+        return;
+    }
+    const float *pos = (const float *)buffers[0];
+    const float *g = (const float *)buffers[1];
+    const bool ok = d.F == 1 ? launch_backward<1, false>(d, stream, pos, g, nullptr, ws, target, &f)
+                             : launch_backward<2, false>(d, stream, pos, g, nullptr, ws, target, &f);
+    if (!ok) return;
     check_launch(target);
 }
 

@@ -82,6 +82,28 @@ inline uint32_t div_up(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) /
 // device math
 // ---------------------------------------------------------------------------
 
+// One element of optax.adam(lr, b1, b2, eps, eps_root) followed by
+// add_decayed_weights(weight_decay), in optax's order of operations
+// (app/nerf/_utils.py:19-77); gradients are multiplied by gs first.  Shared by
+// adam_update and the table gradient's fused update, so both round alike.
+struct AdamStep {
+    float lr, b1, b2, eps, eps_root, c1, c2, weight_decay, gs;
+};
+
+__host__ __device__ inline AdamStep make_adam_step(const NgpAdamDescriptor &d) {
+    return AdamStep{d.lr, d.b1, d.b2, d.eps, d.eps_root, 1.0f - d.b1_pow_t, 1.0f - d.b2_pow_t, d.weight_decay,
+                    d.grad_scale};
+}
+
+__device__ __forceinline__ float adam_step(const AdamStep &a, float pk, float gk_raw, float &mk, float &vk) {
+    const float gk = gk_raw * a.gs;
+    mk = a.b1 * mk + (1.0f - a.b1) * gk;
+    vk = a.b2 * vk + (1.0f - a.b2) * (gk * gk);
+    const float mh = mk / a.c1, vh = vk / a.c2;
+    const float u = -a.lr * (mh / (sqrtf(vh + a.eps_root) + a.eps)) + a.weight_decay * pk;
+    return pk + u;
+}
+
 // x / d for a divisor d shared by a whole launch: a power of two in the normal range
 // takes the multiply by its exact reciprocal (x / 2^k and x * 2^-k are the same exact
 // scaling, rounded alike, also below the normal range), any other d the division.

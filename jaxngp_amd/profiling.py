@@ -36,14 +36,18 @@ TARGET_KERNELS = {
                                   "bwd_reduce_runs_kernel", "bwd_count_kernel", "scan_sums_kernel",
                                   "scan_apply_kernel", "plan_parts_kernel", "bwd_scatter_kernel", "bwd_reduce_kernel",
                                   "bwd_combine_kernel", "zero_table_kernel"], "bwd_combine_kernel"),
+    # the same kernels with the table's optimizer step in the reduce and combine passes
+    "hashgrid_encode_backward_adam": (["bwd_bucket_kernel", "bwd_slice_parts_kernel", "bwd_bucket_plan_kernel",
+                                       "bwd_reduce_runs_kernel", "bwd_combine_kernel", "adam_zero_grad_kernel"],
+                                      "bwd_combine_kernel"),
     "nerf_mlp_forward": (["mlp_forward_kernel", "mlp_forward_h_kernel"], None),
     "nerf_mlp_backward": (["mlp_backward_kernel", "mlp_backward_h_kernel", "sum_rows_kernel"], "sum_rows_kernel"),
     "integrate_rays": (["integrate_kernel"], "integrate_kernel"),
     "integrate_rays_backward": (["integrate_backward_kernel"], "integrate_backward_kernel"),
     "occupancy_grid_update": (["ogrid_update_kernel", "ogrid_decay_kernel", "ogrid_prefix_kernel"],
                               "ogrid_update_kernel"),
-    "proposal_train_samples": (["proposal_train_kernel", "proposal_train_scan_kernel", "proposal_train_tail_kernel"],
-                               "proposal_train_scan_kernel"),
+    "proposal_train_samples": (["proposal_train_kernel", "proposal_train_scan_kernel", "proposal_train_write_kernel",
+                                "proposal_train_tail_kernel"], "proposal_train_scan_kernel"),
 }
 
 

@@ -95,6 +95,13 @@ class TrainConfig:
     # (no grid update after it; in guided mode, no proposal update in it).  The
     # samples are the same either way (False: made at the start of each step)
     prefetch_march: bool = True
+    # FP32: the table's optimizer step taken inside the table gradient's last pass
+    # (hashgrid_encode_backward_adam; same results, the gradient never goes through
+    # memory) when the encoder allows it.  Off by default: measured, the step was no
+    # faster (0.784 against 0.780 ms): the separate update streams its 341 MB at
+    # about 6 TB/s, while in the reducer's tail (one workgroup per CU, held by its
+    # 128 KiB of LDS) the same rows move at about 5.5 TB/s, which eats the 98 MB saved
+    fused_table_update: bool = False
 
 
 def lr_at(step: int, lr: float) -> float:
@@ -443,6 +450,8 @@ class Trainer:
         else:
             mlp_w, pd, flags, state = params, _lib.DTYPE_F32, 0, []
         none = torch.empty(0, device=self.dev)  # a NULL mirror slot
+        fused = (c.fused_table_update and mirror is None and not self.fp16 and
+                 _lib.hashgrid_backward_adam_supported(hdesc))
         side = self.side_stream.cuda_stream  # the optimizer's stream (see step)
         march_out = [b["next_loc"], b["exceeded"], b["valid"], b["ns"], b["start"], b["idcs"], b["xyzs"], b["dirs"],
                      b["dss"], b["zs"]]
@@ -477,7 +486,10 @@ class Trainer:
                             _lib.integrating_backward_descriptor(n, S, c.near)),
             mlp_bwd=P("nerf_mlp_backward", [b["enc"], b["dirs"], mlp_w, b["ddrgbs"], b["denc"], b["dparams"],
                                             b["ws"]], _lib.nerf_mlp_descriptor(S, b["ws"].shape[0], LM, pd)),
-            encode_bwd=P("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"], b["hg_ws"]], hdesc),
+            encode_bwd=(P("hashgrid_encode_backward_adam", [b["xyzs"], b["denc"], table, o["tm"], o["tv"],
+                                                            b["hg_ws"]], hdesc) if fused else
+                        P("hashgrid_encode_backward", [b["xyzs"], b["denc"], b["dtable"], b["hg_ws"]], hdesc)),
+            fused_table=fused,
             adam_table=P("adam_update", [tab_master, b["dtable"], o["tm"], o["tv"]] +
                          [mirror if mirror is not None else none] + state, b"", stream=side),
             adam_mlp=P("adam_update", [params, b["dparams"], o["mm"], o["mv"]] +
@@ -582,6 +594,11 @@ class Trainer:
             # their gradient after the integrator's (app/nerf/_utils.py:128-150)
             order = ("bg_fwd", "bg_blend") + order[:5] + ("bg_grad", "bg_bwd") + order[5:]
         train_prop = self.prop is not None and s0 % c.proposal_every == 0
+        t = s0 + 1  # the optimizer's step count after this step
+        lr = lr_at(t - 1, c.lr)
+        b1, b2 = 0.9, 0.99
+        bp1, bp2 = float(np.float32(b1) ** t), float(np.float32(b2) ** t)
+        n_table = k["adam_table"].keep[0].numel()
         for name in order:
             if name == self._pf_before and train_prop:
                 # the proposal's update, on this step's samples and the field's weights at
@@ -600,12 +617,13 @@ class Trainer:
                     self._rays_and_samples(self._calls[nxt], s0 + 1, pf.cuda_stream)
                 self._pf_ready[nxt].record(pf)
                 self._pf_pending[nxt] = (bits, s0 + 1)
-            k[name]()
+            if name == "encode_bwd" and k["fused_table"]:
+                # the table's optimizer step inside its gradient's last pass
+                k[name](_lib.hashgrid_adam_descriptor(
+                    k[name].opaque, _lib.adam_descriptor(n_table, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0)))
+            else:
+                k[name]()
         self.step_count += 1
-        t = self.step_count
-        lr = lr_at(t - 1, c.lr)
-        b1, b2 = 0.9, 0.99
-        bp1, bp2 = float(np.float32(b1) ** t), float(np.float32(b2) ** t)
         # the update on the side stream, once the gradients exist; the next step's
         # rays and march run meanwhile
         self._bwd_done.record(self.main_stream)
@@ -616,10 +634,10 @@ class Trainer:
                 k["check_mlp"]()
                 if bg:
                     k["check_bg"]()
-            n_table = k["adam_table"].keep[0].numel()
-            k["adam_table"](_lib.adam_descriptor(n_table, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
-                                                 mirror_dtype=_lib.DTYPE_F16 if k["mirror"] else _lib.DTYPE_F32,
-                                                 flags=k["flags"]))
+            if not k["fused_table"]:
+                k["adam_table"](_lib.adam_descriptor(n_table, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 0.0,
+                                                     mirror_dtype=_lib.DTYPE_F16 if k["mirror"] else _lib.DTYPE_F32,
+                                                     flags=k["flags"]))
             k["adam_mlp"](_lib.adam_descriptor(N_PARAMS, lr, b1, b2, 1e-15, 1e-15, bp1, bp2, 1e-6,
                                                mirror_dtype=_lib.DTYPE_F16 if self.fp16 else _lib.DTYPE_F32,
                                                flags=k["flags"]))

@@ -98,6 +98,7 @@ _Static_assert(sizeof(ProposalSampleDescriptor) == 44, "");
 _Static_assert(sizeof(ProposalRenderDescriptor) == sizeof(HashGridDescriptor) + 48, "");
 _Static_assert(sizeof(NerfFieldChunksDescriptor) == sizeof(HashGridDescriptor) + 16, "");
 _Static_assert(sizeof(TcnnHashGridDescriptor) == 20, "");
+_Static_assert(sizeof(HashGridAdamDescriptor) == sizeof(HashGridDescriptor) + 56, "");
 _Static_assert(sizeof(SkySphereBgDescriptor) == 16, "");
 int main(void) { return 0; }
 '''
@@ -207,4 +208,26 @@ def test_error_is_seen_from_another_thread():
     _lib.target("integrate_rays")(_lib.ctypes.c_void_p(0), arr, b"\x00", 1)
     code, msg = _lib.last_error()
     assert "total_samples" in msg
+    assert _lib.last_error() == (0, "")
+
+
+def test_fused_table_update_host_checks():
+    """hashgrid_encode_backward_adam: which descriptors it takes (F <= 2, f32 table,
+    the one-pass table gradient), and that the others are errors before any launch."""
+    from jaxngp_amd.encoders import level_table
+    lv = level_table(16, 2 ** 19, 16, 2048)
+    desc = lambda F=2, dt=0, T=2 ** 19, n=1000: _lib.hashgrid_descriptor(
+        n, 16, F, T, lv.first_hash_level, dt, lv.n_rows, 1.0, list(lv.scales), lv.resolutions, lv.offsets)
+    assert _lib.hashgrid_backward_adam_supported(desc())
+    assert not _lib.hashgrid_backward_adam_supported(desc(dt=1))  # f16 table: the loss-scaled path
+    assert not _lib.hashgrid_backward_adam_supported(desc(F=4))
+    adam = lambda n, flags=0: _lib.adam_descriptor(n, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9, 0.99, 0.0, flags=flags)
+    n = lv.n_rows * 2
+    with pytest.raises(_lib.NGPError, match="adam.n"):
+        _lib.call_raw("hashgrid_encode_backward_adam", [0] * 6, _lib.hashgrid_adam_descriptor(desc(), adam(n - 1)))
+    with pytest.raises(_lib.NGPError, match="loss scaling"):
+        _lib.call_raw("hashgrid_encode_backward_adam", [0] * 6,
+                      _lib.hashgrid_adam_descriptor(desc(), adam(n, _lib.LOSS_SCALED)))
+    with pytest.raises(_lib.NGPError, match="NULL"):
+        _lib.call_raw("hashgrid_encode_backward_adam", [0] * 6, _lib.hashgrid_adam_descriptor(desc(), adam(n)))
     assert _lib.last_error() == (0, "")

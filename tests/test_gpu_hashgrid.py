@@ -269,3 +269,45 @@ def test_fp16_table_2pow22(dev):
     absw = _c_backward(enc, pos, np.abs(g))
     assert (dt[absw == 0] == 0).all()
     assert (np.abs(dt - want) <= 1e-3 * absw + 1e-6).all()
+
+
+def test_fused_table_update_matches_gradient_then_adam(dev):
+    """hashgrid_encode_backward_adam (the optimizer step taken in the table gradient's
+    last pass) against hashgrid_encode_backward followed by adam_update on the same
+    state, over three steps: 2^16 points (coarse slices summed in several parts, so
+    the combine pass updates them), 20,001 points, and no points (every row takes a
+    zero-gradient step).  Both use one element update; the gradient's f64 sums are
+    taken in LDS-atomic order, so a row's f32 gradient may differ by its last bit
+    between two runs, hence the tolerance; with no points the results are equal."""
+    from jaxngp_amd import _lib
+    rng = np.random.default_rng(31)
+    enc = HashGridEncoder(**CONFIGS["l16_t2e19"], generator=torch.Generator().manual_seed(5)).to(dev)
+    L, F = 16, 2
+    with torch.no_grad():
+        enc.table.uniform_(-1e-2, 1e-2)
+    p1 = enc.table.detach().clone()
+    m1 = torch.from_numpy(rng.normal(0, 1e-3, tuple(p1.shape)).astype(np.float32)).to(dev)
+    v1 = torch.from_numpy(rng.uniform(0, 1e-5, tuple(p1.shape)).astype(np.float32)).to(dev)
+    p2, m2, v2 = p1.clone(), m1.clone(), v1.clone()
+    n = p1.numel()
+    for t, N in ((1, 1 << 16), (2, 20001), (3, 0)):
+        print(f"fused table update: step {t}, {N} points", flush=True)
+        pos = torch.from_numpy(rng.uniform(-1, 1, (max(N, 1), 3)).astype(np.float32)).to(dev)[:N]
+        g = torch.from_numpy(rng.normal(0, 1e-3, (L, max(N, 1), F)).astype(np.float32)).to(dev)[:, :N].contiguous()
+        desc = enc.descriptor(N, 1.0, _lib.ENC_LEVEL_MAJOR)
+        assert _lib.hashgrid_backward_adam_supported(desc)
+        ws = torch.empty(_lib.hashgrid_backward_workspace_bytes(desc) if N else 256, dtype=torch.uint8, device=dev)
+        adam = _lib.adam_descriptor(n, 1e-2, 0.9, 0.99, 1e-15, 1e-15, 0.9 ** t, 0.99 ** t, 0.0)
+        d = torch.empty_like(p1)
+        _lib.call("hashgrid_encode_backward", [pos, g, d, ws], desc)
+        _lib.call("adam_update", [p1, d, m1, v1, torch.empty(0, device=dev)], adam)
+        _lib.call("hashgrid_encode_backward_adam", [pos, g, p2, m2, v2, ws], _lib.hashgrid_adam_descriptor(desc, adam))
+        torch.cuda.synchronize()
+        for a, b, name in ((p2, p1, "params"), (m2, m1, "m"), (v2, v1, "v")):
+            if N == 0:
+                assert torch.equal(a, b), name
+            else:
+                torch.testing.assert_close(a, b, rtol=1e-5, atol=1e-9, msg=name)
+        same = (p2 == p1).float().mean().item()
+        print(f"  params equal bit for bit: {same:.6f}", flush=True)
+        assert same > 0.999
