@@ -543,7 +543,8 @@ __global__ void __launch_bounds__(kBlock)
     zs[j] = stage[at];
 }
 
-// Eight consecutive rays per thread, so 8,192 rays take one block scan.
+// Rays in chunks of 8 x 1024: every load of a chunk is issued up front (coalesced: ray
+// c + 1024 q + thread), then eight block scans in ray order place them.
 constexpr int kTrainScanThreads = 1024, kTrainScanPer = 8;
 __global__ void __launch_bounds__(kTrainScanThreads)
     proposal_train_scan_kernel(uint32_t n_rays, uint32_t total_samples, const float *__restrict__ t0s,
@@ -552,32 +553,32 @@ __global__ void __launch_bounds__(kTrainScanThreads)
                                uint32_t *__restrict__ rays_n_samples, uint32_t *__restrict__ startidx) {
     uint64_t carry = 0, over = 0;
     for (uint64_t c = 0; c < n_rays; c += (uint64_t)kTrainScanThreads * kTrainScanPer) {
-        const uint64_t r0 = c + (uint64_t)threadIdx.x * kTrainScanPer;
-        uint32_t n[kTrainScanPer], sum = 0;
+        uint32_t n[kTrainScanPer];
+        bool seg[kTrainScanPer];
 #pragma unroll
         for (int q = 0; q < kTrainScanPer; ++q) {
-            n[q] = r0 + q < n_rays ? rays_n_samples[r0 + q] : 0u;
-            sum += n[q];
+            const uint64_t r = c + (uint64_t)q * kTrainScanThreads + threadIdx.x;
+            n[q] = r < n_rays ? rays_n_samples[r] : 0u;
+            seg[q] = r < n_rays && t1s[r] > t0s[r];
         }
-        uint32_t tot;
-        const uint32_t ex = block_exclusive_scan<kTrainScanThreads>(sum, &tot);
-        uint64_t start = carry + ex;
         uint32_t miss = 0;
 #pragma unroll
         for (int q = 0; q < kTrainScanPer; ++q) {
-            const uint64_t r = r0 + q;
+            const uint64_t r = c + (uint64_t)q * kTrainScanThreads + threadIdx.x;
+            uint32_t tot;
+            const uint32_t ex = block_exclusive_scan<kTrainScanThreads>(n[q], &tot);
+            const uint64_t start = carry + ex;
+            const bool ok = seg[q] && start < total_samples && start + n[q] <= total_samples;
             if (r < n_rays) {
-                const bool ok = t1s[r] > t0s[r] && start < total_samples && start + n[q] <= total_samples;
                 valid[r] = ok;
                 rays_n_samples[r] = ok ? n[q] : 0u;
                 startidx[r] = ok && n[q] > 0 ? (uint32_t)start : 0u;
                 miss += ok ? 0u : n[q];
             }
-            start += n[q];
+            carry += tot;
         }
         uint32_t miss_tot;
         block_exclusive_scan<kTrainScanThreads>(miss, &miss_tot);
-        carry += tot;
         over += miss_tot;
     }
     if (threadIdx.x == 0) {

@@ -136,3 +136,34 @@ def test_proposal_training_and_render():
         print(f"view {i}: plain {p_plain:.2f} dB, proposal {p_prop:.2f} dB {st}, at mass 0.9 {p_prop09:.2f} dB {st09}")
         assert st["proposal_rays"] > 0 and st09["proposal_rays"] >= st["proposal_rays"]
         assert abs(p_prop - p_plain) < 0.1 and abs(p_prop09 - p_plain) < 0.1
+
+
+def test_guided_proposal_render_at_its_settings():
+    """A proposal trained on the guided samples renders the field it guided with its
+    own settings (Trainer.proposal_render_settings: 32 support-placed samples per ray,
+    occupancy-grid fallback below predicted mass 0.5) close to the occupancy-grid
+    render of the same field.  A small-scene sanity bound (0.3 dB per view); the
+    full-scale comparison over 20 views is the bench's."""
+    from jaxngp_amd.renderer import render_image_proposal
+    dev = torch.device("cuda:0")
+    cam = sc.Camera(width=96, height=96, fx=133.0, fy=133.0, cx=48.0, cy=48.0)
+    blobs = sc.BlobScene.random(seed=3)
+    data = sc.make_views(blobs, cam, sc.look_at_poses(16, seed=1), dev, n_steps=256)
+    tr = Trainer(data, TrainConfig(total_samples=1 << 16, proposal=True, proposal_guided=True, precision="fp16"))
+    for _ in range(600):
+        tr.step()
+    settings = tr.proposal_render_settings
+    assert settings["placement"] == _lib.PROPOSAL_SUPPORT
+    kw = dict(K=tr.K, G=128, diagonal_n_steps=1024, stepsize_portion=tr.portion, march_steps_cap=8)
+    poses = sc.look_at_poses(2, seed=5, golden=False)
+    gt = sc.make_views(blobs, cam, poses, dev, n_steps=256)
+    for i in range(2):
+        pose = torch.as_tensor(poses[i], dtype=torch.float32, device=dev)
+        _, plain, _ = render_image_inference(tr.model, cam, pose, tr.grid.bitfield, **kw)
+        st = {}
+        _, prop, _ = render_image_proposal(tr.model, tr.prop, cam, pose, tr.grid.bitfield, stats=st, **settings, **kw)
+        ref = sc.blend_u8(gt.rgba[i], (1.0, 1.0, 1.0))
+        p_plain, p_prop = sc.psnr_u8(plain, ref), sc.psnr_u8(prop, ref)
+        print(f"view {i}: plain {p_plain:.2f} dB, guided proposal {p_prop:.2f} dB {st}", flush=True)
+        assert st["proposal_rays"] > 0
+        assert p_prop > p_plain - 0.3
