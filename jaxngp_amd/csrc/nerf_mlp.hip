@@ -617,7 +617,7 @@ __device__ __forceinline__ void forward_from_x_h(const _Float16 *lds, float dx, 
 }
 
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
     mlp_forward_h_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const float *__restrict__ dirs,
                          const _Float16 *__restrict__ params, float *__restrict__ drgbs) {
     __shared__ __align__(16) _Float16 lds[H_FEND];
@@ -1300,6 +1300,7 @@ extern "C" void nerf_field_chunks(ngp_stream_t stream, void **buffers, const cha
     }
     const uint64_t tiles = ((uint64_t)d.n_slots * d.cap + kTile - 1) / kTile;
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((tiles + kWaves - 1) / kWaves, 2048));
+    const bool t16 = g.table_dtype == NGP_DTYPE_F16, w16 = d.param_dtype == NGP_DTYPE_F16;
     hipStream_t s = (hipStream_t)stream;
     const float *xyzs = (const float *)buffers[0];
     const uint32_t *ns = (const uint32_t *)buffers[1];
@@ -1307,7 +1308,6 @@ extern "C" void nerf_field_chunks(ngp_stream_t stream, void **buffers, const cha
     const float *rd = (const float *)buffers[3];
     const void *params = buffers[5];
     float *out = (float *)buffers[6];
-    const bool t16 = g.table_dtype == NGP_DTYPE_F16, w16 = d.param_dtype == NGP_DTYPE_F16;
     if (t16 && w16)
         hipLaunchKernelGGL((field_chunks_kernel<_Float16, true>), dim3(grid), dim3(kThreads), 0, s, g, d.n_slots, d.cap,
                            d.n_total_rays, xyzs, ns, idx, rd, (const _Float16 *)buffers[4], params, out);
