@@ -82,6 +82,19 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // One workspace row of kParams per workgroup; waves add in a fixed order.
+// One halving step over N values per lane: lanes with bit d set keep values
+// [N/2, N), the others [0, N/2), each adding its partner's (lane ^ d) copy of them.
+template <int N>
+__device__ __forceinline__ void halve_lanes(float *v, int lane, int d) {
+    const bool hi = (lane & d) != 0;
+#pragma unroll
+    for (int k = 0; k < N / 2; ++k) {
+        const float keep = hi ? v[k + N / 2] : v[k];
+        const float give = hi ? v[k] : v[k + N / 2];
+        v[k] = keep + __shfl_xor(give, d, kWave);
+    }
+}
+
 template <typename PT>
 __global__ void __launch_bounds__(kBlock)
     proposal_bwd_kernel(uint32_t n, bool level_major, const float *__restrict__ enc, const PT *__restrict__ params,
@@ -118,11 +131,26 @@ __global__ void __launch_bounds__(kBlock)
             dL_denc[at + 1] = dx[2 * l + 1];
         }
     }
-    const int wave = (int)threadIdx.x / kWave;
+    // the wavefront's sums of the 176 gradients by recursive halving: at distance d
+    // (32, 16, 8, 4) a lane keeps the half of its remaining parameters that its lane
+    // bit d picks and adds its partner's partial sums of that half (one exchange per
+    // kept value instead of six per parameter); the 11 left per lane are summed over
+    // the lanes that differ in bits 0-1, and lane 4m holds parameters [off, off + 11)
+    static_assert(kParams == 176, "halving 176 -> 11");
+    const int wave = (int)threadIdx.x / kWave, lane = lane_id();
+    halve_lanes<176>(gW, lane, 32);
+    halve_lanes<88>(gW, lane, 16);
+    halve_lanes<44>(gW, lane, 8);
+    halve_lanes<22>(gW, lane, 4);
 #pragma unroll
-    for (int k = 0; k < kParams; ++k) {
-        const float s = wave_sumf(gW[k]);
-        if (lane_id() == 0) part[wave][k] = s;
+    for (int k = 0; k < 11; ++k) {
+        gW[k] += __shfl_xor(gW[k], 2, kWave);
+        gW[k] += __shfl_xor(gW[k], 1, kWave);
+    }
+    if ((lane & 3) == 0) {
+        const int off = ((lane >> 5) & 1) * 88 + ((lane >> 4) & 1) * 44 + ((lane >> 3) & 1) * 22 + ((lane >> 2) & 1) * 11;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) part[wave][off + k] = gW[k];
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kParams; k += kBlock) {
