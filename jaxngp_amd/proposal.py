@@ -9,7 +9,8 @@ mass is below the renderer's fallback threshold (``render_image_proposal``'s
 ``mass_threshold``, 0.98 by default) are rendered by the occupancy-grid marcher.
 With ``TrainConfig(proposal_guided=True)`` the field itself trains on samples the
 proposal places (``proposal_train_samples``), and the proposal on the field's
-weights at them.
+weights at them; such a proposal renders with 32 samples spread over the support of
+its predicted mass and falls back below 0.5 (``Trainer.proposal_render_settings``).
 """
 from __future__ import annotations
 
