@@ -210,7 +210,7 @@ def render_image_proposal(model, proposal, camera: Camera, pose: torch.Tensor, o
                           bg=(1.0, 1.0, 1.0), *, n_per_ray: int = 48, mass_threshold: float = 0.98,
                           placement: int = _lib.PROPOSAL_QUANTILES, K=1, G=128,
                           diagonal_n_steps=1024, stepsize_portion=0.0, march_steps_cap=8, pool: int = 1 << 18,
-                          fallback_steps_cap: int = 32, fallback_pool: int = 1 << 16,
+                          fallback_steps_cap: int = 64, fallback_pool: int = 1 << 16,
                           empty_mass: float = 0.0, stats: dict | None = None):
     """Renders one frame with samples placed by the learned proposal: its density at
     64 bins of every ray gives 48 samples, by default at the quantiles of the ray's

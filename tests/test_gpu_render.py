@@ -86,12 +86,13 @@ def test_render_pool_invariance_and_quality():
         _, a, da = render_image_inference(tr.model, cam, pose, tr.grid.bitfield, pool=300, **kw)
         _, b, db = render_image_inference(tr.model, cam, pose, tr.grid.bitfield, pool=1 << 14, **kw)
         assert torch.equal(a, b) and torch.equal(da, db)
-        # nor on the samples marched per round (the proposal renderer's fallback uses 32)
+        # nor on the samples marched per round (the proposal renderer's fallback uses 64)
         o, d = make_rays_worldspace(cam, pose)
         r8, _ = render_rays_inference(tr.model, o, d, tr.grid.bitfield, pool=300, **kw)
         assert torch.equal(f32_to_u8(r8[:, :3]).reshape(a.shape), a)
-        r32, _ = render_rays_inference(tr.model, o, d, tr.grid.bitfield, pool=1 << 12, **dict(kw, march_steps_cap=32))
-        _assert_cap_differences_are_far_plane_ties(tr, o, d, r8, 8, r32, 32)
+        for cap in (32, 64):
+            rc, _ = render_rays_inference(tr.model, o, d, tr.grid.bitfield, pool=1 << 12, **dict(kw, march_steps_cap=cap))
+            _assert_cap_differences_are_far_plane_ties(tr, o, d, r8, 8, rc, cap)
         # the fused, pre-bound renderer equals the op-by-op loop with the reference's pool
         c = _render_with_ops(tr.model, cam, pose, tr.grid.bitfield, 8192, march_steps_cap=8, K=tr.K, G=128,
                              diagonal_n_steps=1024, stepsize_portion=tr.portion)
